@@ -1,0 +1,84 @@
+"""The reference's analytic acceptance tests, restated as functions.
+
+Scripts/tests/fast_flavor_test.py, fast_flavor_k_test.py and msw_test.py read
+plotfiles written every step and compare growth rates / survival
+probabilities with closed forms.  Here each test receives the per-step
+snapshots directly (time and the particle attributes it inspects), so the same
+checks run on the oracle and on the HIP paths.
+"""
+import json
+import math
+import os
+
+import numpy as np
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+# Scripts/tests/msw_test.py:16-22 and amrex_plot_tools.py:5-10
+clight = 2.99792458e10
+hbar = 1.05457266e-27
+eV = 1.60218e-12
+mp = 1.6726219e-24
+GF = 1.1663787e-5 / (1e9 * eV) ** 2 * (hbar * clight) ** 3
+
+
+def reference_deck(name: str) -> dict:
+    with open(os.path.join(GOLDEN, "reference_decks.json")) as f:
+        return dict(json.load(f)["decks"][name])
+
+
+def fast_flavor_errors(t, fR, fI, fRb, fIb, i0=50, i1=70):
+    """fast_flavor_test.py:15-90: growth of f01 of particle 0 vs Im(Omega)=V/hbar,
+    V = dm21 c^4 / (2E) (Chakraborty 2016 eq. 2.7), tolerance 2e-2."""
+    dm21c4 = 7.39e-5 * eV ** 2
+    E = 50. * 1e6 * eV
+    V = dm21c4 / (2. * E)
+    ImOmega = V / hbar
+    dt = t[i1] - t[i0]
+    out = []
+    for a in (fR, fI, fRb, fIb):
+        om = np.log(a[i1] / a[i0]) / dt
+        out.append(abs(ImOmega - om) / ImOmega)
+    return out
+
+
+def fast_flavor_k_errors(t, fR, fI, fRb, fIb, Lx=1.0, i0=100, i1=160):
+    """fast_flavor_k_test.py:16-94: growth of max|f01| over particles vs
+    Im(Omega) = (V + k hbar c)/hbar with V = 0 and k = 2 pi / Lx, tolerance 1e-2.
+    (The script reads Lx for a perturbation along z; Lx = Lz = 1 in the deck.)"""
+    k = 2 * np.pi / Lx
+    ImOmega = (0.0 + k * hbar * clight) / hbar
+    dt = t[i1] - t[i0]
+    out = []
+    for a in (fR, fI, fRb, fIb):
+        om = np.log(np.abs(a[i1] / a[i0])) / dt
+        out.append(abs(ImOmega - om) / ImOmega)
+    return out
+
+
+def msw_errors(t, fee, fxx, feebar, fxxbar):
+    """msw_test.py:15-111: Richers (2019) B3 survival probabilities, tolerance 7e-2."""
+    t = np.asarray(t)
+    theta12 = 33.82 * np.pi / 180.
+    dm21c4 = 7.39e-5 * eV ** 2
+    E = dm21c4 * np.sin(2. * theta12) / (8. * np.pi * hbar * clight)
+    rhoYe = 4. * np.pi * hbar * clight * mp / (np.tan(2. * theta12) * np.sqrt(2.) * GF)
+    V = np.sqrt(2.) * GF * rhoYe / mp
+    C = np.cos(2. * theta12) + 2. * V * E / dm21c4
+    Cbar = np.cos(2. * theta12) - 2. * V * E / dm21c4
+    sin2_eff = np.sin(2. * theta12) ** 2 / (np.sin(2. * theta12) ** 2 + C ** 2)
+    sin2_effbar = np.sin(2. * theta12) ** 2 / (np.sin(2. * theta12) ** 2 + Cbar ** 2)
+    dm2_eff = dm21c4 * np.sqrt(np.sin(2. * theta12) ** 2 + C ** 2)
+    dm2_effbar = dm21c4 * np.sqrt(np.sin(2. * theta12) ** 2 + Cbar ** 2)
+
+    def Psurv(dm2, sin2theta, E):
+        return 1. - np.sin(t * dm2 / (4. * E * hbar)) ** 2 * sin2theta
+
+    return {
+        "ee": np.max(np.abs(np.asarray(fee) - Psurv(dm2_eff, sin2_eff, E))),
+        "xx": np.max(np.abs(np.asarray(fxx) - (1. - Psurv(dm2_eff, sin2_eff, E)))),
+        "eebar": np.max(np.abs(np.asarray(feebar) - Psurv(dm2_effbar, sin2_effbar, E))),
+        "xxbar": np.max(np.abs(np.asarray(fxxbar) - (1. - Psurv(dm2_effbar, sin2_effbar, E)))),
+        "cons": np.max(np.abs(np.asarray(fee) + np.asarray(fxx) - 1.)),
+        "consbar": np.max(np.abs(np.asarray(feebar) + np.asarray(fxxbar) - 1.)),
+    }
