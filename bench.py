@@ -1,0 +1,203 @@
+#!/usr/bin/env python3
+"""Particle-updates/s of the fused RK4 flavor-evolution step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ff64]
+
+One particle-update is one full RK4 step of one particle (four right-hand-side
+evaluations with their deposits, the stage updates, Renormalize and the
+periodic wrap; each particle carries nu and nubar) -- the reference's figure of
+merit (Source/main.cpp:181,219) times 1e6.  The workload is the 3-flavour
+fast-flavor box of 64^3 cells x 92 directions (24,117,248 particles).  Both
+the plain FP64 step and the FP32-bulk step run K timed steps from the same
+initial particles; `value` is the FP32-bulk rate, the FP64 rate and the
+growth-rate error of |N_emu| between the two are reported beside it.
+
+The timed region is the step loop only (inputs resident in HBM), bracketed by
+stream synchronisation and HIP events on the library's stream.  A second, short
+run records an event pair around every stage-kernel launch for the roofline.
+The CPU baseline is the FP64 oracle (OpenMP, reference-structured AoS with the
+same Copy/Saxpy RK4 pattern) on a 16^3-cell box of the same physics, timed on
+this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD public figure
+
+
+def algorithmic_bytes(nf: int, ppc: float, precision: str) -> float:
+    """Compulsory HBM bytes per particle-update of a 3-register classical RK4
+    whose mesh coupling deposits before every gather: 65 + 20 S values per
+    update (S = 2 NF^2 flavour reals), 27 of them positions kept in FP64, plus
+    the grid term 12 G 8 / ppc (G = 8 NF^2 deposited components, FP64
+    read-modify-write deposit and gather read over 4 stages)."""
+    S = 2 * nf * nf
+    G = 8 * nf * nf
+    values = 65 + 20 * S
+    part = 8 * values if precision == "fp64" else 8 * 27 + 4 * (values - 27)
+    return part + 12 * G * 8 / ppc
+
+
+def run_precision(core, parms, parts, precision, steps, warmup, profile_steps):
+    import torch
+    emu = parts.emu
+    st = core.FusedState(emu, parts.np, precision)
+    st.load(parts)
+    if warmup:
+        st.evolve(warmup)
+    info0 = st.info()
+    a0 = st.flavor_amplitude()
+    stream = torch.cuda.current_stream()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t_wall0 = time.perf_counter()
+    ev0.record(stream)
+    st.evolve(steps)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    t_wall = time.perf_counter() - t_wall0
+    ms = ev0.elapsed_time(ev1)
+    info1 = st.info()
+    a1 = st.flavor_amplitude()
+    rate = math.log(a1[0] / a0[0]) / (info1["time"] - info0["time"]) if a0[0] > 0 and a1[0] > 0 else float("nan")
+    # per-launch kernel times (separate short run so events do not perturb the timed loop)
+    st.profile(True)
+    st.evolve(profile_steps)
+    stage_ms, n_stage = st.kernel_ms(0)
+    fill_ms, _ = st.kernel_ms(1)
+    sort_ms, _ = st.kernel_ms(2)
+    dt_ms, _ = st.kernel_ms(3)
+    st.profile(False)
+    st.info()
+    st.close()
+    return {"ms_total": ms, "wall_s": t_wall, "ms_per_step": ms / steps, "updates_per_s": parts.np * steps / (ms * 1e-3),
+            "growth_rate": rate, "time_window": [info0["time"], info1["time"]], "amp": [a0[0], a1[0]],
+            "stage_ms": stage_ms, "stage_launches": n_stage, "fill_ms": fill_ms, "sort_ms": sort_ms, "dt_ms": dt_ms}
+
+
+def cpu_baseline(parms_dict, budget_s=15.0):
+    """FP64 oracle, OpenMP over this host's cores, on a 16^3-cell box of the
+    same physics (same cell size, directions and densities)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    ncores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    threads = max(1, min(16, ncores))
+    d = dict(parms_dict)
+    cell = (d["Lx"] / d["ncell"][0], d["Ly"] / d["ncell"][1], d["Lz"] / d["ncell"][2])
+    d["ncell"] = (16, 16, 16)
+    d["Lx"], d["Ly"], d["Lz"] = 16 * cell[0], 16 * cell[1], 16 * cell[2]
+    deck = O.Deck(**{k: v for k, v in d.items() if k in O.Deck.__dataclass_fields__})
+    parts = O.init_particles(deck)
+    prm = deck.params(nthreads=threads)
+    g = O.Grid(deck.nf, prm)
+    vv = O.vvac_max(deck.nf, prm, parts)
+    O.deposit_to_mesh(deck.nf, prm, parts, g)
+    err, dt = O.compute_dt(deck.nf, prm, g, vv)
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        err, dt, _ = O.rk4_step(deck.nf, prm, parts, g, dt, vv)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or steps >= 8:
+            break
+    cpu_model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu_model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"value": len(parts) * steps / el, "unit": "particle-updates/s", "cores": threads, "kind": "port",
+            "sample": f"FP64 oracle (OpenMP, AoS, RK4 Copy/Saxpy) on 16^3 cells x {len(parts) // 4096} directions "
+                      f"= {len(parts)} particles, {steps} steps in {el:.1f} s, same cell size and densities; "
+                      f"{threads} threads of {ncores} ({cpu_model})"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="ff64")
+    ap.add_argument("--profile-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--precision", default="both", choices=["both", "fp64", "fp32"])
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this build runs on a single MI355X; "
+              "the multi-GPU decomposition is a design, not an implementation)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from emu_amd import capi, configs, core
+    capi.lib()  # fails loudly without the HIP library
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    parms = configs.WORKLOADS[args.workload]()
+    emu = core.Emu(parms)
+    parts = core.FlavoredNeutrinoContainer.init_particles(emu)
+    torch.cuda.synchronize()
+    ncell = parms.ncell[0] * parms.ncell[1] * parms.ncell[2]
+    ppc = parts.np / ncell
+    res = {}
+    precs = ["fp64", "fp32"] if args.precision == "both" else [args.precision]
+    for prec in precs:
+        res[prec] = run_precision(core, parms, parts, prec, args.steps, args.warmup, args.profile_steps)
+        torch.cuda.empty_cache()
+    head = "fp32" if "fp32" in res else "fp64"
+    r = res[head]
+    bpu = algorithmic_bytes(parms.nf, ppc, head)
+    # one stage-kernel launch processes every particle for one of the four stages
+    bytes_per_launch = parts.np * bpu / 4.0
+    achieved = bytes_per_launch / (r["stage_ms"] * 1e-3) / 1e9 if r["stage_ms"] > 0 else None
+    out = {
+        "metric": "particle-updates/s FP32-bulk vs FP64 plain, 1 GPU, at <=1% growth err; %HBM peak",
+        "value": r["updates_per_s"],
+        "unit": "particle-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if head == "fp32" else "f64",
+        "data": "synthetic (seeded initial conditions of the workload, generated on the GPU)",
+        "config": {"workload": f"{args.workload}: {parms.nf} flavours, {parms.ncell[0]}x{parms.ncell[1]}x{parms.ncell[2]} "
+                               f"periodic cells, {round(ppc)} directions/cell, {parts.np} particles, fast-flavor",
+                   "particles": parts.np, "precision": head},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+                     "kernel": "k_stage (gather + RHS + RK update + deposit, one launch per RK stage)",
+                     "algorithmic_bytes_per_update": bpu, "avg_launch_ms": r["stage_ms"]},
+    }
+    if "fp64" in res:
+        f64 = res["fp64"]
+        b64 = algorithmic_bytes(parms.nf, ppc, "fp64")
+        out["fp64"] = {"value": f64["updates_per_s"], "ms_per_step": f64["ms_per_step"], "stage_ms": f64["stage_ms"],
+                       "roofline_frac": (parts.np * b64 / 4.0 / (f64["stage_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS)
+                       if f64["stage_ms"] > 0 else None,
+                       "growth_rate": f64["growth_rate"]}
+    if "fp32" in res and "fp64" in res:
+        g32, g64 = res["fp32"]["growth_rate"], res["fp64"]["growth_rate"]
+        out["fp32_vs_fp64"] = {"speedup": res["fp32"]["updates_per_s"] / res["fp64"]["updates_per_s"],
+                               "growth_rate_rel_err": abs(g32 - g64) / abs(g64) if g64 else None}
+    out["kernels_ms"] = {p: {k: res[p][k] for k in ("stage_ms", "fill_ms", "sort_ms", "dt_ms")} for p in res}
+    if not args.no_cpu:
+        d = {k: getattr(parms, k) for k in parms.__dataclass_fields__ if k != "extra"}
+        out["cpu_baseline"] = cpu_baseline(d)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

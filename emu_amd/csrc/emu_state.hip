@@ -1,0 +1,1238 @@
+// emu_state.hip -- the fused RK4 step on a library-owned, cell-sorted SoA
+// particle state (the integrator loop of main.cpp:129-213).
+//
+// Per step (stage s reads the grid deposited from the stage-s state):
+//   stage 1..3 kernels: gather + Hamiltonian + commutator + RK update of the
+//     particle, then deposit of the next stage state into the other grid;
+//   compute_dt reduction over the stage-4 deposit (main.cpp:193-197);
+//   stage 4 kernel: y_{n+1}, positions, periodic shift, Renormalize (FP64),
+//     the deposit of y_{n+1} that the next step's stage 1 reads, and the new
+//     cell index;
+//   counting sort of the particles by cell (stable, so deterministic).
+//
+// The mesh holds, per cell and flavour component c, the combined fields
+//   M_c  = N_c - conj(Nbar)_c  (+ rho Ye/Mp dV on 00, the matter term)
+//   Phi_d,c = F_d,c - conj(Fbar_d,c),  d = x, y, z
+// which is all the Hamiltonian needs (H_SI = sqrt2 GF/dV (M - p.Phi),
+// Hbar = conj(H_vac - H_SI - H_matter), generate_code.py:296-357); compute_dt
+// additionally needs N_ii, Nbar_ii and Im(Fbar_d) off the diagonal, which the
+// stage-3 kernel deposits into an auxiliary grid.
+#include "emu_internal.h"
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace emu {
+namespace {
+
+constexpr int kWave = 64;
+
+template <typename T> struct StageArgs {
+    // persistent particle arrays (current sort order)
+    double* pos[3];
+    T* pup[4];  // pupx, pupy, pupz, pupt
+    T* Nn[2];   // N, Nbar
+    T* Ll[2];   // L, Lbar
+    T* f;       // y_n: [2*NH][np] (nu components then nubar)
+    T* Y;       // stage state
+    T* acc;     // RK accumulator
+    int* newcell;
+    int* rank;
+    const int* cell_start;
+    const double* gin;   // grid read by this stage
+    double* gout;        // grid deposited by this stage
+    double* aux;         // compute_dt auxiliaries (stage 3 and the initial deposit)
+    const double* dt;    // device scalar
+    unsigned long long* err; // Renormalize status (lowest id << 3 | code)
+    const int64_t* id;
+    int64_t np;
+    int stage;           // 0 = deposit only, 1..4
+    int want_aux;
+    Geom g;
+    double vac[9];       // M2 c^4 / (2 hbar), per component
+    double coef;         // sqrt2 GF / dV / hbar
+    double maxError;
+};
+
+// Base value of the deposit grid: zero except the matter term rho Ye/Mp dV
+// on M_00 (inside_parentheses of generate_code.py:335-342, moved into the
+// field because it enters the gather with the same weights).
+__global__ void k_fill_grid(double* g, int64_t ncell, int Q, double m00, double* aux, int naux)
+{
+    const int64_t n = ncell * Q;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+        g[t] = (t % Q) == 0 ? m00 : 0.0;
+    if (aux) {
+        const int64_t na = ncell * naux;
+        for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < na; t += (int64_t)gridDim.x * blockDim.x)
+            aux[t] = 0.0;
+    }
+}
+
+__device__ __forceinline__ int bin1(double pos, double plo, double dxi, int n)
+{
+#pragma clang fp contract(off)
+    return pmod(nearest_cell_center_index((pos - plo) * dxi), n);
+}
+
+__device__ __forceinline__ double advance_pos(double x, double v, const double a[4])
+{
+#pragma clang fp contract(off)
+    x = a[0] * v + x;
+    x = a[1] * v + x;
+    x = a[2] * v + x;
+    x = a[3] * v + x;
+    return x;
+}
+
+__device__ __forceinline__ double wrap1(double x, double rlo, double rhi, double L)
+{
+#pragma clang fp contract(off)
+    // one period per step at most; the bound keeps non-finite input from looping
+    if (x > rhi) {
+        for (int it = 0; it < 8 && x > rhi; it++) x -= L;
+        if (x < rlo || x > rhi) x = rlo;
+    } else if (x < rlo) {
+        for (int it = 0; it < 8 && x < rlo; it++) x += L;
+        if (x > rhi || x < rlo) x = rhi;
+    }
+    return x;
+}
+
+template <int NF>
+__device__ __forceinline__ int renorm_fp64(double* f, double L, double maxError)
+{
+#pragma clang fp contract(off)
+    double sumP = 0;
+#pragma unroll
+    for (int i = 0; i < NF; i++) sumP += f[herm_re<NF>(i, i)];
+    double error = sumP - 1.0;
+    if (fabs(error) > 100. * maxError) return EMU_ERR_TRACE;
+    if (fabs(error) > maxError) {
+#pragma unroll
+        for (int i = 0; i < NF; i++) f[herm_re<NF>(i, i)] -= error / NF;
+    }
+#pragma unroll
+    for (int i = 0; i < NF; i++) {
+        double& fii = f[herm_re<NF>(i, i)];
+        if (fii < -100. * maxError) return EMU_ERR_NEGDIAG;
+        if (fii < -maxError) fii = 0;
+    }
+    const double length = sqrt(su_mag2<NF, double>(f));
+    error = length - L;
+    if (fabs(error) > 100. * maxError) return EMU_ERR_LENGTH;
+    if (fabs(error) > maxError) {
+#pragma unroll
+        for (int c = 0; c < NH<NF>; c++) f[c] /= length / L;
+    }
+    return EMU_OK;
+}
+
+template <int NF> __device__ __forceinline__ constexpr bool is_im_comp(int c)
+{
+    for (int a = 0; a < NF; a++)
+        for (int b = a + 1; b < NF; b++)
+            if (c == herm_im<NF>(a, b)) return true;
+    return false;
+}
+
+// Per-particle deposit of the combined source q = N f - conj(Nbar fbar) with
+// weights w into the grid (and, when requested, the compute_dt auxiliaries).
+template <int NF, typename T, int WX, int WY, int WZ>
+__device__ __forceinline__ void deposit_particle(double* grid, double* aux, bool want_aux, const Geom& g,
+                                                 const double pos[3], const T vhat[3], T N, T Nb,
+                                                 const T* fnu, const T* fbar)
+{
+    constexpr int nh = NH<NF>;
+    constexpr int QG = 4 * nh;
+    constexpr int NAUX = 2 * NF + 3 * (NF * (NF - 1) / 2);
+    int idx[3];
+    double wd[3][3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) shape3((pos[d] - g.plo[d]) * g.dxi[d], g.order[d], idx[d], wd[d]);
+    T q[nh];
+#pragma unroll
+    for (int c = 0; c < nh; c++) q[c] = is_im_comp<NF>(c) ? N * fnu[c] + Nb * fbar[c] : N * fnu[c] - Nb * fbar[c];
+#pragma unroll
+    for (int kk = 0; kk < WZ; kk++)
+#pragma unroll
+        for (int jj = 0; jj < WY; jj++)
+#pragma unroll
+            for (int ii = 0; ii < WX; ii++) {
+                const int ox = WX == 1 ? 1 : ii, oy = WY == 1 ? 1 : jj, oz = WZ == 1 ? 1 : kk;
+                const T w = T(wd[0][ox] * wd[1][oy] * wd[2][oz]);
+                const int ci = pmod(idx[0] + ox - 1, g.n[0]), cj = pmod(idx[1] + oy - 1, g.n[1]),
+                          ck = pmod(idx[2] + oz - 1, g.n[2]);
+                const int64_t cell = ((int64_t)ck * g.n[1] + cj) * g.n[0] + ci;
+                double* G = grid + cell * QG;
+#pragma unroll
+                for (int c = 0; c < nh; c++) {
+                    const T wq = w * q[c];
+                    unsafeAtomicAdd(&G[c], (double)wq);
+                    unsafeAtomicAdd(&G[nh + c], (double)(wq * vhat[0]));
+                    unsafeAtomicAdd(&G[2 * nh + c], (double)(wq * vhat[1]));
+                    unsafeAtomicAdd(&G[3 * nh + c], (double)(wq * vhat[2]));
+                }
+                if (want_aux) {
+                    double* A = aux + cell * NAUX;
+#pragma unroll
+                    for (int a = 0; a < NF; a++) {
+                        unsafeAtomicAdd(&A[a], (double)(w * N * fnu[herm_re<NF>(a, a)]));
+                        unsafeAtomicAdd(&A[NF + a], (double)(w * Nb * fbar[herm_re<NF>(a, a)]));
+                    }
+                    int o = 2 * NF;
+#pragma unroll
+                    for (int a = 0; a < NF; a++)
+#pragma unroll
+                        for (int b = a + 1; b < NF; b++) {
+                            const T wf = w * Nb * fbar[herm_im<NF>(a, b)];
+#pragma unroll
+                            for (int d = 0; d < 3; d++) unsafeAtomicAdd(&A[o + d], (double)(wf * vhat[d]));
+                            o += 3;
+                        }
+                }
+            }
+}
+
+// One block per cell; its waves take the cell's particles 64 at a time.
+template <int NF, typename T, int WX, int WY, int WZ>
+__global__ __launch_bounds__(256) void k_stage(StageArgs<T> a)
+{
+    constexpr int nh = NH<NF>;
+    constexpr int QG = 4 * nh;
+    const Geom& g = a.g;
+    const int cell = blockIdx.x;
+    const int cx = cell % g.n[0], cy = (cell / g.n[0]) % g.n[1], cz = cell / (g.n[0] * g.n[1]);
+    const int beg = a.cell_start[cell], end = a.cell_start[cell + 1];
+    const int nw = blockDim.x / kWave, wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const double dt = *a.dt;
+    // Classical RK4 (AMReX RKIntegrator, rk.type = 4): stage weights b and the
+    // coefficient of the next stage state, both scaled by dt in double.
+    const double bw[4] = {1. / 6., 1. / 3., 1. / 3., 1. / 6.};
+    const double cnext[4] = {0.5, 0.5, 1.0, 0.0};
+    const int s = a.stage;
+
+    for (int base = beg + wave * kWave; base < end; base += nw * kWave) {
+        const int64_t ip = base + lane;
+        if (ip >= end) continue;
+        double pos[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) pos[d] = a.pos[d][ip];
+        const T pupt = a.pup[3][ip];
+        const T vhat[3] = {a.pup[0][ip] / pupt, a.pup[1][ip] / pupt, a.pup[2][ip] / pupt};
+        const T N = a.Nn[0][ip], Nb = a.Nn[1][ip];
+        T Y[2 * nh];
+        if (s <= 1) {
+#pragma unroll
+            for (int c = 0; c < 2 * nh; c++) Y[c] = a.f[(int64_t)c * a.np + ip];
+        } else {
+#pragma unroll
+            for (int c = 0; c < 2 * nh; c++) Y[c] = a.Y[(int64_t)c * a.np + ip];
+        }
+        if (s == 0) { // initial deposit only
+            deposit_particle<NF, T, WX, WY, WZ>(a.gout, a.aux, a.want_aux, g, pos, vhat, N, Nb, Y, Y + nh);
+            continue;
+        }
+        // ---- gather: inside = sum_s w_s (M - vhat . Phi) over the stencil
+        int idx[3];
+        double wd[3][3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) shape3((pos[d] - g.plo[d]) * g.dxi[d], g.order[d], idx[d], wd[d]);
+        T inside[nh];
+#pragma unroll
+        for (int c = 0; c < nh; c++) inside[c] = T(0);
+#pragma unroll
+        for (int kk = 0; kk < WZ; kk++)
+#pragma unroll
+            for (int jj = 0; jj < WY; jj++)
+#pragma unroll
+                for (int ii = 0; ii < WX; ii++) {
+                    const int ox = WX == 1 ? 1 : ii, oy = WY == 1 ? 1 : jj, oz = WZ == 1 ? 1 : kk;
+                    const int ci = pmod(cx + ox - 1, g.n[0]), cj = pmod(cy + oy - 1, g.n[1]),
+                              ck = pmod(cz + oz - 1, g.n[2]);
+                    const double* G = a.gin + (((int64_t)ck * g.n[1] + cj) * g.n[0] + ci) * QG;
+                    const T w = T(wd[0][ox] * wd[1][oy] * wd[2][oz]);
+                    const T wx = w * vhat[0], wy = w * vhat[1], wz = w * vhat[2];
+#pragma unroll
+                    for (int c = 0; c < nh; c++)
+                        inside[c] += w * T(G[c]) - wx * T(G[nh + c]) - wy * T(G[2 * nh + c]) - wz * T(G[3 * nh + c]);
+                }
+        // ---- Hamiltonians / hbar (1/s): H = H_vac + coef*inside, Hbar = conj(H_vac - coef*inside)
+        T H[nh], Hb[nh];
+        const T inv_pupt = T(1) / pupt;
+#pragma unroll
+        for (int c = 0; c < nh; c++) {
+            const T hv = T(a.vac[c]) * inv_pupt;
+            const T hs = T(a.coef) * inside[c];
+            H[c] = hv + hs;
+            Hb[c] = is_im_comp<NF>(c) ? -(hv - hs) : hv - hs;
+        }
+        T k[2 * nh];
+        commutator_rhs<NF, T>(H, Y, k, T(1));
+        commutator_rhs<NF, T>(Hb, Y + nh, k + nh, T(1));
+        // ---- RK update
+        const T a_acc = T(dt * bw[s - 1]);
+        if (s < 4) {
+            const T a_next = T(dt * cnext[s - 1]);
+            T yn[2 * nh], Ynext[2 * nh];
+#pragma unroll
+            for (int c = 0; c < 2 * nh; c++) {
+                yn[c] = s == 1 ? Y[c] : a.f[(int64_t)c * a.np + ip];
+                const T acc_old = s == 1 ? yn[c] : a.acc[(int64_t)c * a.np + ip];
+                a.acc[(int64_t)c * a.np + ip] = a_acc * k[c] + acc_old;
+                Ynext[c] = a_next * k[c] + yn[c];
+                a.Y[(int64_t)c * a.np + ip] = Ynext[c];
+            }
+            deposit_particle<NF, T, WX, WY, WZ>(a.gout, a.aux, s == 3 && a.want_aux, g, pos, vhat, N, Nb, Ynext,
+                                                Ynext + nh);
+        } else {
+            // y_{n+1} = acc + (dt/6) k4, then the post-step hook (main.cpp:155-199)
+            double y[2 * nh];
+#pragma unroll
+            for (int c = 0; c < 2 * nh; c++) y[c] = double(a_acc * k[c] + a.acc[(int64_t)c * a.np + ip]);
+            int err = renorm_fp64<NF>(y, double(a.Ll[0][ip]), a.maxError);
+            if (err == EMU_OK) err = renorm_fp64<NF>(y + nh, double(a.Ll[1][ip]), a.maxError);
+            if (err != EMU_OK) atomicMin(a.err, ((unsigned long long)a.id[ip] << 3) | (unsigned long long)err);
+            T ynew[2 * nh];
+#pragma unroll
+            for (int c = 0; c < 2 * nh; c++) {
+                ynew[c] = T(y[c]);
+                a.f[(int64_t)c * a.np + ip] = ynew[c];
+            }
+            // positions: x += (dt b_i) c pup/pupt for i = 1..4, then the periodic shift
+            const double ai[4] = {dt * bw[0], dt * bw[1], dt * bw[2], dt * bw[3]};
+            double npos[3];
+            int nc[3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const double v = double(a.pup[d][ip]) / double(pupt) * phys::c;
+                npos[d] = wrap1(advance_pos(pos[d], v, ai), g.rlo[d], g.rhi[d], g.L[d]);
+                a.pos[d][ip] = npos[d];
+                nc[d] = bin1(npos[d], g.plo[d], g.dxi[d], g.n[d]);
+            }
+            const int newc = (nc[2] * g.n[1] + nc[1]) * g.n[0] + nc[0];
+            a.newcell[ip] = newc;
+            deposit_particle<NF, T, WX, WY, WZ>(a.gout, a.aux, false, g, npos, vhat, N, Nb, ynew, ynew + nh);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Stable counting sort by cell after each step.  A particle moves at most one
+// cell per dimension per step (c dt < dx, checked on the host), so its move
+// is one of 27 offsets e.  Particles arriving in cell t are ordered by the
+// canonical index of e, then by their position in their old cell: every
+// particle's destination is base[old cell][e] + (its rank among the particles
+// of its old cell with the same e), which makes the order deterministic.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int move1(int from, int to, int n, int* bad)
+{
+    const int dd = pmod(to - from, n);
+    if (dd == 0) return 0;
+    if (dd == 1) return 1;
+    if (dd == n - 1) return -1;
+    *bad = 1;
+    return 0;
+}
+
+// One wave per old cell: move index and rank within (cell, move); m[c][e].
+__global__ void k_move_ranks(const int* __restrict__ newcell, const int* __restrict__ cell_start, Geom g,
+                             int* __restrict__ rank, unsigned char* __restrict__ mv, int* __restrict__ m,
+                             unsigned long long* err)
+{
+    const int c = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int cx = c % g.n[0], cy = (c / g.n[0]) % g.n[1], cz = c / (g.n[0] * g.n[1]);
+    const int beg = cell_start[c], end = cell_start[c + 1];
+    int running = 0; // lane e (< 27) holds the running count of move e
+    for (int base = beg; base < end; base += 64) {
+        const int ip = base + lane;
+        const bool act = ip < end;
+        int e = 27;
+        if (act) {
+            const int t = newcell[ip];
+            const int tx = t % g.n[0], ty = (t / g.n[0]) % g.n[1], tz = t / (g.n[0] * g.n[1]);
+            int bad = 0;
+            const int ex = move1(cx, tx, g.n[0], &bad), ey = move1(cy, ty, g.n[1], &bad),
+                      ez = move1(cz, tz, g.n[2], &bad);
+            if (bad) atomicMin(err, (unsigned long long)EMU_ERR_MOVE | (~0ull << 3));
+            e = (ez + 1) * 9 + (ey + 1) * 3 + (ex + 1);
+        }
+        int myrank = 0;
+        for (int q = 0; q < 27; q++) {
+            const unsigned long long msk = __ballot(e == q);
+            if (msk == 0) continue;
+            const int before = __shfl(running, q);
+            if (e == q) myrank = before + __popcll(msk & ((1ull << lane) - 1ull));
+            if (lane == q) running += __popcll(msk);
+        }
+        if (act) {
+            rank[ip] = myrank;
+            mv[ip] = (unsigned char)e;
+        }
+    }
+    if (lane < 27) m[(int64_t)c * 27 + lane] = running;
+}
+
+__device__ __forceinline__ bool valid_move(int e, const Geom& g)
+{
+    const int ex = e % 3 - 1, ey = (e / 3) % 3 - 1, ez = e / 9 - 1;
+    const int ed[3] = {ex, ey, ez};
+    for (int d = 0; d < 3; d++) {
+        if (g.n[d] == 1 && ed[d] != 0) return false;
+        if (g.n[d] == 2 && ed[d] == -1) return false;
+    }
+    return true;
+}
+
+__device__ __forceinline__ int64_t source_of(int t, int e, const Geom& g)
+{
+    const int tx = t % g.n[0], ty = (t / g.n[0]) % g.n[1], tz = t / (g.n[0] * g.n[1]);
+    const int sx = pmod(tx - (e % 3 - 1), g.n[0]), sy = pmod(ty - ((e / 3) % 3 - 1), g.n[1]),
+              sz = pmod(tz - (e / 9 - 1), g.n[2]);
+    return ((int64_t)sz * g.n[1] + sy) * g.n[0] + sx;
+}
+
+// counts[t] = number of particles arriving in t.
+__global__ void k_new_counts(const int* __restrict__ m, Geom g, int64_t ncell, int* __restrict__ counts)
+{
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= ncell) return;
+    int n = 0;
+    for (int e = 0; e < 27; e++)
+        if (valid_move(e, g)) n += m[source_of((int)t, e, g) * 27 + e];
+    counts[t] = n;
+}
+
+// base[s][e] for every source s = t - e, in canonical move order.
+__global__ void k_bases(const int* __restrict__ m, const int* __restrict__ new_start, Geom g, int64_t ncell,
+                        int* __restrict__ base)
+{
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= ncell) return;
+    int run = new_start[t];
+    for (int e = 0; e < 27; e++) {
+        if (!valid_move(e, g)) continue;
+        const int64_t sidx = source_of((int)t, e, g) * 27 + e;
+        base[sidx] = run;
+        run += m[sidx];
+    }
+}
+
+template <typename T> struct Arrays {
+    double* pos[3];
+    T* pup[4];
+    T* Nn[2];
+    T* Ll[2];
+    T* f;
+    int64_t* id;
+};
+
+template <typename T>
+__global__ void k_scatter(Arrays<T> src, Arrays<T> dst, const int* __restrict__ cell_start,
+                          const int* __restrict__ base, const int* __restrict__ rank,
+                          const unsigned char* __restrict__ mv, int64_t np, int ncomp, unsigned long long* err)
+{
+    const int c = blockIdx.x;
+    const int beg = cell_start[c], end = cell_start[c + 1];
+    for (int ip = beg + threadIdx.x; ip < end; ip += blockDim.x) {
+        const int64_t j = base[(int64_t)c * 27 + mv[ip]] + rank[ip];
+        if (j < 0 || j >= np) { // cannot happen for a consistent sort; never write outside the arrays
+            atomicMin(err, (unsigned long long)EMU_ERR_MOVE | (~0ull << 3));
+            continue;
+        }
+#pragma unroll
+        for (int d = 0; d < 3; d++) dst.pos[d][j] = src.pos[d][ip];
+#pragma unroll
+        for (int d = 0; d < 4; d++) dst.pup[d][j] = src.pup[d][ip];
+        dst.Nn[0][j] = src.Nn[0][ip];
+        dst.Nn[1][j] = src.Nn[1][ip];
+        dst.Ll[0][j] = src.Ll[0][ip];
+        dst.Ll[1][j] = src.Ll[1][ip];
+        dst.id[j] = src.id[ip];
+        for (int q = 0; q < ncomp; q++) dst.f[(int64_t)q * np + j] = src.f[(int64_t)q * np + ip];
+    }
+}
+
+// compute_dt over the cells (Evolve.cpp:40-56) from the combined fields and
+// the auxiliaries.  HSI = N - conj(Nbar) + matter = M; F - Fbar keeps Re and the
+// diagonal of Phi and takes Im(F - Fbar) = Im(Phi) - 2 Im(Fbar).
+template <int NF>
+__global__ void k_dt_reduce(const double* __restrict__ grid, const double* __restrict__ aux, int64_t ncell,
+                            double cell_volume, double m00, unsigned long long* red)
+{
+    constexpr int nh = NH<NF>;
+    constexpr int QG = 4 * nh;
+    constexpr int NAUX = 2 * NF + 3 * (NF * (NF - 1) / 2);
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    double Vad = 0.0, Vst = 0.0;
+    if (t < ncell) {
+        const double* G = grid + t * QG;
+        const double* A = aux + t * NAUX;
+        double h[nh];
+#pragma unroll
+        for (int c = 0; c < nh; c++) h[c] = G[c];
+        double V2 = su_mag2<NF, double>(h);
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+#pragma unroll
+            for (int c = 0; c < nh; c++) h[c] = G[(d + 1) * nh + c];
+            int o = 2 * NF;
+#pragma unroll
+            for (int x = 0; x < NF; x++)
+#pragma unroll
+                for (int y = x + 1; y < NF; y++) {
+                    h[herm_im<NF>(x, y)] -= 2.0 * A[o + d];
+                    o += 3;
+                }
+            V2 += su_mag2<NF, double>(h);
+        }
+        Vad = sqrt(V2) * M_SQRT2 * phys::GF / cell_volume;
+#pragma unroll
+        for (int x = 0; x < NF; x++) {
+            Vst = Vst > A[x] ? Vst : A[x];
+            Vst = Vst > A[NF + x] ? Vst : A[NF + x];
+        }
+        Vst = Vst > m00 ? Vst : m00;
+        Vst *= 2.0 * NF * M_SQRT2 * phys::GF / cell_volume;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        Vad = fmax(Vad, __shfl_xor(Vad, off));
+        Vst = fmax(Vst, __shfl_xor(Vst, off));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(&red[0], (unsigned long long)__double_as_longlong(Vad > 0 ? Vad : 0.0));
+        atomicMax(&red[1], (unsigned long long)__double_as_longlong(Vst > 0 ? Vst : 0.0));
+    }
+}
+
+struct DtParams {
+    double dt_translation, flavor_cfl, max_adaptive_speedup, Vvac_max, fixed_dt;
+    int use_flavor;
+};
+
+// main.cpp:193-198 and Evolve.cpp:57-88: the next dt from the reductions.
+__global__ void k_dt_finalize(const unsigned long long* red, DtParams p, double* dt_next, unsigned long long* err)
+{
+    if (threadIdx.x != 0) return;
+    if (p.fixed_dt > 0) { *dt_next = p.fixed_dt; return; }
+    double dt_flavor = 0.0;
+    if (p.use_flavor) {
+        const double vad = __longlong_as_double((long long)red[0]) + p.Vvac_max;
+        const double vst = __longlong_as_double((long long)red[1]) + p.Vvac_max;
+        const double dfa = phys::hbar / vad * p.flavor_cfl;
+        const double dfs = phys::hbar / vst * p.flavor_cfl;
+        dt_flavor = dfs;
+        if (p.max_adaptive_speedup > 1) dt_flavor = fmin(dfs * p.max_adaptive_speedup, dfa);
+    }
+    double dt = 0.0;
+    if (p.dt_translation != 0.0 && dt_flavor != 0.0) dt = fmin(p.dt_translation, dt_flavor);
+    else if (p.dt_translation != 0.0) dt = p.dt_translation;
+    else if (dt_flavor != 0.0) dt = dt_flavor;
+    else atomicMin(err, (unsigned long long)EMU_ERR_DT | (~0ull << 3));
+    *dt_next = dt;
+}
+
+// End of step: particle time advanced by the four weighted stages, dt <- next.
+__global__ void k_step_end(double* time, double* dt, const double* dt_next, int64_t* steps)
+{
+#pragma clang fp contract(off)
+    if (threadIdx.x != 0) return;
+    const double h = *dt;
+    double t = *time;
+    t = h * (1. / 6.) * 1.0 + t;
+    t = h * (1. / 3.) * 1.0 + t;
+    t = h * (1. / 3.) * 1.0 + t;
+    t = h * (1. / 6.) * 1.0 + t;
+    *time = t;
+    *dt = *dt_next;
+    *steps += 1;
+}
+
+__global__ void k_fill_int(int* p, int64_t n, int v)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) p[t] = v;
+}
+
+__global__ void k_fill_u64v(unsigned long long* p, int n, unsigned long long v)
+{
+    if ((int)threadIdx.x < n) p[threadIdx.x] = v;
+}
+
+// Load: bin every particle of a caller's SoA view (PIdx order).
+__global__ void k_bin_input(PView P, Geom g, int* cellidx, int* counts)
+{
+    for (int64_t ip = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; ip < P.np; ip += (int64_t)gridDim.x * blockDim.x) {
+        int nc[3];
+        for (int d = 0; d < 3; d++) nc[d] = bin1(P.pos[d][ip], g.plo[d], g.dxi[d], g.n[d]);
+        const int c = (nc[2] * g.n[1] + nc[1]) * g.n[0] + nc[0];
+        cellidx[ip] = c;
+        atomicAdd(&counts[c], 1);
+    }
+}
+
+__global__ void k_iota(int* p, int64_t n)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+        p[t] = (int)t;
+}
+
+// Load: sorted slot j takes input particle order[j] (stable radix sort by cell).
+template <int NF, typename T>
+__global__ void k_load(PView P, const int64_t* ids, const int* __restrict__ order, Arrays<T> dst, int64_t np)
+{
+    using I = PIdx<NF>;
+    constexpr int nh = NH<NF>;
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < np; j += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t ip = order[j];
+        for (int d = 0; d < 3; d++) dst.pos[d][j] = P.pos[d][ip];
+        for (int d = 0; d < 4; d++) dst.pup[d][j] = T(P.r[I::pupx + d][ip]);
+        dst.Nn[0][j] = T(P.r[I::N][ip]);
+        dst.Nn[1][j] = T(P.r[I::Nbar][ip]);
+        dst.Ll[0][j] = T(P.r[I::L][ip]);
+        dst.Ll[1][j] = T(P.r[I::Lbar][ip]);
+        dst.id[j] = ids ? ids[ip] : ip;
+        for (int c = 0; c < nh; c++) {
+            dst.f[(int64_t)c * np + j] = T(P.r[I::f + c][ip]);
+            dst.f[(int64_t)(nh + c) * np + j] = T(P.r[I::fbar + c][ip]);
+        }
+    }
+}
+
+template <int NF, typename T>
+__global__ void k_store(Arrays<T> src, PView P, int64_t* ids, const double* time, int64_t np)
+{
+    using I = PIdx<NF>;
+    constexpr int nh = NH<NF>;
+    const double t = *time;
+    for (int64_t ip = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; ip < np; ip += (int64_t)gridDim.x * blockDim.x) {
+        for (int d = 0; d < 3; d++) {
+            P.pos[d][ip] = src.pos[d][ip];
+            P.r[I::x + d][ip] = src.pos[d][ip];
+        }
+        P.r[I::time][ip] = t;
+        for (int d = 0; d < 4; d++) P.r[I::pupx + d][ip] = double(src.pup[d][ip]);
+        P.r[I::N][ip] = double(src.Nn[0][ip]);
+        P.r[I::Nbar][ip] = double(src.Nn[1][ip]);
+        P.r[I::L][ip] = double(src.Ll[0][ip]);
+        P.r[I::Lbar][ip] = double(src.Ll[1][ip]);
+        for (int c = 0; c < nh; c++) {
+            P.r[I::f + c][ip] = double(src.f[(int64_t)c * np + ip]);
+            P.r[I::fbar + c][ip] = double(src.f[(int64_t)(nh + c) * np + ip]);
+        }
+        if (ids) ids[ip] = src.id[ip];
+    }
+}
+
+__global__ void k_min_pupt_t(const double* __restrict__ pupt, int64_t np, unsigned long long* red)
+{
+    double v = INFINITY;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < np; t += (int64_t)gridDim.x * blockDim.x)
+        v = fmin(v, pupt[t]);
+    for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off));
+    if ((threadIdx.x & 63) == 0) atomicMin(red, (unsigned long long)__double_as_longlong(v));
+}
+
+// Diagnostic: per cell N_01 = sum_p w N f01 and Nbar_01 (Re, Im), deposited
+// with the particle shape factors, then the cell sums of |N_01| and |Nbar_01|
+// and of N_00, Nbar_00.
+template <int NF, typename T>
+__global__ void k_diag_deposit(Arrays<T> A, const int* __restrict__ cell_start, Geom g, int64_t np, double* out4)
+{
+    const int cell = blockIdx.x;
+    for (int ip = cell_start[cell] + threadIdx.x; ip < cell_start[cell + 1]; ip += blockDim.x) {
+        int idx[3];
+        double wd[3][3];
+        for (int d = 0; d < 3; d++) shape3((A.pos[d][ip] - g.plo[d]) * g.dxi[d], g.order[d], idx[d], wd[d]);
+        const double N = A.Nn[0][ip], Nb = A.Nn[1][ip];
+        const double v[4] = {N * (double)A.f[(int64_t)herm_re<NF>(0, 1) * np + ip],
+                             N * (double)A.f[(int64_t)herm_im<NF>(0, 1) * np + ip],
+                             Nb * (double)A.f[(int64_t)(NH<NF> + herm_re<NF>(0, 1)) * np + ip],
+                             Nb * (double)A.f[(int64_t)(NH<NF> + herm_im<NF>(0, 1)) * np + ip]};
+        for (int kk = 0; kk < 3; kk++)
+            for (int jj = 0; jj < 3; jj++)
+                for (int ii = 0; ii < 3; ii++) {
+                    const double w = wd[0][ii] * wd[1][jj] * wd[2][kk];
+                    if (w == 0.0) continue;
+                    const int64_t c = ((int64_t)pmod(idx[2] + kk - 1, g.n[2]) * g.n[1] + pmod(idx[1] + jj - 1, g.n[1])) *
+                                          g.n[0] + pmod(idx[0] + ii - 1, g.n[0]);
+                    for (int q = 0; q < 4; q++) unsafeAtomicAdd(&out4[c * 4 + q], w * v[q]);
+                }
+    }
+}
+
+__global__ void k_diag_reduce(const double* __restrict__ g4, int64_t ncell, double* sums)
+{
+    double a = 0, b = 0;
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < ncell; c += (int64_t)gridDim.x * blockDim.x) {
+        a += sqrt(g4[4 * c] * g4[4 * c] + g4[4 * c + 1] * g4[4 * c + 1]);
+        b += sqrt(g4[4 * c + 2] * g4[4 * c + 2] + g4[4 * c + 3] * g4[4 * c + 3]);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off);
+        b += __shfl_xor(b, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        unsafeAtomicAdd(&sums[0], a);
+        unsafeAtomicAdd(&sums[1], b);
+    }
+}
+
+} // namespace
+} // namespace emu
+
+using namespace emu;
+
+// ---------------------------------------------------------------------------
+// the state object
+// ---------------------------------------------------------------------------
+struct emu_state {
+    emu_ctx* ctx;
+    int64_t np;
+    int precision; // EMU_PLAIN_FP64 / EMU_FP32_BULK
+    int nf, nh, QG, naux;
+    int64_t ncell;
+    size_t tsize;
+    // persistent arrays, two copies for the sort
+    void* block[2] = {nullptr, nullptr};
+    // scratch
+    void* Y = nullptr;
+    void* acc = nullptr;
+    int* newcell = nullptr;
+    int* rank = nullptr;
+    int* counts = nullptr;
+    int* cell_start = nullptr;     // ncell + 1
+    int* new_start = nullptr;      // ncell + 1
+    int* mcount = nullptr;         // [ncell][27] particles per (old cell, move)
+    int* mbase = nullptr;          // [ncell][27] destination of the first of them
+    unsigned char* mv = nullptr;   // move index per particle
+    double* grid[2] = {nullptr, nullptr};
+    double* aux = nullptr;
+    double* scal = nullptr;        // [0] time [1] dt [2] dt_next
+    int64_t* steps = nullptr;
+    unsigned long long* red = nullptr; // [0..1] dt maxima, [2] error, [3] pupt min
+    void* cub_tmp = nullptr;
+    size_t cub_bytes = 0;
+    int cur = 0;      // which persistent copy is current
+    int gcur = 0;     // grid holding the deposit of the current state
+    double Vvac_max = 0.0;
+    double m00 = 0.0;
+    int blocksize = 64;
+    // timing: an event pair around every profiled launch, by kernel class
+    int profile = 0;
+    struct Mark { int k; hipEvent_t a, b; };
+    std::vector<Mark> marks;
+};
+
+namespace {
+
+template <typename T> Arrays<T> arrays_of(emu_state* st, int copy)
+{
+    Arrays<T> A;
+    char* b = (char*)st->block[copy];
+    const int64_t np = st->np;
+    for (int d = 0; d < 3; d++) { A.pos[d] = (double*)b; b += np * sizeof(double); }
+    A.id = (int64_t*)b;
+    b += np * sizeof(int64_t);
+    for (int d = 0; d < 4; d++) { A.pup[d] = (T*)b; b += np * sizeof(T); }
+    for (int d = 0; d < 2; d++) { A.Nn[d] = (T*)b; b += np * sizeof(T); }
+    for (int d = 0; d < 2; d++) { A.Ll[d] = (T*)b; b += np * sizeof(T); }
+    A.f = (T*)b;
+    return A;
+}
+
+size_t persistent_bytes(int64_t np, int nh, size_t tsize)
+{
+    return (size_t)np * (3 * sizeof(double) + sizeof(int64_t) + (8 + 2 * nh) * tsize);
+}
+
+unsigned grid_blocks(int64_t n, int bs = 256)
+{
+    return (unsigned)std::min<int64_t>(65535 * 16, std::max<int64_t>(1, (n + bs - 1) / bs));
+}
+
+int ensure_cub(emu_state* st, size_t need)
+{
+    if (need > st->cub_bytes) {
+        if (st->cub_tmp) (void)hipFree(st->cub_tmp);
+        st->cub_tmp = nullptr;
+        st->cub_bytes = 0;
+        if (hipMalloc(&st->cub_tmp, need) != hipSuccess) return EMU_ERR_NOMEM;
+        st->cub_bytes = need;
+    }
+    return EMU_OK;
+}
+
+// out = exclusive scan of counts (ncell + 1 entries, counts[ncell] = 0)
+int scan_counts(emu_state* st, int* out)
+{
+    size_t need = 0;
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, need, st->counts, out, (int)(st->ncell + 1), st->ctx->stream);
+    if (ensure_cub(st, need)) return set_error(st->ctx, EMU_ERR_NOMEM, "scan scratch");
+    EMU_HIP(st->ctx, hipcub::DeviceScan::ExclusiveSum(st->cub_tmp, st->cub_bytes, st->counts, out,
+                                                     (int)(st->ncell + 1), st->ctx->stream));
+    return EMU_OK;
+}
+
+template <int NF, typename T, int WX, int WY, int WZ>
+void launch_stage_t(emu_state* st, StageArgs<T>& a)
+{
+    hipLaunchKernelGGL((k_stage<NF, T, WX, WY, WZ>), dim3((unsigned)st->ncell), dim3(st->blocksize), 0,
+                       st->ctx->stream, a);
+}
+
+template <int NF, typename T> void launch_stage_shape(emu_state* st, StageArgs<T>& a)
+{
+    const int* o = st->ctx->g.order;
+    const int wx = o[0] > 0 ? 3 : 1, wy = o[1] > 0 ? 3 : 1, wz = o[2] > 0 ? 3 : 1;
+    const int key = (wx == 3) * 4 + (wy == 3) * 2 + (wz == 3);
+    switch (key) {
+    case 0: launch_stage_t<NF, T, 1, 1, 1>(st, a); break;
+    case 1: launch_stage_t<NF, T, 1, 1, 3>(st, a); break;
+    case 2: launch_stage_t<NF, T, 1, 3, 1>(st, a); break;
+    case 3: launch_stage_t<NF, T, 1, 3, 3>(st, a); break;
+    case 4: launch_stage_t<NF, T, 3, 1, 1>(st, a); break;
+    case 5: launch_stage_t<NF, T, 3, 1, 3>(st, a); break;
+    case 6: launch_stage_t<NF, T, 3, 3, 1>(st, a); break;
+    default: launch_stage_t<NF, T, 3, 3, 3>(st, a); break;
+    }
+}
+
+template <typename T> StageArgs<T> stage_args(emu_state* st, int stage, const double* gin, double* gout, bool want_aux)
+{
+    StageArgs<T> a;
+    Arrays<T> A = arrays_of<T>(st, st->cur);
+    for (int d = 0; d < 3; d++) a.pos[d] = A.pos[d];
+    for (int d = 0; d < 4; d++) a.pup[d] = A.pup[d];
+    for (int d = 0; d < 2; d++) { a.Nn[d] = A.Nn[d]; a.Ll[d] = A.Ll[d]; }
+    a.f = A.f;
+    a.id = A.id;
+    a.Y = (T*)st->Y;
+    a.acc = (T*)st->acc;
+    a.newcell = st->newcell;
+    a.rank = st->rank;
+    a.cell_start = st->cell_start;
+    a.gin = gin;
+    a.gout = gout;
+    a.aux = st->aux;
+    a.dt = st->scal + 1;
+    a.err = st->red + 2;
+    a.np = st->np;
+    a.stage = stage;
+    a.want_aux = want_aux ? 1 : 0;
+    a.g = st->ctx->g;
+    const emu_ctx* ctx = st->ctx;
+    for (int c = 0; c < 9; c++) a.vac[c] = c < st->nh ? ctx->M2[c] * phys::c4 / 2.0 / phys::hbar : 0.0;
+    a.coef = M_SQRT2 * phys::GF * ctx->g.inv_cell_volume / phys::hbar;
+    a.maxError = ctx->p.maxError;
+    return a;
+}
+
+void launch_stage(emu_state* st, int stage, const double* gin, double* gout, bool want_aux)
+{
+    if (st->precision == EMU_FP32_BULK) {
+        auto a = stage_args<float>(st, stage, gin, gout, want_aux);
+        if (st->nf == 2) launch_stage_shape<2, float>(st, a);
+        else launch_stage_shape<3, float>(st, a);
+    } else {
+        auto a = stage_args<double>(st, stage, gin, gout, want_aux);
+        if (st->nf == 2) launch_stage_shape<2, double>(st, a);
+        else launch_stage_shape<3, double>(st, a);
+    }
+}
+
+void fill_grid(emu_state* st, double* g, bool with_aux)
+{
+    hipLaunchKernelGGL(k_fill_grid, dim3(grid_blocks(st->ncell * st->QG)), dim3(256), 0, st->ctx->stream, g,
+                       st->ncell, st->QG, st->m00, with_aux ? st->aux : nullptr, st->naux);
+}
+
+int dt_from_grid(emu_state* st, const double* g, double fixed_dt)
+{
+    emu_ctx* ctx = st->ctx;
+    const emu_params& p = ctx->p;
+    hipLaunchKernelGGL(k_fill_u64v, dim3(1), dim3(64), 0, ctx->stream, st->red, 2, 0ull);
+    const bool use_flavor = p.flavor_cfl_factor > 0.0;
+    if (use_flavor && fixed_dt <= 0) {
+        if (st->nf == 2)
+            hipLaunchKernelGGL(k_dt_reduce<2>, dim3(grid_blocks(st->ncell)), dim3(256), 0, ctx->stream, g, st->aux,
+                               st->ncell, ctx->g.cell_volume, st->m00, st->red);
+        else
+            hipLaunchKernelGGL(k_dt_reduce<3>, dim3(grid_blocks(st->ncell)), dim3(256), 0, ctx->stream, g, st->aux,
+                               st->ncell, ctx->g.cell_volume, st->m00, st->red);
+    }
+    DtParams dp;
+    dp.dt_translation = p.cfl_factor > 0.0
+                            ? std::min(std::min(ctx->g.dx[0], ctx->g.dx[1]), ctx->g.dx[2]) / phys::c * p.cfl_factor
+                            : 0.0;
+    dp.flavor_cfl = p.flavor_cfl_factor;
+    dp.max_adaptive_speedup = p.max_adaptive_speedup;
+    dp.Vvac_max = st->Vvac_max;
+    dp.fixed_dt = fixed_dt;
+    dp.use_flavor = use_flavor;
+    hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(64), 0, ctx->stream, st->red, dp, st->scal + 2, st->red + 2);
+    EMU_LAUNCH_CHECK(ctx, "compute_dt");
+    return EMU_OK;
+}
+
+// Kernel classes: 0 stage kernels, 1 grid fills, 2 sort, 3 dt reduction.
+void ev_record(emu_state* st, int k, bool start)
+{
+    if (!st->profile) return;
+    hipEvent_t e;
+    (void)hipEventCreate(&e);
+    (void)hipEventRecord(e, st->ctx->stream);
+    if (start) st->marks.push_back({k, e, nullptr});
+    else st->marks.back().b = e;
+}
+
+} // namespace
+
+extern "C" int emu_state_destroy(emu_state* st);
+
+extern "C" int emu_state_create(emu_ctx* ctx, int64_t np, int precision, emu_state** out)
+{
+    if (!ctx || !out || np < 0 || (precision != EMU_PLAIN_FP64 && precision != EMU_FP32_BULK)) return EMU_ERR_ARG;
+    if (np > (int64_t)INT32_MAX) return set_error(ctx, EMU_ERR_UNSUPPORTED, "more than 2^31-1 particles per state");
+    *out = nullptr;
+    auto* st = new emu_state();
+    st->ctx = ctx;
+    st->np = np;
+    st->precision = precision;
+    st->nf = ctx->p.nf;
+    st->nh = st->nf * st->nf;
+    st->QG = 4 * st->nh;
+    st->naux = 2 * st->nf + 3 * (st->nf * (st->nf - 1) / 2);
+    st->ncell = (int64_t)ctx->g.n[0] * ctx->g.n[1] * ctx->g.n[2];
+    st->tsize = precision == EMU_FP32_BULK ? sizeof(float) : sizeof(double);
+    st->m00 = ctx->p.rho * ctx->p.Ye / phys::Mp / ctx->g.inv_cell_volume;
+    const int64_t npa = std::max<int64_t>(np, 1);
+    const size_t pb = persistent_bytes(npa, st->nh, st->tsize);
+    bool ok = hipMalloc(&st->block[0], pb) == hipSuccess && hipMalloc(&st->block[1], pb) == hipSuccess &&
+              hipMalloc(&st->Y, (size_t)npa * 2 * st->nh * st->tsize) == hipSuccess &&
+              hipMalloc(&st->acc, (size_t)npa * 2 * st->nh * st->tsize) == hipSuccess &&
+              hipMalloc(&st->newcell, (size_t)npa * sizeof(int)) == hipSuccess &&
+              hipMalloc(&st->rank, (size_t)npa * sizeof(int)) == hipSuccess &&
+              hipMalloc(&st->counts, (size_t)(st->ncell + 1) * sizeof(int)) == hipSuccess &&
+              hipMalloc(&st->cell_start, (size_t)(st->ncell + 1) * sizeof(int)) == hipSuccess &&
+              hipMalloc(&st->new_start, (size_t)(st->ncell + 1) * sizeof(int)) == hipSuccess &&
+              hipMalloc(&st->mcount, (size_t)st->ncell * 27 * sizeof(int)) == hipSuccess &&
+              hipMalloc(&st->mbase, (size_t)st->ncell * 27 * sizeof(int)) == hipSuccess &&
+              hipMalloc(&st->mv, (size_t)npa) == hipSuccess &&
+              hipMalloc(&st->grid[0], (size_t)st->ncell * st->QG * sizeof(double)) == hipSuccess &&
+              hipMalloc(&st->grid[1], (size_t)st->ncell * st->QG * sizeof(double)) == hipSuccess &&
+              hipMalloc(&st->aux, (size_t)st->ncell * st->naux * sizeof(double)) == hipSuccess &&
+              hipMalloc(&st->scal, 4 * sizeof(double)) == hipSuccess &&
+              hipMalloc(&st->steps, sizeof(int64_t)) == hipSuccess &&
+              hipMalloc(&st->red, 8 * sizeof(unsigned long long)) == hipSuccess;
+    if (!ok) {
+        emu_state_destroy(st);
+        return set_error(ctx, EMU_ERR_NOMEM, "state allocation failed");
+    }
+    // waves per block from the average number of particles per cell
+    const double ppc = st->ncell > 0 ? (double)np / (double)st->ncell : 0.0;
+    st->blocksize = ppc > 192 ? 256 : ppc > 128 ? 192 : ppc > 64 ? 128 : 64;
+    *out = st;
+    return EMU_OK;
+}
+
+extern "C" int emu_state_destroy(emu_state* st)
+{
+    if (!st) return EMU_OK;
+    void* ptrs[] = {st->block[0], st->block[1], st->Y, st->acc, st->newcell, st->rank, st->counts, st->cell_start,
+                    st->new_start, st->mcount, st->mbase, st->mv, st->grid[0], st->grid[1], st->aux, st->scal,
+                    st->steps, st->red, st->cub_tmp};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    for (auto& m : st->marks) {
+        (void)hipEventDestroy(m.a);
+        if (m.b) (void)hipEventDestroy(m.b);
+    }
+    delete st;
+    return EMU_OK;
+}
+
+template <typename T> static int load_t(emu_state* st, const PView& P, const int64_t* ids)
+{
+    emu_ctx* ctx = st->ctx;
+    hipStream_t s = ctx->stream;
+    const int64_t np = st->np;
+    hipLaunchKernelGGL(k_fill_int, dim3(grid_blocks(st->ncell + 1)), dim3(256), 0, s, st->counts, st->ncell + 1, 0);
+    if (np > 0)
+        hipLaunchKernelGGL(k_bin_input, dim3(grid_blocks(np)), dim3(256), 0, s, P, ctx->g, st->newcell, st->counts);
+    int e = scan_counts(st, st->cell_start);
+    if (e) return e;
+    if (np == 0) return EMU_OK;
+    // stable radix sort of (cell, input index): sorted slot j <- input order[j]
+    int *keys_out = nullptr, *order = nullptr;
+    if (hipMalloc(&keys_out, np * sizeof(int)) != hipSuccess || hipMalloc(&order, np * sizeof(int)) != hipSuccess) {
+        if (keys_out) (void)hipFree(keys_out);
+        return set_error(ctx, EMU_ERR_NOMEM, "load sort scratch");
+    }
+    hipLaunchKernelGGL(k_iota, dim3(grid_blocks(np)), dim3(256), 0, s, st->rank, np);
+    int end_bit = 1;
+    while ((1ll << end_bit) < st->ncell) end_bit++;
+    size_t need = 0;
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, need, st->newcell, keys_out, st->rank, order, (int)np, 0, end_bit,
+                                             s);
+    if (ensure_cub(st, need)) e = set_error(ctx, EMU_ERR_NOMEM, "radix sort scratch");
+    if (!e && hipcub::DeviceRadixSort::SortPairs(st->cub_tmp, st->cub_bytes, st->newcell, keys_out, st->rank, order,
+                                                 (int)np, 0, end_bit, s) != hipSuccess)
+        e = set_error(ctx, EMU_ERR_HIP, "radix sort");
+    if (!e) {
+        Arrays<T> A = arrays_of<T>(st, st->cur);
+        if (st->nf == 2)
+            hipLaunchKernelGGL((k_load<2, T>), dim3(grid_blocks(np)), dim3(256), 0, s, P, ids, order, A, np);
+        else
+            hipLaunchKernelGGL((k_load<3, T>), dim3(grid_blocks(np)), dim3(256), 0, s, P, ids, order, A, np);
+        if (hipGetLastError() != hipSuccess) e = set_error(ctx, EMU_ERR_HIP, "state_load");
+    }
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(keys_out);
+    (void)hipFree(order);
+    return e;
+}
+
+extern "C" int emu_state_load(emu_state* st, const emu_particles* parts, const int64_t* ids)
+{
+    if (!st || !parts || parts->np != st->np) return EMU_ERR_ARG;
+    emu_ctx* ctx = st->ctx;
+    PView P;
+    P.np = parts->np;
+    const int na = 12 + 2 * st->nh;
+    for (int d = 0; d < 3; d++) P.pos[d] = parts->pos[d];
+    for (int k = 0; k < EMU_MAX_ATTRIBS; k++) P.r[k] = k < na ? parts->rdata[k] : nullptr;
+    int e = st->precision == EMU_FP32_BULK ? load_t<float>(st, P, ids) : load_t<double>(st, P, ids);
+    if (e) return e;
+    hipStream_t s = ctx->stream;
+    // time of the particles (all particles share it; main.cpp:101-110)
+    double t0 = 0.0;
+    if (st->np > 0) EMU_HIP(ctx, hipMemcpyAsync(&t0, parts->rdata[PIdx<2>::time], sizeof(double), hipMemcpyDeviceToHost, s));
+    // Vvac_max from the minimum energy (FlavoredNeutrinoContainerInit.cpp:657-660)
+    hipLaunchKernelGGL(k_fill_u64v, dim3(1), dim3(64), 0, s, st->red, 4, ~0ull >> 1);
+    if (st->np > 0)
+        hipLaunchKernelGGL(k_min_pupt_t, dim3(grid_blocks(st->np)), dim3(256), 0, s, parts->rdata[PIdx<2>::pupt], st->np,
+                           st->red + 3);
+    unsigned long long h;
+    EMU_HIP(ctx, hipMemcpyAsync(&h, st->red + 3, 8, hipMemcpyDeviceToHost, s));
+    EMU_HIP(ctx, hipStreamSynchronize(s));
+    double pupt_min;
+    std::memcpy(&pupt_min, &h, 8);
+    double d[9] = {0};
+    if (st->nf == 2) {
+        for (int i = 0; i < 2; i++) d[herm_re<2>(i, i)] = ctx->p.mass[i] * ctx->p.mass[i];
+        st->Vvac_max = std::sqrt(su_mag2<2, double>(d)) * phys::c4 / pupt_min;
+    } else {
+        for (int i = 0; i < 3; i++) d[herm_re<3>(i, i)] = ctx->p.mass[i] * ctx->p.mass[i];
+        st->Vvac_max = std::sqrt(su_mag2<3, double>(d)) * phys::c4 / pupt_min;
+    }
+    double sc[4] = {t0, 0.0, 0.0, 0.0};
+    EMU_HIP(ctx, hipMemcpyAsync(st->scal, sc, sizeof sc, hipMemcpyHostToDevice, s));
+    EMU_HIP(ctx, hipMemsetAsync(st->steps, 0, sizeof(int64_t), s));
+    hipLaunchKernelGGL(k_fill_u64v, dim3(1), dim3(64), 0, s, st->red + 2, 1, ~0ull);
+    // initial deposit (main.cpp:118) and starting dt (main.cpp:206)
+    st->gcur = 0;
+    fill_grid(st, st->grid[0], true);
+    launch_stage(st, 0, nullptr, st->grid[0], true);
+    e = dt_from_grid(st, st->grid[0], 0.0);
+    if (e) return e;
+    EMU_HIP(ctx, hipMemcpyAsync(st->scal + 1, st->scal + 2, sizeof(double), hipMemcpyDeviceToDevice, s));
+    EMU_HIP(ctx, hipStreamSynchronize(s));
+    return EMU_OK;
+}
+
+extern "C" int emu_state_store(emu_state* st, emu_particles* parts, int64_t* ids)
+{
+    if (!st || !parts || parts->np != st->np) return EMU_ERR_ARG;
+    emu_ctx* ctx = st->ctx;
+    PView P;
+    P.np = parts->np;
+    const int na = 12 + 2 * st->nh;
+    for (int d = 0; d < 3; d++) P.pos[d] = parts->pos[d];
+    for (int k = 0; k < EMU_MAX_ATTRIBS; k++) P.r[k] = k < na ? parts->rdata[k] : nullptr;
+    if (st->np > 0) {
+        if (st->precision == EMU_FP32_BULK) {
+            Arrays<float> A = arrays_of<float>(st, st->cur);
+            if (st->nf == 2) hipLaunchKernelGGL((k_store<2, float>), dim3(grid_blocks(st->np)), dim3(256), 0, ctx->stream, A, P, ids, st->scal, st->np);
+            else hipLaunchKernelGGL((k_store<3, float>), dim3(grid_blocks(st->np)), dim3(256), 0, ctx->stream, A, P, ids, st->scal, st->np);
+        } else {
+            Arrays<double> A = arrays_of<double>(st, st->cur);
+            if (st->nf == 2) hipLaunchKernelGGL((k_store<2, double>), dim3(grid_blocks(st->np)), dim3(256), 0, ctx->stream, A, P, ids, st->scal, st->np);
+            else hipLaunchKernelGGL((k_store<3, double>), dim3(grid_blocks(st->np)), dim3(256), 0, ctx->stream, A, P, ids, st->scal, st->np);
+        }
+    }
+    EMU_LAUNCH_CHECK(ctx, "state_store");
+    EMU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return EMU_OK;
+}
+
+template <typename T> static void scatter_t(emu_state* st)
+{
+    Arrays<T> src = arrays_of<T>(st, st->cur), dst = arrays_of<T>(st, 1 - st->cur);
+    hipLaunchKernelGGL(k_scatter<T>, dim3((unsigned)st->ncell), dim3(64), 0, st->ctx->stream, src, dst,
+                       st->cell_start, st->mbase, st->rank, st->mv, st->np, 2 * st->nh, st->red + 2);
+}
+
+extern "C" int emu_state_evolve(emu_state* st, int nsteps, double fixed_dt)
+{
+    if (!st || nsteps < 0) return EMU_ERR_ARG;
+    emu_ctx* ctx = st->ctx;
+    hipStream_t s = ctx->stream;
+    const Geom& g = ctx->g;
+    hipLaunchKernelGGL(k_fill_int, dim3(grid_blocks(st->ncell + 1)), dim3(256), 0, s, st->counts, st->ncell + 1, 0);
+    // the deterministic sort assumes moves of at most one cell per step
+    if (ctx->p.cfl_factor <= 0.0 || ctx->p.cfl_factor >= 1.0)
+        if (fixed_dt <= 0 || fixed_dt * phys::c >= std::min(std::min(g.dx[0], g.dx[1]), g.dx[2]))
+            return set_error(ctx, EMU_ERR_MOVE, "fused step needs c*dt < dx (0 < cfl_factor < 1)");
+    if (fixed_dt > 0) EMU_HIP(ctx, hipMemcpyAsync(st->scal + 1, &fixed_dt, sizeof(double), hipMemcpyHostToDevice, s));
+    for (int step = 0; step < nsteps; step++) {
+        double* A = st->grid[st->gcur];
+        double* B = st->grid[1 - st->gcur];
+        const bool want_aux = ctx->p.flavor_cfl_factor > 0.0 && fixed_dt <= 0;
+        ev_record(st, 1, true);
+        fill_grid(st, B, false);
+        ev_record(st, 1, false);
+        ev_record(st, 0, true);
+        launch_stage(st, 1, A, B, false);
+        ev_record(st, 0, false);
+        ev_record(st, 1, true);
+        fill_grid(st, A, false);
+        ev_record(st, 1, false);
+        ev_record(st, 0, true);
+        launch_stage(st, 2, B, A, false);
+        ev_record(st, 0, false);
+        ev_record(st, 1, true);
+        fill_grid(st, B, want_aux);
+        ev_record(st, 1, false);
+        ev_record(st, 0, true);
+        launch_stage(st, 3, A, B, want_aux);
+        ev_record(st, 0, false);
+        ev_record(st, 3, true);
+        int e = dt_from_grid(st, B, fixed_dt);
+        ev_record(st, 3, false);
+        if (e) return e;
+        ev_record(st, 1, true);
+        fill_grid(st, A, false);
+        ev_record(st, 1, false);
+        ev_record(st, 0, true);
+        launch_stage(st, 4, B, A, false);
+        ev_record(st, 0, false);
+        // stable counting sort by the new cell
+        ev_record(st, 2, true);
+        hipLaunchKernelGGL(k_move_ranks, dim3((unsigned)st->ncell), dim3(64), 0, s, st->newcell, st->cell_start,
+                           ctx->g, st->rank, st->mv, st->mcount, st->red + 2);
+        hipLaunchKernelGGL(k_new_counts, dim3(grid_blocks(st->ncell)), dim3(256), 0, s, st->mcount, ctx->g,
+                           st->ncell, st->counts);
+        e = scan_counts(st, st->new_start);
+        if (e) return e;
+        hipLaunchKernelGGL(k_bases, dim3(grid_blocks(st->ncell)), dim3(256), 0, s, st->mcount, st->new_start, ctx->g,
+                           st->ncell, st->mbase);
+        if (st->precision == EMU_FP32_BULK) scatter_t<float>(st);
+        else scatter_t<double>(st);
+        ev_record(st, 2, false);
+        st->cur = 1 - st->cur;
+        std::swap(st->cell_start, st->new_start);
+        // the stage-4 kernel deposited y_{n+1} into A for the next step
+        hipLaunchKernelGGL(k_step_end, dim3(1), dim3(64), 0, s, st->scal, st->scal + 1, st->scal + 2, st->steps);
+        if (fixed_dt > 0) EMU_HIP(ctx, hipMemcpyAsync(st->scal + 1, &fixed_dt, sizeof(double), hipMemcpyHostToDevice, s));
+        EMU_LAUNCH_CHECK(ctx, "state_evolve");
+    }
+    return EMU_OK;
+}
+
+extern "C" int emu_state_info(emu_state* st, double* time, double* dt, int64_t* steps, int64_t* bad_index)
+{
+    if (!st) return EMU_ERR_ARG;
+    emu_ctx* ctx = st->ctx;
+    double sc[4];
+    int64_t n = 0;
+    unsigned long long e = 0;
+    EMU_HIP(ctx, hipMemcpyAsync(sc, st->scal, sizeof sc, hipMemcpyDeviceToHost, ctx->stream));
+    EMU_HIP(ctx, hipMemcpyAsync(&n, st->steps, sizeof n, hipMemcpyDeviceToHost, ctx->stream));
+    EMU_HIP(ctx, hipMemcpyAsync(&e, st->red + 2, sizeof e, hipMemcpyDeviceToHost, ctx->stream));
+    EMU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (time) *time = sc[0];
+    if (dt) *dt = sc[1];
+    if (steps) *steps = n;
+    if (e == ~0ull) {
+        if (bad_index) *bad_index = -1;
+        return EMU_OK;
+    }
+    if (bad_index) *bad_index = (e >> 3) == (~0ull >> 3) ? -1 : (int64_t)(e >> 3);
+    return (int)(e & 7);
+}
+
+
+extern "C" int emu_state_kernel_ms(emu_state* st, int k, double* avg_ms, int64_t* launches)
+{
+    if (!st || k < 0 || k > 3) return EMU_ERR_ARG;
+    emu_ctx* ctx = st->ctx;
+    EMU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    double tot = 0.0;
+    int64_t n = 0;
+    for (auto& m : st->marks) {
+        if (m.k != k || !m.b) continue;
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, m.a, m.b) == hipSuccess) {
+            tot += ms;
+            n++;
+        }
+    }
+    if (avg_ms) *avg_ms = n ? tot / n : 0.0;
+    if (launches) *launches = n;
+    return EMU_OK;
+}
+
+extern "C" int emu_state_profile(emu_state* st, int enable)
+{
+    if (!st) return EMU_ERR_ARG;
+    (void)hipStreamSynchronize(st->ctx->stream);
+    for (auto& m : st->marks) {
+        (void)hipEventDestroy(m.a);
+        if (m.b) (void)hipEventDestroy(m.b);
+    }
+    st->marks.clear();
+    st->profile = enable;
+    return EMU_OK;
+}
+
+template <typename T> static void diag_t(emu_state* st, double* g4)
+{
+    Arrays<T> A = arrays_of<T>(st, st->cur);
+    if (st->nf == 2)
+        hipLaunchKernelGGL((k_diag_deposit<2, T>), dim3((unsigned)st->ncell), dim3(64), 0, st->ctx->stream, A,
+                           st->cell_start, st->ctx->g, st->np, g4);
+    else
+        hipLaunchKernelGGL((k_diag_deposit<3, T>), dim3((unsigned)st->ncell), dim3(64), 0, st->ctx->stream, A,
+                           st->cell_start, st->ctx->g, st->np, g4);
+}
+
+extern "C" int emu_state_flavor_amplitude(emu_state* st, double* sums)
+{
+    if (!st || !sums) return EMU_ERR_ARG;
+    emu_ctx* ctx = st->ctx;
+    hipStream_t s = ctx->stream;
+    // the auxiliary grid (>= 4 values per cell) is free between steps
+    hipLaunchKernelGGL(k_fill_grid, dim3(grid_blocks(st->ncell * st->naux)), dim3(256), 0, s, st->aux,
+                       st->ncell * st->naux, 1, 0.0, nullptr, 0);
+    if (st->precision == EMU_FP32_BULK) diag_t<float>(st, st->aux);
+    else diag_t<double>(st, st->aux);
+    double* dsum = st->scal + 3; // one free slot; a second comes from red[4] reinterpreted
+    double* dsum2 = reinterpret_cast<double*>(st->red + 4);
+    (void)dsum2;
+    hipLaunchKernelGGL(k_fill_u64v, dim3(1), dim3(64), 0, s, st->red + 4, 2, 0ull);
+    hipLaunchKernelGGL(k_diag_reduce, dim3(grid_blocks(st->ncell)), dim3(256), 0, s, st->aux, st->ncell,
+                       reinterpret_cast<double*>(st->red + 4));
+    (void)dsum;
+    EMU_LAUNCH_CHECK(ctx, "flavor_amplitude");
+    double h[2];
+    EMU_HIP(ctx, hipMemcpyAsync(h, st->red + 4, sizeof h, hipMemcpyDeviceToHost, s));
+    EMU_HIP(ctx, hipStreamSynchronize(s));
+    sums[0] = h[0];
+    sums[1] = h[1];
+    return EMU_OK;
+}

@@ -415,6 +415,11 @@ def init_particles(deck: Deck) -> np.ndarray:
             setf("f02_Im", a * r2)
             setf("f02_Re", a * r3, True)
             setf("f02_Im", a * r4, True)
+        if deck.st3_amplitude != 0.0:  # single mode on top of the noise
+            k = (2. * math.pi) / (L[2] / float(deck.st3_wavelength_fraction_of_domain))
+            v = deck.st3_amplitude * np.sin(k * p["pos"][:, 2])
+            rd[:, K["f01_Re"]] += v
+            rd[:, K["f01_Rebar"]] += v
         fh = np.array([math.cos(deck.st4_phi) * math.sin(deck.st4_theta),
                        math.sin(deck.st4_phi) * math.sin(deck.st4_theta), math.cos(deck.st4_theta)])
         fhb = np.array([math.cos(deck.st4_phibar) * math.sin(deck.st4_thetabar),

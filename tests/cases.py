@@ -1,0 +1,52 @@
+"""Shared helpers: one parameter dict drives both the oracle (pyoracle.Deck)
+and the product (emu_amd TestParams), so both see the same inputs."""
+import math
+
+import numpy as np
+
+import kat
+
+
+def fast_flavor_dict(nf=3, ncell=(4, 4, 4), L=None, directions="sphere", nphi=4, **kw):
+    L = L if L is not None else tuple(0.5 * n for n in ncell)
+    d = dict(nf=nf, simulation_type=4, ncell=tuple(ncell), Lx=L[0], Ly=L[1], Lz=L[2], nphi_equator=nphi,
+             directions=directions, st4_ndens=4.89e32, st4_ndensbar=4.89e32, st4_fluxfac=0.0,
+             st4_fluxfacbar=0.33, st4_theta=0.0, st4_thetabar=0.0, st4_amplitude=1e-6,
+             cfl_factor=0.4, flavor_cfl_factor=0.4, maxError=1e-6, rho_g_ccm=0.0, Ye=1.0,
+             mass1_eV=0.0, mass2_eV=math.sqrt(7.4e-5), mass3_eV=math.sqrt(2.5e-3),
+             theta12_degrees=33.82, theta13_degrees=48.3, theta23_degrees=8.61, deltaCP_degrees=222.0, seed=99)
+    if nf == 2:
+        d.update(mass1_eV=-math.sqrt(7.4e-5), mass2_eV=0.0, mass3_eV=0.0)
+    d.update(kw)
+    return d
+
+
+def deck_dict(name, nf=2):
+    d = kat.reference_deck(name)
+    d = {k: (tuple(v) if isinstance(v, list) else v) for k, v in d.items()}
+    d["nf"] = nf
+    return d
+
+
+def oracle_deck(oracle, d):
+    fields = oracle.Deck.__dataclass_fields__
+    return oracle.Deck(**{k: v for k, v in d.items() if k in fields})
+
+
+def product_params(d):
+    from emu_amd.params import TestParams
+    return TestParams.from_dict(d)
+
+
+def by_id(arr):
+    return arr[np.argsort(arr["id"], kind="stable")]
+
+
+def max_rel_err(a, b):
+    """max over components of |a - b| / max|b| (per-component normalisation:
+    per-element relative error is ill-defined for cancelled fluxes)."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    scale = np.abs(b).max(axis=0)
+    scale = np.where(scale > 0, scale, 1.0)
+    return float((np.abs(a - b) / scale).max())

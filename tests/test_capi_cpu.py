@@ -1,0 +1,66 @@
+"""CPU-side checks of the product: the C ABI library loads and exports every
+function declared in include/*.h; host-side parameter handling."""
+import ctypes
+import glob
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?(?:int|int64_t|void|char)\s*\*?\s*(emu_\w+)\s*\(", text, flags=re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    from emu_amd import capi
+    lib = capi.lib()
+    declared = declared_functions()
+    assert len(declared) >= 25
+    missing = [n for n in sorted(declared) if not hasattr(lib, n)]
+    assert not missing, missing
+    # the Python binding covers every declaration
+    assert declared == set(capi.EXPORTED), declared ^ set(capi.EXPORTED)
+
+
+def test_struct_layouts_match_header():
+    """ctypes mirrors of the public structs have the C layout (sizes from
+    the field lists in include/emu_amd.h and include/emu_init.h)."""
+    from emu_amd import capi
+    assert ctypes.sizeof(capi.EmuParams) == 4 * 5 + 4 + 8 * (3 + 3 + 3 + 3 + 6 + 4)  # 5 ints (+pad) then doubles
+    assert ctypes.sizeof(capi.EmuParticles) == 8 + 8 * 3 + 8 * capi.MAX_ATTRIBS
+    assert ctypes.sizeof(capi.EmuGrid) == 8 + 4 * 7 + 4
+
+
+def test_deck_parsing_and_units(tmp_path):
+    from emu_amd.params import TestParams
+    deck = tmp_path / "inputs"
+    deck.write_text("simulation_type = 3\nncell = (1, 1, 50)\nLz = 1.0 # cm\nmass1_eV = -0.008596511\n"
+                    "theta12_degrees = 33.82\nst3_amplitude = 1e-6\nintegration.rk.type = 4\n")
+    p = TestParams.from_file(str(deck), nf=2)
+    assert p.simulation_type == 3 and p.ncell == (1, 1, 50) and p.st3_amplitude == 1e-6
+    assert p.extra["integration.rk.type"] == 4
+    e = p.emu_params()
+    # Parameters.H:81-85: mass *= eV/c^2, theta *= pi/180
+    assert e.mass[0] == -0.008596511 * (1.60218e-12 / (2.99792458e10 * 2.99792458e10))
+    assert e.theta12 == 33.82 * (3.141592653589793 / 180.)
+
+
+def test_workloads_particle_counts():
+    from emu_amd import configs
+    import emu_amd.capi as capi
+    lib = capi.lib()
+    expect = {"cpu16": 106496, "ff64": 24117248, "fiducial1d": 387072, "adjoint32": 3014656,
+              "large128": 192937984}
+    for name, n in expect.items():
+        p = configs.WORKLOADS[name]()
+        e, c = p.emu_params(), p.init_config()
+        assert lib.emu_init_count(ctypes.byref(e), ctypes.byref(c)) == n, name

@@ -1,0 +1,297 @@
+"""Parity of the HIP paths with the FP64 oracle on identical inputs.
+
+Bars: cell indices and positions bit-exact where the arithmetic allows it;
+FP64 fields within 1e-12 relative to each component's max-abs (per-element
+relative error is ill-defined for cancelled fluxes); FP32-bulk within the
+tolerance stated in each test.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import cases
+import kat
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def core():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    from emu_amd import core as c
+    return c
+
+
+def random_case(oracle, nf, ncell, L, n, seed):
+    """Random positions, directions, energies, weights and flavour states."""
+    rng = np.random.default_rng(seed)
+    parts = np.zeros(n, dtype=oracle.particle_dtype(nf))
+    K = {nm: i for i, nm in enumerate(oracle.attribute_names(nf))}
+    pos = rng.uniform(0, 1, size=(n, 3)) * np.array(L)
+    parts["pos"] = pos
+    rd = parts["rdata"]
+    rd[:, 1:4] = pos
+    u = rng.normal(size=(n, 3))
+    u /= np.linalg.norm(u, axis=1)[:, None]
+    E = 50e6 * oracle.eV * rng.uniform(0.5, 2.0, size=n)
+    rd[:, K["pupt"]] = E
+    rd[:, K["pupx"]], rd[:, K["pupy"]], rd[:, K["pupz"]] = u[:, 0] * E, u[:, 1] * E, u[:, 2] * E
+    rd[:, K["N"]] = rng.uniform(1e30, 2e30, n)
+    rd[:, K["Nbar"]] = rng.uniform(1e30, 2e30, n)
+    nh = nf * nf
+    for t in ("", "bar"):
+        k0 = K["f00_Re" + t]
+        rd[:, k0:k0 + nh] = rng.normal(size=(n, nh)) * 0.1
+    parts["id"] = np.arange(1, n + 1)
+    return parts, K
+
+
+def make(core, oracle, d):
+    deck = cases.oracle_deck(oracle, d)
+    emu = core.Emu(cases.product_params(d))
+    return deck, deck.params(), emu
+
+
+# ---------------------------------------------------------------------------
+# per-function entry points
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("nf", [2, 3])
+def test_init_matches_oracle(core, oracle, nf):
+    d = cases.fast_flavor_dict(nf=nf, ncell=(3, 4, 5), nphi=8, st3_amplitude=1e-5)
+    deck, prm, emu = make(core, oracle, d)
+    want = oracle.init_particles(deck)
+    got = core.FlavoredNeutrinoContainer.init_particles(emu).to_aos()
+    assert len(got) == len(want)
+    np.testing.assert_array_equal(got["id"], want["id"])
+    np.testing.assert_array_equal(got["pos"], want["pos"])
+    assert cases.max_rel_err(got["rdata"], want["rdata"]) < 1e-14
+
+
+@pytest.mark.parametrize("nf", [2, 3])
+@pytest.mark.parametrize("ncell", [(4, 3, 5), (1, 1, 6), (2, 2, 2)])
+def test_deposit_fill_interpolate_parity(core, oracle, nf, ncell):
+    L = tuple(0.5 * n for n in ncell)
+    d = cases.fast_flavor_dict(nf=nf, ncell=ncell, L=L, rho_g_ccm=1e10, Ye=0.45)
+    deck, prm, emu = make(core, oracle, d)
+    parts, K = random_case(oracle, nf, ncell, L, 300, seed=nf * 100 + ncell[2])
+    g_or = oracle.Grid(nf, prm)
+    oracle.deposit_to_mesh(nf, prm, parts, g_or)
+    c = core.FlavoredNeutrinoContainer.from_aos(emu, parts)
+    state = core.MultiFab(emu)
+    core.deposit_to_mesh(c, state)
+    fab = state.data.cpu().numpy().reshape(emu.ncomp, -1).T
+    want = g_or.fab().reshape(emu.ncomp, -1).T
+    # deposit components over valid and ghost cells (ghosts keep SumBoundary partials)
+    assert cases.max_rel_err(fab[:, 3:], want[:, 3:]) < 1e-12
+    oracle.fill_boundary(nf, prm, g_or)
+    state.FillBoundary()
+    fab = state.data.cpu().numpy().reshape(emu.ncomp, -1).T
+    want = g_or.fab().reshape(emu.ncomp, -1).T
+    assert cases.max_rel_err(fab, want) < 1e-12
+    # interpolate_rhs_from_mesh on the oracle's own (filled) grid
+    state.data.copy_(__import__("torch").from_numpy(g_or.fab().copy()))
+    rhs_or = parts.copy()
+    oracle.interpolate_rhs_from_mesh(nf, prm, rhs_or, g_or)
+    rhs = core.FlavoredNeutrinoContainer.from_aos(emu, parts)
+    core.interpolate_rhs_from_mesh(rhs, state)
+    got = rhs.to_aos()["rdata"]
+    assert cases.max_rel_err(got, rhs_or["rdata"]) < 1e-12
+    # compute_dt over the fab box
+    vv = oracle.vvac_max(nf, prm, parts)
+    assert c.Vvac_max() == pytest.approx(vv, rel=1e-15)
+    err, dt_or = oracle.compute_dt(nf, prm, g_or, vv)
+    assert err == 0
+    assert core.compute_dt(state, vv) == pytest.approx(dt_or, rel=1e-14)
+
+
+@pytest.mark.parametrize("nf", [2, 3])
+def test_renormalize_sync_wrap_apply_parity(core, oracle, nf):
+    d = cases.fast_flavor_dict(nf=nf, ncell=(4, 4, 4), nphi=4, st4_amplitude=1e-3)
+    deck, prm, emu = make(core, oracle, d)
+    parts = oracle.init_particles(deck)
+    K = {nm: i for i, nm in enumerate(oracle.attribute_names(nf))}
+    nh = nf * nf
+    rng = np.random.default_rng(4)
+    for t in ("", "bar"):
+        k0 = K["f00_Re" + t]
+        parts["rdata"][:, k0:k0 + nh] *= 1 + rng.uniform(-3e-5, 3e-5, size=(len(parts), 1))
+    # positions pushed across the boundary for the wrap
+    parts["rdata"][:, K["x"]] += rng.choice([-2.5, 0.0, 2.5], size=len(parts))
+    c = core.FlavoredNeutrinoContainer.from_aos(emu, parts)
+    oracle.renormalize(nf, prm, parts)
+    c.Renormalize()
+    for tp in (0,):
+        oracle.sync_location(nf, parts, tp)
+        c.SyncLocation(tp)
+    oracle.redistribute(nf, prm, parts)
+    c.RedistributeLocal()
+    oracle.sync_location(nf, parts, 1)
+    c.SyncLocation(1)
+    got = c.to_aos()
+    np.testing.assert_array_equal(got["pos"], parts["pos"])
+    assert cases.max_rel_err(got["rdata"], parts["rdata"]) < 1e-15
+    # apply_rhs is a plain saxpy: bit-exact
+    dF = core.FlavoredNeutrinoContainer.from_aos(emu, parts)
+    oracle.apply_rhs(nf, parts, 1.7e-12, parts.copy())
+    c.apply_rhs(1.7e-12, dF)
+    np.testing.assert_array_equal(c.to_aos()["rdata"], parts["rdata"])
+    # error path: a trace error beyond 100*maxError reports the lowest index
+    bad = parts.copy()
+    bad["rdata"][7, K["f00_Re"]] += 1e-3
+    bad["rdata"][9, K["f00_Re"]] += 1e-3
+    cb = core.FlavoredNeutrinoContainer.from_aos(emu, bad)
+    with pytest.raises(Exception) as ei:
+        cb.Renormalize()
+    assert ei.value.code == 1 and ei.value.bad_index == 7
+
+
+# ---------------------------------------------------------------------------
+# whole steps
+# ---------------------------------------------------------------------------
+def oracle_steps(oracle, nf, prm, parts, nsteps, fixed_dt=None):
+    g = oracle.Grid(nf, prm)
+    vv = oracle.vvac_max(nf, prm, parts)
+    oracle.deposit_to_mesh(nf, prm, parts, g)
+    err, dt = oracle.compute_dt(nf, prm, g, vv)
+    assert err == 0
+    dts = []
+    for _ in range(nsteps):
+        h = fixed_dt if fixed_dt else dt
+        dts.append(h)
+        err, dt, bad = oracle.rk4_step(nf, prm, parts, g, h, vv)
+        assert err == 0
+    return dts, dt
+
+
+@pytest.mark.parametrize("nf", [2, 3])
+def test_integrator_parity(core, oracle, nf):
+    d = cases.fast_flavor_dict(nf=nf, ncell=(4, 4, 4), nphi=4)
+    deck, prm, emu = make(core, oracle, d)
+    parts = oracle.init_particles(deck)
+    c = core.FlavoredNeutrinoContainer.from_aos(emu, parts)
+    integ = core.Integrator(emu, c)
+    dts, dt_next = oracle_steps(oracle, nf, prm, parts, 4)
+    out = integ.integrate(4).to_aos()
+    assert integ.dt == pytest.approx(dt_next, rel=1e-13)
+    np.testing.assert_allclose(out["pos"], parts["pos"], rtol=1e-14, atol=0)
+    assert cases.max_rel_err(out["rdata"], parts["rdata"]) < 1e-12
+
+
+@pytest.mark.parametrize("nf", [2, 3])
+@pytest.mark.parametrize("ncell", [(4, 4, 4), (1, 1, 8)])
+def test_fused_fp64_parity(core, oracle, nf, ncell):
+    d = cases.fast_flavor_dict(nf=nf, ncell=ncell, nphi=8)
+    deck, prm, emu = make(core, oracle, d)
+    parts = oracle.init_particles(deck)
+    # a fixed dt large enough that particles change cells during the run
+    fixed = 0.3 * 0.5 / oracle.clight
+    c = core.FlavoredNeutrinoContainer.from_aos(emu, parts)
+    st = core.FusedState(emu, c.np, "fp64")
+    st.load(c)
+    st.evolve(6, fixed_dt=fixed)
+    st.store(c)
+    info = st.info()
+    oracle_steps(oracle, nf, prm, parts, 6, fixed_dt=fixed)
+    got = cases.by_id(c.to_aos())
+    want = cases.by_id(parts)
+    np.testing.assert_array_equal(got["pos"], want["pos"])  # positions and cell binning bit-exact
+    assert cases.max_rel_err(got["rdata"], want["rdata"]) < 1e-12
+    assert info["steps"] == 6
+    assert info["time"] == pytest.approx(want["rdata"][0, 0], rel=1e-15)
+
+
+@pytest.mark.parametrize("nf", [2, 3])
+def test_fused_fp64_computed_dt(core, oracle, nf):
+    d = cases.fast_flavor_dict(nf=nf, ncell=(4, 4, 4), nphi=4)
+    deck, prm, emu = make(core, oracle, d)
+    parts = oracle.init_particles(deck)
+    c = core.FlavoredNeutrinoContainer.from_aos(emu, parts)
+    st = core.FusedState(emu, c.np, "fp64")
+    st.load(c)
+    dt0 = st.info()["dt"]
+    dts, dt_next = oracle_steps(oracle, nf, prm, parts, 5)
+    assert dt0 == pytest.approx(dts[0], rel=1e-14)
+    st.evolve(5)
+    st.store(c)
+    assert st.info()["dt"] == pytest.approx(dt_next, rel=1e-13)
+    got = cases.by_id(c.to_aos())
+    want = cases.by_id(parts)
+    np.testing.assert_allclose(got["pos"], want["pos"], rtol=1e-14)
+    assert cases.max_rel_err(got["rdata"], want["rdata"]) < 1e-11
+
+
+@pytest.mark.parametrize("nf", [2, 3])
+def test_fused_fp32_bulk_tolerance(core, oracle, nf):
+    """FP32 bulk (FP64 positions, deposit accumulation, Renormalize, dt): fields
+    within 2e-5 of each component's max-abs after 5 steps."""
+    d = cases.fast_flavor_dict(nf=nf, ncell=(4, 4, 4), nphi=4)
+    deck, prm, emu = make(core, oracle, d)
+    parts = oracle.init_particles(deck)
+    c = core.FlavoredNeutrinoContainer.from_aos(emu, parts)
+    st = core.FusedState(emu, c.np, "fp32")
+    st.load(c)
+    dts, _ = oracle_steps(oracle, nf, prm, parts, 5)
+    st.evolve(5)
+    st.store(c)
+    got = cases.by_id(c.to_aos())
+    want = cases.by_id(parts)
+    np.testing.assert_allclose(got["pos"], want["pos"], rtol=1e-7)
+    assert cases.max_rel_err(got["rdata"], want["rdata"]) < 2e-5
+
+
+# ---------------------------------------------------------------------------
+# the reference's analytic acceptance tests on every HIP path
+# ---------------------------------------------------------------------------
+def run_kat(core, deckname, nsteps, path, keys, reduce=None):
+    d = cases.deck_dict(deckname)
+    emu = core.Emu(cases.product_params(d))
+    c = core.FlavoredNeutrinoContainer.init_particles(emu)
+    K = emu.K
+    snaps = []
+
+    def snap(arr):
+        r = arr["rdata"]
+        if reduce is None:
+            r0 = r[np.argmin(arr["id"])]
+            return [r0[K[k]] for k in keys]
+        return [r[0, K[keys[0]]]] + [np.abs(r[:, K[k]]).max() for k in keys[1:]]
+
+    snaps.append(snap(c.to_aos()))
+    if path == "integrator":
+        integ = core.Integrator(emu, c)
+        for _ in range(nsteps):
+            integ.advance()
+            snaps.append(snap(integ.old.to_aos()))
+    else:
+        st = core.FusedState(emu, c.np, path)
+        st.load(c)
+        for _ in range(nsteps):
+            st.evolve(1)
+            st.store(c)
+            snaps.append(snap(c.to_aos()))
+        st.info()
+    return np.array(snaps)
+
+
+@pytest.mark.parametrize("path", ["integrator", "fp64", "fp32"])
+def test_kat_fast_flavor_gpu(core, path):
+    s = run_kat(core, "inputs_fast_flavor", 100, path, ("time", "f01_Re", "f01_Im", "f01_Rebar", "f01_Imbar"))
+    errs = kat.fast_flavor_errors(*s.T)
+    assert max(errs) < 2e-2, errs
+
+
+@pytest.mark.parametrize("path", ["integrator", "fp64", "fp32"])
+def test_kat_fast_flavor_nonzerok_gpu(core, path):
+    s = run_kat(core, "inputs_fast_flavor_nonzerok", 160, path,
+                ("time", "f01_Re", "f01_Im", "f01_Rebar", "f01_Imbar"), reduce="max")
+    errs = kat.fast_flavor_k_errors(*s.T)
+    assert max(errs) < 1e-2, errs
+
+
+@pytest.mark.parametrize("path", ["integrator", "fp64", "fp32"])
+def test_kat_msw_gpu(core, path):
+    s = run_kat(core, "inputs_msw_test", 50, path, ("time", "f00_Re", "f11_Re", "f00_Rebar", "f11_Rebar"))
+    e = kat.msw_errors(*s.T)
+    assert max(e.values()) < 7e-2, e
