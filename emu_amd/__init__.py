@@ -13,6 +13,6 @@ __all__ = ["capi", "configs", "TestParams", "core"]
 def __getattr__(name):
     # core imports torch; load it on first use
     if name == "core":
-        from . import core
-        return core
+        import importlib
+        return importlib.import_module(".core", __name__)
     raise AttributeError(name)

@@ -28,6 +28,7 @@ SOURCES = {
     "emu_ref.hip": ["-ffp-contract=off"],
     "emu_state.hip": [],
     "emu_init.hip": [],
+    "emu_adjoint.hip": [],
 }
 
 

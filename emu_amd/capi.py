@@ -79,6 +79,13 @@ _SIGS = [
     ("emu_state_kernel_ms", ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                            ctypes.POINTER(ctypes.c_int64)]),
     ("emu_state_flavor_amplitude", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double)]),
+    ("emu_adj_work_doubles", ctypes.c_int64, [_P, ctypes.c_int64]),
+    ("emu_adj_rhs", ctypes.c_int, [_P, ctypes.c_int64, _P, _P, _P, _P, _P, _P]),
+    ("emu_adj_rhs_vjp", ctypes.c_int, [_P, ctypes.c_int64, _P, _P, _P, _P, _P, _P, _P, _P]),
+    ("emu_adj_objective", ctypes.c_int, [_P, ctypes.c_int64, _P, _P, _P, ctypes.POINTER(ctypes.c_double), _P, _P]),
+    ("emu_adj_advance_positions", ctypes.c_int, [_P, ctypes.c_int64, _P, _P, ctypes.c_double]),
+    ("emu_adj_renormalize", ctypes.c_int, [_P, ctypes.c_int64, _P, _P, ctypes.POINTER(ctypes.c_int64)]),
+    ("emu_adj_axpy", ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_double, _P, _P, _P]),
     ("emu_init_count", ctypes.c_int64, [ctypes.POINTER(EmuParams), ctypes.POINTER(EmuInitConfig)]),
     ("emu_init_particles", ctypes.c_int, [_P, ctypes.POINTER(EmuInitConfig), ctypes.POINTER(EmuParticles), _P]),
 ]

@@ -1,0 +1,491 @@
+// emu_adjoint.hip -- the flavour right-hand side and its vector-Jacobian
+// product (VJP), for reverse-mode gradients through the RK4 evolution.
+//
+// Particles stay in a fixed order (gradients are per particle), so every
+// kernel handles one particle per thread with its own stencil and the
+// deposits use FP64 atomics.  All arrays are SoA: [component][np].
+//
+// Forward RHS for stage state Y (positions frozen within the step):
+//   q_p  = N_p f_p - conj(Nbar_p fbar_p)                    (per particle)
+//   M_t  = base + sum_p w_t(p) q_p,  Phi_t = sum_p w_t(p) v_p q_p      (deposit)
+//   s_p  = coef sum_t w_t(p) (M_t - v_p . Phi_t)                       (gather)
+//   k_p  = -i [Hvac_p + s_p, f_p],  kbar_p = -i [conj(Hvac_p - s_p), fbar_p]
+// VJP, given lambda (the adjoint of k):
+//   the commutator K = -i[A, B] with L = lambda . K = tr(K Lam) (Lam the
+//   Hermitian matrix of lambda with halved off-diagonals) has
+//   dL/dB = -i[Lam, A] and dL/dA = -i[B, Lam] as matrices; as components the
+//   off-diagonal entries count twice.
+//   mu_p = coef (dL/dH_p + sigma dL/dHbar_p), sigma = -1 (Re), +1 (Im)
+//   scatter (adjoint of the gather): LM_t += w_t(p) mu_p, LPhi_t -= w_t(p) v_p mu_p
+//   gather (adjoint of the deposit): Lq_p = sum_t w_t(p) (LM_t + v_p . LPhi_t)
+//   dL/df_p += N_p Lq_p;  dL/dfbar_p += sigma Nbar_p Lq_p;  dL/dNbar_p += sigma fbar_p . Lq_p
+#include "emu_internal.h"
+#include "emu_stage.h"
+#include "../../include/emu_adjoint.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace emu {
+namespace {
+
+constexpr int kB = 256;
+
+unsigned nblk(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + kB - 1) / kB, 1 << 20)); }
+
+struct Stencil {
+    int64_t cell[27];
+    double w[27];
+    int n;
+};
+
+__device__ __forceinline__ void stencil_of(const Geom& g, const double pos[3], Stencil& S)
+{
+    int idx[3];
+    double wd[3][3];
+    for (int d = 0; d < 3; d++) shape3((pos[d] - g.plo[d]) * g.dxi[d], g.order[d], idx[d], wd[d]);
+    const int wx = g.order[0] > 0 ? 3 : 1, wy = g.order[1] > 0 ? 3 : 1, wz = g.order[2] > 0 ? 3 : 1;
+    int n = 0;
+    for (int kk = 0; kk < wz; kk++)
+        for (int jj = 0; jj < wy; jj++)
+            for (int ii = 0; ii < wx; ii++) {
+                const int ox = wx == 1 ? 1 : ii, oy = wy == 1 ? 1 : jj, oz = wz == 1 ? 1 : kk;
+                const int ci = pmod(idx[0] + ox - 1, g.n[0]), cj = pmod(idx[1] + oy - 1, g.n[1]),
+                          ck = pmod(idx[2] + oz - 1, g.n[2]);
+                S.cell[n] = ((int64_t)ck * g.n[1] + cj) * g.n[0] + ci;
+                S.w[n] = wd[0][ox] * wd[1][oy] * wd[2][oz];
+                n++;
+            }
+    S.n = n;
+}
+
+struct AdjIn {
+    int64_t np;
+    const double* pos; // [3][np]
+    const double* pup; // [4][np]
+    const double* Nn;  // [2][np]
+    const double* Y;   // [2 NH][np]
+};
+
+template <int NF>
+__global__ void k_fill_base(double* grid, int64_t ncell, double m00)
+{
+    const int64_t n = ncell * 4 * NH<NF>;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+        grid[t] = (t % (4 * NH<NF>)) == 0 ? m00 : 0.0;
+}
+
+// deposit of q (forward) into [ncell][4 NH]
+template <int NF>
+__global__ void k_deposit_q(AdjIn in, Geom g, double* grid)
+{
+    constexpr int nh = NH<NF>, QG = 4 * nh;
+    const int64_t np = in.np;
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
+        const double pos[3] = {in.pos[p], in.pos[np + p], in.pos[2 * np + p]};
+        const double pt = in.pup[3 * np + p];
+        const double v[3] = {in.pup[p] / pt, in.pup[np + p] / pt, in.pup[2 * np + p] / pt};
+        const double N = in.Nn[p], Nb = in.Nn[np + p];
+        double q[nh];
+        for (int c = 0; c < nh; c++) {
+            const double f = in.Y[(int64_t)c * np + p], fb = in.Y[(int64_t)(nh + c) * np + p];
+            q[c] = is_im_comp<NF>(c) ? N * f + Nb * fb : N * f - Nb * fb;
+        }
+        Stencil S;
+        stencil_of(g, pos, S);
+        for (int t = 0; t < S.n; t++) {
+            double* G = grid + S.cell[t] * QG;
+            for (int c = 0; c < nh; c++) {
+                const double wq = S.w[t] * q[c];
+                unsafeAtomicAdd(&G[c], wq);
+                unsafeAtomicAdd(&G[nh + c], wq * v[0]);
+                unsafeAtomicAdd(&G[2 * nh + c], wq * v[1]);
+                unsafeAtomicAdd(&G[3 * nh + c], wq * v[2]);
+            }
+        }
+    }
+}
+
+struct HamParams {
+    double vac[9];
+    double coef;
+};
+
+// H/hbar and Hbar/hbar of a particle from the deposited grid
+template <int NF>
+__device__ __forceinline__ void hamiltonians(const AdjIn& in, int64_t p, const Geom& g, const double* grid,
+                                             const HamParams& hp, Stencil& S, double v[3], double* H, double* Hb)
+{
+    constexpr int nh = NH<NF>, QG = 4 * nh;
+    const int64_t np = in.np;
+    const double pos[3] = {in.pos[p], in.pos[np + p], in.pos[2 * np + p]};
+    const double pt = in.pup[3 * np + p];
+    v[0] = in.pup[p] / pt;
+    v[1] = in.pup[np + p] / pt;
+    v[2] = in.pup[2 * np + p] / pt;
+    stencil_of(g, pos, S);
+    double inside[nh];
+    for (int c = 0; c < nh; c++) inside[c] = 0.0;
+    for (int t = 0; t < S.n; t++) {
+        const double* G = grid + S.cell[t] * QG;
+        const double w = S.w[t];
+        for (int c = 0; c < nh; c++)
+            inside[c] += w * (G[c] - v[0] * G[nh + c] - v[1] * G[2 * nh + c] - v[2] * G[3 * nh + c]);
+    }
+    for (int c = 0; c < nh; c++) {
+        const double hv = hp.vac[c] / pt, hs = hp.coef * inside[c];
+        H[c] = hv + hs;
+        Hb[c] = is_im_comp<NF>(c) ? hs - hv : hv - hs;
+    }
+}
+
+template <int NF>
+__global__ void k_rhs(AdjIn in, Geom g, const double* grid, HamParams hp, double* k)
+{
+    constexpr int nh = NH<NF>;
+    const int64_t np = in.np;
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
+        Stencil S;
+        double v[3], H[nh], Hb[nh], f[nh], out[nh];
+        hamiltonians<NF>(in, p, g, grid, hp, S, v, H, Hb);
+        for (int c = 0; c < nh; c++) f[c] = in.Y[(int64_t)c * np + p];
+        commutator_rhs<NF, double>(H, f, out, 1.0);
+        for (int c = 0; c < nh; c++) k[(int64_t)c * np + p] = out[c];
+        for (int c = 0; c < nh; c++) f[c] = in.Y[(int64_t)(nh + c) * np + p];
+        commutator_rhs<NF, double>(Hb, f, out, 1.0);
+        for (int c = 0; c < nh; c++) k[(int64_t)(nh + c) * np + p] = out[c];
+    }
+}
+
+// Hermitian component vector of Lam: off-diagonals halved.
+template <int NF> __device__ __forceinline__ void lam_matrix(const double* lam, double* L)
+{
+    for (int c = 0; c < NH<NF>; c++) L[c] = lam[c];
+    for (int a = 0; a < NF; a++)
+        for (int b = a + 1; b < NF; b++) {
+            L[herm_re<NF>(a, b)] *= 0.5;
+            L[herm_im<NF>(a, b)] *= 0.5;
+        }
+}
+
+template <int NF> __device__ __forceinline__ void double_offdiag(double* x)
+{
+    for (int a = 0; a < NF; a++)
+        for (int b = a + 1; b < NF; b++) {
+            x[herm_re<NF>(a, b)] *= 2.0;
+            x[herm_im<NF>(a, b)] *= 2.0;
+        }
+}
+
+// VJP part 1: commutator adjoints.  lamY <- dL/dY through [H, .];
+// mu <- the adjoint of the gathered field (per component).
+template <int NF>
+__global__ void k_vjp_comm(AdjIn in, Geom g, const double* grid, HamParams hp, const double* lamk, double* lamY,
+                           double* mu)
+{
+    constexpr int nh = NH<NF>;
+    const int64_t np = in.np;
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
+        Stencil S;
+        double v[3], H[nh], Hb[nh];
+        hamiltonians<NF>(in, p, g, grid, hp, S, v, H, Hb);
+        double gH[2][nh];
+        for (int bar = 0; bar < 2; bar++) {
+            double lam[nh], L[nh], f[nh], gy[nh];
+            for (int c = 0; c < nh; c++) {
+                lam[c] = lamk[(int64_t)(bar * nh + c) * np + p];
+                f[c] = in.Y[(int64_t)(bar * nh + c) * np + p];
+            }
+            lam_matrix<NF>(lam, L);
+            commutator_rhs<NF, double>(L, bar ? Hb : H, gy, 1.0); // -i[Lam, A]
+            double_offdiag<NF>(gy);
+            for (int c = 0; c < nh; c++) lamY[(int64_t)(bar * nh + c) * np + p] = gy[c];
+            commutator_rhs<NF, double>(f, L, gH[bar], 1.0);       // -i[B, Lam]
+            double_offdiag<NF>(gH[bar]);
+        }
+        for (int c = 0; c < nh; c++)
+            mu[(int64_t)c * np + p] = hp.coef * (gH[0][c] + (is_im_comp<NF>(c) ? gH[1][c] : -gH[1][c]));
+    }
+}
+
+// VJP part 2: scatter mu (adjoint of the gather) into the grid adjoint.
+template <int NF>
+__global__ void k_vjp_scatter(AdjIn in, Geom g, const double* mu, double* gadj)
+{
+    constexpr int nh = NH<NF>, QG = 4 * nh;
+    const int64_t np = in.np;
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
+        const double pos[3] = {in.pos[p], in.pos[np + p], in.pos[2 * np + p]};
+        const double pt = in.pup[3 * np + p];
+        const double v[3] = {in.pup[p] / pt, in.pup[np + p] / pt, in.pup[2 * np + p] / pt};
+        Stencil S;
+        stencil_of(g, pos, S);
+        for (int t = 0; t < S.n; t++) {
+            double* G = gadj + S.cell[t] * QG;
+            for (int c = 0; c < nh; c++) {
+                const double wm = S.w[t] * mu[(int64_t)c * np + p];
+                unsafeAtomicAdd(&G[c], wm);
+                unsafeAtomicAdd(&G[nh + c], -wm * v[0]);
+                unsafeAtomicAdd(&G[2 * nh + c], -wm * v[1]);
+                unsafeAtomicAdd(&G[3 * nh + c], -wm * v[2]);
+            }
+        }
+    }
+}
+
+// VJP part 3: gather of the grid adjoint (adjoint of the deposit).
+template <int NF>
+__global__ void k_vjp_gather(AdjIn in, Geom g, const double* gadj, double* lamY, double* lamNbar)
+{
+    constexpr int nh = NH<NF>, QG = 4 * nh;
+    const int64_t np = in.np;
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
+        const double pos[3] = {in.pos[p], in.pos[np + p], in.pos[2 * np + p]};
+        const double pt = in.pup[3 * np + p];
+        const double v[3] = {in.pup[p] / pt, in.pup[np + p] / pt, in.pup[2 * np + p] / pt};
+        Stencil S;
+        stencil_of(g, pos, S);
+        double lq[nh];
+        for (int c = 0; c < nh; c++) lq[c] = 0.0;
+        for (int t = 0; t < S.n; t++) {
+            const double* G = gadj + S.cell[t] * QG;
+            for (int c = 0; c < nh; c++)
+                lq[c] += S.w[t] * (G[c] + v[0] * G[nh + c] + v[1] * G[2 * nh + c] + v[2] * G[3 * nh + c]);
+        }
+        const double N = in.Nn[p], Nb = in.Nn[np + p];
+        double ln = 0.0;
+        for (int c = 0; c < nh; c++) {
+            const double sg = is_im_comp<NF>(c) ? 1.0 : -1.0;
+            lamY[(int64_t)c * np + p] += N * lq[c];
+            lamY[(int64_t)(nh + c) * np + p] += sg * Nb * lq[c];
+            ln += sg * in.Y[(int64_t)(nh + c) * np + p] * lq[c];
+        }
+        if (lamNbar) lamNbar[p] += ln;
+    }
+}
+
+// Objective J = sum over cells of |N_01|^2 with N_01 = sum_p w N f01 (nu).
+template <int NF>
+__global__ void k_obj_deposit(AdjIn in, Geom g, double* g2)
+{
+    const int64_t np = in.np;
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
+        const double pos[3] = {in.pos[p], in.pos[np + p], in.pos[2 * np + p]};
+        Stencil S;
+        stencil_of(g, pos, S);
+        const double N = in.Nn[p];
+        const double re = N * in.Y[(int64_t)herm_re<NF>(0, 1) * np + p], im = N * in.Y[(int64_t)herm_im<NF>(0, 1) * np + p];
+        for (int t = 0; t < S.n; t++) {
+            unsafeAtomicAdd(&g2[2 * S.cell[t]], S.w[t] * re);
+            unsafeAtomicAdd(&g2[2 * S.cell[t] + 1], S.w[t] * im);
+        }
+    }
+}
+
+__global__ void k_obj_sum(const double* g2, int64_t ncell, double* J)
+{
+    double s = 0.0;
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < ncell; c += (int64_t)gridDim.x * blockDim.x)
+        s += g2[2 * c] * g2[2 * c] + g2[2 * c + 1] * g2[2 * c + 1];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if ((threadIdx.x & 63) == 0) unsafeAtomicAdd(J, s);
+}
+
+template <int NF>
+__global__ void k_obj_grad(AdjIn in, Geom g, const double* g2, double* dJdY)
+{
+    const int64_t np = in.np;
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
+        const double pos[3] = {in.pos[p], in.pos[np + p], in.pos[2 * np + p]};
+        Stencil S;
+        stencil_of(g, pos, S);
+        double re = 0.0, im = 0.0;
+        for (int t = 0; t < S.n; t++) {
+            re += S.w[t] * g2[2 * S.cell[t]];
+            im += S.w[t] * g2[2 * S.cell[t] + 1];
+        }
+        const double N = in.Nn[p];
+        for (int c = 0; c < 2 * NH<NF>; c++) dJdY[(int64_t)c * np + p] = 0.0;
+        dJdY[(int64_t)herm_re<NF>(0, 1) * np + p] = 2.0 * N * re;
+        dJdY[(int64_t)herm_im<NF>(0, 1) * np + p] = 2.0 * N * im;
+    }
+}
+
+__global__ void k_positions(int64_t np, double* pos, const double* pup, double dt, Geom g)
+{
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
+        double x[3], v[3], nx[3];
+        for (int d = 0; d < 3; d++) {
+            x[d] = pos[(int64_t)d * np + p];
+            v[d] = pup[(int64_t)d * np + p] / pup[3 * np + p] * phys::c;
+        }
+        int nc;
+        end_position(x, v, dt, g, nx, nc);
+        for (int d = 0; d < 3; d++) pos[(int64_t)d * np + p] = nx[d];
+    }
+}
+
+template <int NF>
+__global__ void k_renorm_soa(int64_t np, double* Y, const double* Ll, double maxError, unsigned long long* err)
+{
+    constexpr int nh = NH<NF>;
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
+        for (int bar = 0; bar < 2; bar++) {
+            double f[nh];
+            for (int c = 0; c < nh; c++) f[c] = Y[(int64_t)(bar * nh + c) * np + p];
+            const int e = renorm_fp64<NF>(f, Ll[(int64_t)bar * np + p], maxError);
+            if (e != EMU_OK) {
+                atomicMin(err, ((unsigned long long)p << 3) | (unsigned long long)e);
+                break;
+            }
+            for (int c = 0; c < nh; c++) Y[(int64_t)(bar * nh + c) * np + p] = f[c];
+        }
+    }
+}
+
+__global__ void k_axpy(int64_t n, double a, const double* x, const double* y, double* out)
+{
+#pragma clang fp contract(off)
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+        out[t] = a * x[t] + y[t];
+}
+
+__global__ void k_set_u64(unsigned long long* p, unsigned long long v)
+{
+    if (threadIdx.x == 0) *p = v;
+}
+
+__global__ void k_zero(double* p, int64_t n)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) p[t] = 0.0;
+}
+
+HamParams ham_params(const emu_ctx* ctx)
+{
+    HamParams hp;
+    const int nh = ctx->p.nf * ctx->p.nf;
+    for (int c = 0; c < 9; c++) hp.vac[c] = c < nh ? ctx->M2[c] * phys::c4 / 2.0 / phys::hbar : 0.0;
+    hp.coef = M_SQRT2 * phys::GF * ctx->g.inv_cell_volume / phys::hbar;
+    return hp;
+}
+
+int64_t ncell_of(const emu_ctx* ctx) { return (int64_t)ctx->g.n[0] * ctx->g.n[1] * ctx->g.n[2]; }
+
+} // namespace
+} // namespace emu
+
+using namespace emu;
+
+extern "C" int64_t emu_adj_work_doubles(const emu_ctx* ctx, int64_t np)
+{
+    if (!ctx) return -1;
+    const int nh = ctx->p.nf * ctx->p.nf;
+    return 2 * ncell_of(ctx) * 4 * nh + (int64_t)nh * np + 2 * ncell_of(ctx) + 8;
+}
+
+#define ADJ_DISPATCH(kernel, ...)                                                                          \
+    do {                                                                                                   \
+        if (ctx->p.nf == 2) hipLaunchKernelGGL(kernel<2>, dim3(nblk(np)), dim3(kB), 0, ctx->stream, __VA_ARGS__); \
+        else hipLaunchKernelGGL(kernel<3>, dim3(nblk(np)), dim3(kB), 0, ctx->stream, __VA_ARGS__);          \
+    } while (0)
+
+static double m00_of(const emu_ctx* ctx)
+{
+    return ctx->p.rho * ctx->p.Ye / phys::Mp / ctx->g.inv_cell_volume;
+}
+
+static void deposit(emu_ctx* ctx, const AdjIn& in, double* grid)
+{
+    const int64_t np = in.np;
+    const int64_t ng = ncell_of(ctx) * 4 * ctx->p.nf * ctx->p.nf;
+    if (ctx->p.nf == 2) hipLaunchKernelGGL(k_fill_base<2>, dim3(nblk(ng)), dim3(kB), 0, ctx->stream, grid, ncell_of(ctx), m00_of(ctx));
+    else hipLaunchKernelGGL(k_fill_base<3>, dim3(nblk(ng)), dim3(kB), 0, ctx->stream, grid, ncell_of(ctx), m00_of(ctx));
+    ADJ_DISPATCH(k_deposit_q, in, ctx->g, grid);
+}
+
+extern "C" int emu_adj_rhs(emu_ctx* ctx, int64_t np, const double* pos, const double* pup, const double* Nn,
+                           const double* Y, double* k, double* work)
+{
+    if (!ctx || np < 0 || !work) return EMU_ERR_ARG;
+    if (np == 0) return EMU_OK;
+    const AdjIn in{np, pos, pup, Nn, Y};
+    double* grid = work;
+    deposit(ctx, in, grid);
+    ADJ_DISPATCH(k_rhs, in, ctx->g, grid, ham_params(ctx), k);
+    EMU_LAUNCH_CHECK(ctx, "adj_rhs");
+    return EMU_OK;
+}
+
+extern "C" int emu_adj_rhs_vjp(emu_ctx* ctx, int64_t np, const double* pos, const double* pup, const double* Nn,
+                               const double* Y, const double* lam_k, double* lam_Y, double* lam_Nbar, double* work)
+{
+    if (!ctx || np < 0 || !work) return EMU_ERR_ARG;
+    if (np == 0) return EMU_OK;
+    const AdjIn in{np, pos, pup, Nn, Y};
+    const int nh = ctx->p.nf * ctx->p.nf;
+    const int64_t ng = ncell_of(ctx) * 4 * nh;
+    double* grid = work;
+    double* gadj = work + ng;
+    double* mu = gadj + ng;
+    deposit(ctx, in, grid);
+    ADJ_DISPATCH(k_vjp_comm, in, ctx->g, grid, ham_params(ctx), lam_k, lam_Y, mu);
+    hipLaunchKernelGGL(k_zero, dim3(nblk(ng)), dim3(kB), 0, ctx->stream, gadj, ng);
+    ADJ_DISPATCH(k_vjp_scatter, in, ctx->g, mu, gadj);
+    ADJ_DISPATCH(k_vjp_gather, in, ctx->g, gadj, lam_Y, lam_Nbar);
+    EMU_LAUNCH_CHECK(ctx, "adj_rhs_vjp");
+    return EMU_OK;
+}
+
+extern "C" int emu_adj_objective(emu_ctx* ctx, int64_t np, const double* pos, const double* Nn, const double* Y,
+                                 double* J, double* dJdY, double* work)
+{
+    if (!ctx || np < 0 || !work || !J) return EMU_ERR_ARG;
+    const AdjIn in{np, pos, nullptr, Nn, Y};
+    const int nh = ctx->p.nf * ctx->p.nf;
+    const int64_t nc = ncell_of(ctx);
+    double* g2 = work + 2 * nc * 4 * nh + (int64_t)nh * np;
+    double* dJ = g2 + 2 * nc;
+    hipLaunchKernelGGL(k_zero, dim3(nblk(2 * nc + 1)), dim3(kB), 0, ctx->stream, g2, 2 * nc + 1);
+    if (np > 0) ADJ_DISPATCH(k_obj_deposit, in, ctx->g, g2);
+    hipLaunchKernelGGL(k_obj_sum, dim3(nblk(nc)), dim3(kB), 0, ctx->stream, g2, nc, dJ);
+    if (dJdY && np > 0) ADJ_DISPATCH(k_obj_grad, in, ctx->g, g2, dJdY);
+    EMU_LAUNCH_CHECK(ctx, "adj_objective");
+    EMU_HIP(ctx, hipMemcpyAsync(J, dJ, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    EMU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return EMU_OK;
+}
+
+extern "C" int emu_adj_advance_positions(emu_ctx* ctx, int64_t np, double* pos, const double* pup, double dt)
+{
+    if (!ctx || np < 0) return EMU_ERR_ARG;
+    if (np == 0) return EMU_OK;
+    hipLaunchKernelGGL(k_positions, dim3(nblk(np)), dim3(kB), 0, ctx->stream, np, pos, pup, dt, ctx->g);
+    EMU_LAUNCH_CHECK(ctx, "adj_advance_positions");
+    return EMU_OK;
+}
+
+extern "C" int emu_adj_renormalize(emu_ctx* ctx, int64_t np, double* Y, const double* Ll, int64_t* bad_index)
+{
+    if (!ctx || np < 0) return EMU_ERR_ARG;
+    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, ctx->stream, ctx->d_red + 6, ~0ull);
+    if (np > 0) ADJ_DISPATCH(k_renorm_soa, np, Y, Ll, ctx->p.maxError, ctx->d_red + 6);
+    EMU_LAUNCH_CHECK(ctx, "adj_renormalize");
+    unsigned long long h;
+    EMU_HIP(ctx, hipMemcpyAsync(&h, ctx->d_red + 6, 8, hipMemcpyDeviceToHost, ctx->stream));
+    EMU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (h == ~0ull) {
+        if (bad_index) *bad_index = -1;
+        return EMU_OK;
+    }
+    if (bad_index) *bad_index = (int64_t)(h >> 3);
+    return (int)(h & 7);
+}
+
+extern "C" int emu_adj_axpy(emu_ctx* ctx, int64_t n, double a, const double* x, const double* y, double* out)
+{
+    if (!ctx || n < 0) return EMU_ERR_ARG;
+    if (n == 0) return EMU_OK;
+    hipLaunchKernelGGL(k_axpy, dim3(nblk(n)), dim3(kB), 0, ctx->stream, n, a, x, y, out);
+    EMU_LAUNCH_CHECK(ctx, "adj_axpy");
+    return EMU_OK;
+}

@@ -1,0 +1,424 @@
+// emu_stage.h -- the fused stage kernel: gather + Hamiltonian + commutator +
+// RK update of every particle of a cell, and the deposit of the next stage
+// state as one small matrix product per cell on the FP64 matrix cores.
+//
+// One wave owns one cell at a time and takes the cell's particles 64 at a
+// time (lane = particle).
+//
+// Gather (VALU): the 27 neighbour cells' fields are the same for every lane of
+// the wave, so their addresses are wave-uniform and the loads go to scalar
+// registers; each particle then needs 27 x (4 + 4 NH) FMAs:
+//   inside_c = sum_t w_t (M_c - vx Phix_c - vy Phiy_c - vz Phiz_c).
+// Deposit (MFMA): the contribution of the cell's particles p to target cell t
+// and field column q is D[t][q] = sum_p W[p][t] R[p][q], with W the shape
+// weights and R = (1, vx, vy, vz) x q_c(p), q = N f - conj(Nbar fbar).  Over a
+// chunk of 64 particles this is a (27 x 64) x (64 x 4NH) product, done with
+// v_mfma_f64_16x16x4 (4 particles per instruction), so the sum over particles
+// never leaves the matrix unit; the lane-layout transposition goes through LDS
+// (9 shape factors and NH + 3 sources per particle).  The wave's accumulators
+// are added to the grid once per cell with FP64 atomics.
+#pragma once
+
+#include "emu_internal.h"
+
+namespace emu {
+
+typedef double mf_d4 __attribute__((ext_vector_type(4)));
+
+template <typename T> struct PArrays {
+    double* pos[3];
+    int64_t* id;
+    T* pup[4]; // pupx, pupy, pupz, pupt
+    T* Nn[2];  // N, Nbar
+    T* Ll[2];  // L, Lbar
+    T* f;      // [2 NH][np]: nu components then nubar
+};
+
+template <typename T> struct StageArgs {
+    PArrays<T> cur; // current copy, sorted by cell_start
+    PArrays<T> dst; // stage 4 writes the step's result here, sorted by the new cells
+    T* Y;           // stage state
+    T* acc;         // RK accumulator
+    int* newcell;   // stage 1 writes each particle's end-of-step cell
+    const int* mbase;
+    const int* rank;
+    const unsigned char* mv;
+    const int* cell_start;
+    const double* gin;  // grid read by the gather (FP64 path)
+    const float* gin32; // its FP32 copy (FP32-bulk path)
+    double* gout;       // grid deposited by this launch
+    double* aux;        // compute_dt auxiliaries
+    const double* dt;
+    unsigned long long* err;
+    int64_t np;
+    int64_t ncell;
+    int stage; // 0 = deposit only, 1..4
+    int want_aux;
+    int blocks_per_xcd;
+    Geom g;
+    double vac[9]; // M2 c^4 / (2 hbar)
+    double coef;   // sqrt2 GF / dV / hbar
+    double maxError;
+};
+
+template <int NF> __device__ __forceinline__ constexpr bool is_im_comp(int c)
+{
+    for (int a = 0; a < NF; a++)
+        for (int b = a + 1; b < NF; b++)
+            if (c == herm_im<NF>(a, b)) return true;
+    return false;
+}
+
+__device__ __forceinline__ int bin1(double pos, double plo, double dxi, int n)
+{
+#pragma clang fp contract(off)
+    return pmod(nearest_cell_center_index((pos - plo) * dxi), n);
+}
+
+// x += (dt b_i) v for the four RK4 stages, rounded as the reference's saxpy
+// sequence (FlavoredNeutrinoContainer.H:43-45) with v = c pup/pupt.
+__device__ __forceinline__ double advance_pos(double x, double v, const double a[4])
+{
+#pragma clang fp contract(off)
+    x = a[0] * v + x;
+    x = a[1] * v + x;
+    x = a[2] * v + x;
+    x = a[3] * v + x;
+    return x;
+}
+
+__device__ __forceinline__ double wrap1(double x, double rlo, double rhi, double L)
+{
+#pragma clang fp contract(off)
+    // one period per step at most; the bound keeps non-finite input from looping
+    if (x > rhi) {
+        for (int it = 0; it < 8 && x > rhi; it++) x -= L;
+        if (x < rlo || x > rhi) x = rlo;
+    } else if (x < rlo) {
+        for (int it = 0; it < 8 && x < rlo; it++) x += L;
+        if (x > rhi || x < rlo) x = rhi;
+    }
+    return x;
+}
+
+template <int NF>
+__device__ __forceinline__ int renorm_fp64(double* f, double L, double maxError)
+{
+#pragma clang fp contract(off)
+    double sumP = 0;
+#pragma unroll
+    for (int i = 0; i < NF; i++) sumP += f[herm_re<NF>(i, i)];
+    double error = sumP - 1.0;
+    if (fabs(error) > 100. * maxError) return EMU_ERR_TRACE;
+    if (fabs(error) > maxError) {
+#pragma unroll
+        for (int i = 0; i < NF; i++) f[herm_re<NF>(i, i)] -= error / NF;
+    }
+#pragma unroll
+    for (int i = 0; i < NF; i++) {
+        double& fii = f[herm_re<NF>(i, i)];
+        if (fii < -100. * maxError) return EMU_ERR_NEGDIAG;
+        if (fii < -maxError) fii = 0;
+    }
+    const double length = sqrt(su_mag2<NF, double>(f));
+    error = length - L;
+    if (fabs(error) > 100. * maxError) return EMU_ERR_LENGTH;
+    if (fabs(error) > maxError) {
+#pragma unroll
+        for (int c = 0; c < NH<NF>; c++) f[c] /= length / L;
+    }
+    return EMU_OK;
+}
+
+// End-of-step position: the four stage saxpys, then the periodic shift.
+__device__ __forceinline__ void end_position(const double pos[3], const double v[3], double dt, const Geom& g,
+                                             double npos[3], int& newcell)
+{
+    const double ai[4] = {dt * (1. / 6.), dt * (1. / 3.), dt * (1. / 3.), dt * (1. / 6.)};
+    int nc[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        npos[d] = wrap1(advance_pos(pos[d], v[d], ai), g.rlo[d], g.rhi[d], g.L[d]);
+        nc[d] = bin1(npos[d], g.plo[d], g.dxi[d], g.n[d]);
+    }
+    newcell = (nc[2] * g.n[1] + nc[1]) * g.n[0] + nc[0];
+}
+
+template <int NF, typename T, int WX, int WY, int WZ, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_stage(StageArgs<T> a)
+{
+    constexpr int nh = NH<NF>;
+    constexpr int QG = 4 * nh;
+    constexpr int NT = WX * WY * WZ;        // target cells of a source cell
+    constexpr int MT = (NT + 15) / 16;       // MFMA row tiles
+    constexpr int NTL = (QG + 15) / 16;      // MFMA column tiles
+    constexpr int NPAIR = NF * (NF - 1) / 2;
+    constexpr int NAUX = 2 * NF + 3 * NPAIR; // compute_dt auxiliaries per cell
+    constexpr int NSRC = nh + 3;             // q_c and vhat per particle
+    constexpr int NSAUX = 2 * NF + NPAIR;    // N f_ii, Nbar fbar_ii, Nbar Im(fbar_ab)
+    extern __shared__ double lds[];
+
+    const Geom& g = a.g;
+    const int s = a.stage;
+    const bool aux = a.want_aux && (s == 3 || s == 0);
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int lane = threadIdx.x & 63;
+    // blocks go round-robin over the 8 XCDs; give each XCD a contiguous run of cells
+    const int b = blockIdx.x;
+    const int cblock = (b & 7) * a.blocks_per_xcd + (b >> 3);
+    const int cell = __builtin_amdgcn_readfirstlane(cblock * WPB + wave);
+    if (cell >= a.ncell) return;
+    const int beg = a.cell_start[cell], end = a.cell_start[cell + 1];
+    if (beg >= end) return;
+    const int cx = cell % g.n[0], cy = (cell / g.n[0]) % g.n[1], cz = cell / (g.n[0] * g.n[1]);
+    const int per_wave = 64 * (9 + NSRC + (aux ? NSAUX : 0));
+    double* Wf = lds + wave * per_wave;     // [64][9]   shape factors by dim
+    double* Src = Wf + 64 * 9;              // [64][NSRC]
+    double* Saux = Src + 64 * NSRC;         // [64][NSAUX]
+
+    const double dt = *a.dt;
+    const double bw[4] = {1. / 6., 1. / 3., 1. / 3., 1. / 6.};
+    const double cnext[4] = {0.5, 0.5, 1.0, 0.0};
+
+    // lane constants for the MFMA operands: target (row) and field column of
+    // this lane in each tile
+    int tdx[MT][3];
+#pragma unroll
+    for (int m = 0; m < MT; m++) {
+        const int t = 16 * m + (lane & 15);
+        const int ii = t % WX, jj = (t / WX) % WY, kk = t / (WX * WY);
+        tdx[m][0] = WX == 1 ? 1 : ii;
+        tdx[m][1] = 3 + (WY == 1 ? 1 : jj);
+        tdx[m][2] = 6 + (WZ == 1 ? 1 : kk);
+    }
+    mf_d4 acc[MT][NTL];
+    mf_d4 acca[MT];
+#pragma unroll
+    for (int m = 0; m < MT; m++) {
+        acca[m] = mf_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int n = 0; n < NTL; n++) acc[m][n] = mf_d4{0.0, 0.0, 0.0, 0.0};
+    }
+    const bool deposits = s <= 3;
+
+    for (int base = beg; base < end; base += 64) {
+        const int64_t ip = base + lane;
+        const bool act = ip < end;
+        const int64_t ipl = act ? ip : beg; // a valid index for inactive lanes' loads
+        double pos[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) pos[d] = a.cur.pos[d][ipl];
+        const T pupt = a.cur.pup[3][ipl];
+        const T vhat[3] = {a.cur.pup[0][ipl] / pupt, a.cur.pup[1][ipl] / pupt, a.cur.pup[2][ipl] / pupt};
+        const T N = a.cur.Nn[0][ipl], Nb = a.cur.Nn[1][ipl];
+        int idx[3];
+        double wd[3][3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) shape3((pos[d] - g.plo[d]) * g.dxi[d], g.order[d], idx[d], wd[d]);
+        T Y[2 * nh];
+#pragma unroll
+        for (int c = 0; c < 2 * nh; c++)
+            Y[c] = s <= 1 ? a.cur.f[(int64_t)c * a.np + ipl] : a.Y[(int64_t)c * a.np + ipl];
+        T Dep[2 * nh]; // state deposited by this launch
+        if (s == 0) {
+#pragma unroll
+            for (int c = 0; c < 2 * nh; c++) Dep[c] = Y[c];
+        } else {
+            // ---- gather over the stencil: the neighbours are wave-uniform.  The
+            // loops over z and y stay rolled (uniform trip counts) to keep the
+            // scalar loads of one row of neighbours in flight at a time.
+            T inside[nh];
+#pragma unroll
+            for (int c = 0; c < nh; c++) inside[c] = T(0);
+#pragma unroll 1
+            for (int kk = 0; kk < WZ; kk++) {
+                const double wz_ = WZ == 1 ? wd[2][1] : (kk == 0 ? wd[2][0] : kk == 1 ? wd[2][1] : wd[2][2]);
+                const int ck = WZ == 1 ? cz : pmod(cz + kk - 1, g.n[2]);
+#pragma unroll 1
+                for (int jj = 0; jj < WY; jj++) {
+                    const double wyz =
+                        (WY == 1 ? wd[1][1] : (jj == 0 ? wd[1][0] : jj == 1 ? wd[1][1] : wd[1][2])) * wz_;
+                    const int cj = WY == 1 ? cy : pmod(cy + jj - 1, g.n[1]);
+#pragma unroll
+                    for (int ii = 0; ii < WX; ii++) {
+                        const int ci = WX == 1 ? cx : pmod(cx + ii - 1, g.n[0]);
+                        const int64_t nb = ((int64_t)ck * g.n[1] + cj) * g.n[0] + ci;
+                        const T w = T(wd[0][WX == 1 ? 1 : ii] * wyz);
+                        const T wx = w * vhat[0], wy = w * vhat[1], wz = w * vhat[2];
+                        if constexpr (sizeof(T) == 8) {
+                            // constant address space: the grid is read-only in this launch and
+                            // the address is wave-uniform, so the loads go to scalar registers
+                            const __attribute__((address_space(4))) double* G =
+                                (const __attribute__((address_space(4))) double*)(a.gin + nb * QG);
+#pragma unroll
+                            for (int c = 0; c < nh; c++)
+                                inside[c] += w * T(G[c]) - wx * T(G[nh + c]) - wy * T(G[2 * nh + c]) -
+                                             wz * T(G[3 * nh + c]);
+                        } else {
+                            const __attribute__((address_space(4))) float* G =
+                                (const __attribute__((address_space(4))) float*)(a.gin32 + nb * QG);
+#pragma unroll
+                            for (int c = 0; c < nh; c++)
+                                inside[c] += w * G[c] - wx * G[nh + c] - wy * G[2 * nh + c] - wz * G[3 * nh + c];
+                        }
+                    }
+                }
+            }
+            // ---- Hamiltonians / hbar: H = H_vac + coef inside, Hbar = conj(H_vac - coef inside)
+            T H[nh], Hb[nh];
+            const T inv_pupt = T(1) / pupt;
+#pragma unroll
+            for (int c = 0; c < nh; c++) {
+                const T hv = T(a.vac[c]) * inv_pupt;
+                const T hs = T(a.coef) * inside[c];
+                H[c] = hv + hs;
+                Hb[c] = is_im_comp<NF>(c) ? hs - hv : hv - hs;
+            }
+            T k[2 * nh];
+            commutator_rhs<NF, T>(H, Y, k, T(1));
+            commutator_rhs<NF, T>(Hb, Y + nh, k + nh, T(1));
+            const T a_acc = T(dt * bw[s - 1]);
+            if (s < 4) {
+                const T a_next = T(dt * cnext[s - 1]);
+#pragma unroll
+                for (int c = 0; c < 2 * nh; c++) {
+                    const T yn = s == 1 ? Y[c] : a.cur.f[(int64_t)c * a.np + ipl];
+                    const T acc_old = s == 1 ? yn : a.acc[(int64_t)c * a.np + ipl];
+                    Dep[c] = a_next * k[c] + yn;
+                    if (act) {
+                        a.acc[(int64_t)c * a.np + ip] = a_acc * k[c] + acc_old;
+                        a.Y[(int64_t)c * a.np + ip] = Dep[c];
+                    }
+                }
+                if (s == 1 && act) { // the end-of-step cell, for the sort before stage 4
+                    double v[3], npos[3];
+#pragma unroll
+                    for (int d = 0; d < 3; d++) v[d] = double(a.cur.pup[d][ip]) / double(pupt) * phys::c;
+                    int nc;
+                    end_position(pos, v, dt, g, npos, nc);
+                    a.newcell[ip] = nc;
+                }
+            } else if (act) {
+                // y_{n+1} = acc + (dt/6) k4, then the post-step hook (main.cpp:155-199):
+                // periodic shift, Renormalize in FP64, and the move to the sorted slot
+                double y[2 * nh];
+#pragma unroll
+                for (int c = 0; c < 2 * nh; c++) y[c] = double(a_acc * k[c] + a.acc[(int64_t)c * a.np + ip]);
+                const T L = a.cur.Ll[0][ip], Lb = a.cur.Ll[1][ip];
+                const int64_t id = a.cur.id[ip];
+                int err = renorm_fp64<NF>(y, double(L), a.maxError);
+                if (err == EMU_OK) err = renorm_fp64<NF>(y + nh, double(Lb), a.maxError);
+                if (err != EMU_OK) atomicMin(a.err, ((unsigned long long)id << 3) | (unsigned long long)err);
+                double v[3], npos[3];
+#pragma unroll
+                for (int d = 0; d < 3; d++) v[d] = double(a.cur.pup[d][ip]) / double(pupt) * phys::c;
+                int nc;
+                end_position(pos, v, dt, g, npos, nc);
+                const int64_t j = (int64_t)a.mbase[(int64_t)cell * 27 + a.mv[ip]] + a.rank[ip];
+                if (j >= 0 && j < a.np) {
+#pragma unroll
+                    for (int d = 0; d < 3; d++) a.dst.pos[d][j] = npos[d];
+#pragma unroll
+                    for (int d = 0; d < 4; d++) a.dst.pup[d][j] = a.cur.pup[d][ip];
+                    a.dst.Nn[0][j] = N;
+                    a.dst.Nn[1][j] = Nb;
+                    a.dst.Ll[0][j] = L;
+                    a.dst.Ll[1][j] = Lb;
+                    a.dst.id[j] = id;
+#pragma unroll
+                    for (int c = 0; c < 2 * nh; c++) a.dst.f[(int64_t)c * a.np + j] = T(y[c]);
+                } else {
+                    atomicMin(a.err, (unsigned long long)EMU_ERR_MOVE | (~0ull << 3));
+                }
+            }
+        }
+        if (!deposits) continue;
+        // ---- stage the chunk's MFMA operands in LDS (zeros for inactive lanes)
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+#pragma unroll
+            for (int e = 0; e < 3; e++) Wf[lane * 9 + 3 * d + e] = act ? wd[d][e] : 0.0;
+#pragma unroll
+        for (int c = 0; c < nh; c++) {
+            const T qc = is_im_comp<NF>(c) ? N * Dep[c] + Nb * Dep[nh + c] : N * Dep[c] - Nb * Dep[nh + c];
+            Src[lane * NSRC + c] = act ? double(qc) : 0.0;
+        }
+#pragma unroll
+        for (int d = 0; d < 3; d++) Src[lane * NSRC + nh + d] = act ? double(vhat[d]) : 0.0;
+        if (aux) {
+#pragma unroll
+            for (int x = 0; x < NF; x++) {
+                Saux[lane * NSAUX + x] = act ? double(N * Dep[herm_re<NF>(x, x)]) : 0.0;
+                Saux[lane * NSAUX + NF + x] = act ? double(Nb * Dep[nh + herm_re<NF>(x, x)]) : 0.0;
+            }
+            int o = 2 * NF;
+#pragma unroll
+            for (int x = 0; x < NF; x++)
+#pragma unroll
+                for (int y2 = x + 1; y2 < NF; y2++) Saux[lane * NSAUX + (o++)] = act ? double(Nb * Dep[nh + herm_im<NF>(x, y2)]) : 0.0;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // ---- D += W^T R, four particles per MFMA
+        const int nact = min(64, end - base);
+        const int nk = (nact + 3) >> 2;
+        for (int kap = 0; kap < nk; kap++) {
+            const int p = 4 * kap + (lane >> 4);
+            const double* wf = Wf + p * 9;
+            const double* sr = Src + p * NSRC;
+            double A[MT];
+#pragma unroll
+            for (int m = 0; m < MT; m++) {
+                const int t = 16 * m + (lane & 15);
+                A[m] = t < NT ? wf[tdx[m][0]] * wf[tdx[m][1]] * wf[tdx[m][2]] : 0.0;
+            }
+#pragma unroll
+            for (int n = 0; n < NTL; n++) {
+                const int q = 16 * n + (lane & 15);
+                const int f = q / nh, c = q % nh;
+                const double B = q < QG ? sr[c] * (f == 0 ? 1.0 : sr[nh + f - 1]) : 0.0;
+#pragma unroll
+                for (int m = 0; m < MT; m++) acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[m], B, acc[m][n], 0, 0, 0);
+            }
+            if (aux) {
+                const int q = lane & 15;
+                double B = 0.0;
+                if (q < 2 * NF) B = Saux[p * NSAUX + q];
+                else if (q < NAUX) {
+                    const int pr = (q - 2 * NF) / 3, d = (q - 2 * NF) % 3;
+                    B = Saux[p * NSAUX + 2 * NF + pr] * sr[nh + d];
+                }
+#pragma unroll
+                for (int m = 0; m < MT; m++) acca[m] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[m], B, acca[m], 0, 0, 0);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    if (!deposits) return;
+    // ---- flush: for the f64 16x16x4 product, element r of lane l holds
+    // D[16m + 4r + l/16][16n + l%16] (rows interleaved across the four
+    // 16-lane groups, unlike the f32 form's 4(l/16) + r)
+#pragma unroll
+    for (int m = 0; m < MT; m++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int t = 16 * m + 4 * r + (lane >> 4);
+            if (t >= NT) continue;
+            const int ii = t % WX, jj = (t / WX) % WY, kk = t / (WX * WY);
+            const int ci = WX == 1 ? cx : pmod(cx + ii - 1, g.n[0]);
+            const int cj = WY == 1 ? cy : pmod(cy + jj - 1, g.n[1]);
+            const int ck = WZ == 1 ? cz : pmod(cz + kk - 1, g.n[2]);
+            const int64_t tc = ((int64_t)ck * g.n[1] + cj) * g.n[0] + ci;
+#pragma unroll
+            for (int n = 0; n < NTL; n++) {
+                const int q = 16 * n + (lane & 15);
+                if (q < QG) unsafeAtomicAdd(&a.gout[tc * QG + q], acc[m][n][r]);
+            }
+            if (aux && (lane & 15) < NAUX) unsafeAtomicAdd(&a.aux[tc * NAUX + (lane & 15)], acca[m][r]);
+        }
+}
+
+} // namespace emu

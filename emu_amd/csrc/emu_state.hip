@@ -2,13 +2,14 @@
 // particle state (the integrator loop of main.cpp:129-213).
 //
 // Per step (stage s reads the grid deposited from the stage-s state):
-//   stage 1..3 kernels: gather + Hamiltonian + commutator + RK update of the
-//     particle, then deposit of the next stage state into the other grid;
+//   stage 1..3 kernels (emu_stage.h): gather + Hamiltonian + commutator + RK
+//     update of the particle, then deposit of the next stage state into the
+//     other grid; stage 1 also bins each particle's end-of-step position;
+//   a stable counting sort plan by the new cell (ranks and bases);
 //   compute_dt reduction over the stage-4 deposit (main.cpp:193-197);
 //   stage 4 kernel: y_{n+1}, positions, periodic shift, Renormalize (FP64),
-//     the deposit of y_{n+1} that the next step's stage 1 reads, and the new
-//     cell index;
-//   counting sort of the particles by cell (stable, so deterministic).
+//     each particle written straight to its slot in the new cell order;
+//   the deposit of y_{n+1} that the next step's stage 1 reads.
 //
 // The mesh holds, per cell and flavour component c, the combined fields
 //   M_c  = N_c - conj(Nbar)_c  (+ rho Ye/Mp dV on 00, the matter term)
@@ -18,6 +19,7 @@
 // additionally needs N_ii, Nbar_ii and Im(Fbar_d) off the diagonal, which the
 // stage-3 kernel deposits into an auxiliary grid.
 #include "emu_internal.h"
+#include "emu_stage.h"
 
 #include <hipcub/hipcub.hpp>
 
@@ -30,33 +32,6 @@ namespace emu {
 namespace {
 
 constexpr int kWave = 64;
-
-template <typename T> struct StageArgs {
-    // persistent particle arrays (current sort order)
-    double* pos[3];
-    T* pup[4];  // pupx, pupy, pupz, pupt
-    T* Nn[2];   // N, Nbar
-    T* Ll[2];   // L, Lbar
-    T* f;       // y_n: [2*NH][np] (nu components then nubar)
-    T* Y;       // stage state
-    T* acc;     // RK accumulator
-    int* newcell;
-    int* rank;
-    const int* cell_start;
-    const double* gin;   // grid read by this stage
-    double* gout;        // grid deposited by this stage
-    double* aux;         // compute_dt auxiliaries (stage 3 and the initial deposit)
-    const double* dt;    // device scalar
-    unsigned long long* err; // Renormalize status (lowest id << 3 | code)
-    const int64_t* id;
-    int64_t np;
-    int stage;           // 0 = deposit only, 1..4
-    int want_aux;
-    Geom g;
-    double vac[9];       // M2 c^4 / (2 hbar), per component
-    double coef;         // sqrt2 GF / dV / hbar
-    double maxError;
-};
 
 // Base value of the deposit grid: zero except the matter term rho Ye/Mp dV
 // on M_00 (inside_parentheses of generate_code.py:335-342, moved into the
@@ -73,254 +48,6 @@ __global__ void k_fill_grid(double* g, int64_t ncell, int Q, double m00, double*
     }
 }
 
-__device__ __forceinline__ int bin1(double pos, double plo, double dxi, int n)
-{
-#pragma clang fp contract(off)
-    return pmod(nearest_cell_center_index((pos - plo) * dxi), n);
-}
-
-__device__ __forceinline__ double advance_pos(double x, double v, const double a[4])
-{
-#pragma clang fp contract(off)
-    x = a[0] * v + x;
-    x = a[1] * v + x;
-    x = a[2] * v + x;
-    x = a[3] * v + x;
-    return x;
-}
-
-__device__ __forceinline__ double wrap1(double x, double rlo, double rhi, double L)
-{
-#pragma clang fp contract(off)
-    // one period per step at most; the bound keeps non-finite input from looping
-    if (x > rhi) {
-        for (int it = 0; it < 8 && x > rhi; it++) x -= L;
-        if (x < rlo || x > rhi) x = rlo;
-    } else if (x < rlo) {
-        for (int it = 0; it < 8 && x < rlo; it++) x += L;
-        if (x > rhi || x < rlo) x = rhi;
-    }
-    return x;
-}
-
-template <int NF>
-__device__ __forceinline__ int renorm_fp64(double* f, double L, double maxError)
-{
-#pragma clang fp contract(off)
-    double sumP = 0;
-#pragma unroll
-    for (int i = 0; i < NF; i++) sumP += f[herm_re<NF>(i, i)];
-    double error = sumP - 1.0;
-    if (fabs(error) > 100. * maxError) return EMU_ERR_TRACE;
-    if (fabs(error) > maxError) {
-#pragma unroll
-        for (int i = 0; i < NF; i++) f[herm_re<NF>(i, i)] -= error / NF;
-    }
-#pragma unroll
-    for (int i = 0; i < NF; i++) {
-        double& fii = f[herm_re<NF>(i, i)];
-        if (fii < -100. * maxError) return EMU_ERR_NEGDIAG;
-        if (fii < -maxError) fii = 0;
-    }
-    const double length = sqrt(su_mag2<NF, double>(f));
-    error = length - L;
-    if (fabs(error) > 100. * maxError) return EMU_ERR_LENGTH;
-    if (fabs(error) > maxError) {
-#pragma unroll
-        for (int c = 0; c < NH<NF>; c++) f[c] /= length / L;
-    }
-    return EMU_OK;
-}
-
-template <int NF> __device__ __forceinline__ constexpr bool is_im_comp(int c)
-{
-    for (int a = 0; a < NF; a++)
-        for (int b = a + 1; b < NF; b++)
-            if (c == herm_im<NF>(a, b)) return true;
-    return false;
-}
-
-// Per-particle deposit of the combined source q = N f - conj(Nbar fbar) with
-// weights w into the grid (and, when requested, the compute_dt auxiliaries).
-template <int NF, typename T, int WX, int WY, int WZ>
-__device__ __forceinline__ void deposit_particle(double* grid, double* aux, bool want_aux, const Geom& g,
-                                                 const double pos[3], const T vhat[3], T N, T Nb,
-                                                 const T* fnu, const T* fbar)
-{
-    constexpr int nh = NH<NF>;
-    constexpr int QG = 4 * nh;
-    constexpr int NAUX = 2 * NF + 3 * (NF * (NF - 1) / 2);
-    int idx[3];
-    double wd[3][3];
-#pragma unroll
-    for (int d = 0; d < 3; d++) shape3((pos[d] - g.plo[d]) * g.dxi[d], g.order[d], idx[d], wd[d]);
-    T q[nh];
-#pragma unroll
-    for (int c = 0; c < nh; c++) q[c] = is_im_comp<NF>(c) ? N * fnu[c] + Nb * fbar[c] : N * fnu[c] - Nb * fbar[c];
-#pragma unroll
-    for (int kk = 0; kk < WZ; kk++)
-#pragma unroll
-        for (int jj = 0; jj < WY; jj++)
-#pragma unroll
-            for (int ii = 0; ii < WX; ii++) {
-                const int ox = WX == 1 ? 1 : ii, oy = WY == 1 ? 1 : jj, oz = WZ == 1 ? 1 : kk;
-                const T w = T(wd[0][ox] * wd[1][oy] * wd[2][oz]);
-                const int ci = pmod(idx[0] + ox - 1, g.n[0]), cj = pmod(idx[1] + oy - 1, g.n[1]),
-                          ck = pmod(idx[2] + oz - 1, g.n[2]);
-                const int64_t cell = ((int64_t)ck * g.n[1] + cj) * g.n[0] + ci;
-                double* G = grid + cell * QG;
-#pragma unroll
-                for (int c = 0; c < nh; c++) {
-                    const T wq = w * q[c];
-                    unsafeAtomicAdd(&G[c], (double)wq);
-                    unsafeAtomicAdd(&G[nh + c], (double)(wq * vhat[0]));
-                    unsafeAtomicAdd(&G[2 * nh + c], (double)(wq * vhat[1]));
-                    unsafeAtomicAdd(&G[3 * nh + c], (double)(wq * vhat[2]));
-                }
-                if (want_aux) {
-                    double* A = aux + cell * NAUX;
-#pragma unroll
-                    for (int a = 0; a < NF; a++) {
-                        unsafeAtomicAdd(&A[a], (double)(w * N * fnu[herm_re<NF>(a, a)]));
-                        unsafeAtomicAdd(&A[NF + a], (double)(w * Nb * fbar[herm_re<NF>(a, a)]));
-                    }
-                    int o = 2 * NF;
-#pragma unroll
-                    for (int a = 0; a < NF; a++)
-#pragma unroll
-                        for (int b = a + 1; b < NF; b++) {
-                            const T wf = w * Nb * fbar[herm_im<NF>(a, b)];
-#pragma unroll
-                            for (int d = 0; d < 3; d++) unsafeAtomicAdd(&A[o + d], (double)(wf * vhat[d]));
-                            o += 3;
-                        }
-                }
-            }
-}
-
-// One block per cell; its waves take the cell's particles 64 at a time.
-template <int NF, typename T, int WX, int WY, int WZ>
-__global__ __launch_bounds__(256) void k_stage(StageArgs<T> a)
-{
-    constexpr int nh = NH<NF>;
-    constexpr int QG = 4 * nh;
-    const Geom& g = a.g;
-    const int cell = blockIdx.x;
-    const int cx = cell % g.n[0], cy = (cell / g.n[0]) % g.n[1], cz = cell / (g.n[0] * g.n[1]);
-    const int beg = a.cell_start[cell], end = a.cell_start[cell + 1];
-    const int nw = blockDim.x / kWave, wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    const double dt = *a.dt;
-    // Classical RK4 (AMReX RKIntegrator, rk.type = 4): stage weights b and the
-    // coefficient of the next stage state, both scaled by dt in double.
-    const double bw[4] = {1. / 6., 1. / 3., 1. / 3., 1. / 6.};
-    const double cnext[4] = {0.5, 0.5, 1.0, 0.0};
-    const int s = a.stage;
-
-    for (int base = beg + wave * kWave; base < end; base += nw * kWave) {
-        const int64_t ip = base + lane;
-        if (ip >= end) continue;
-        double pos[3];
-#pragma unroll
-        for (int d = 0; d < 3; d++) pos[d] = a.pos[d][ip];
-        const T pupt = a.pup[3][ip];
-        const T vhat[3] = {a.pup[0][ip] / pupt, a.pup[1][ip] / pupt, a.pup[2][ip] / pupt};
-        const T N = a.Nn[0][ip], Nb = a.Nn[1][ip];
-        T Y[2 * nh];
-        if (s <= 1) {
-#pragma unroll
-            for (int c = 0; c < 2 * nh; c++) Y[c] = a.f[(int64_t)c * a.np + ip];
-        } else {
-#pragma unroll
-            for (int c = 0; c < 2 * nh; c++) Y[c] = a.Y[(int64_t)c * a.np + ip];
-        }
-        if (s == 0) { // initial deposit only
-            deposit_particle<NF, T, WX, WY, WZ>(a.gout, a.aux, a.want_aux, g, pos, vhat, N, Nb, Y, Y + nh);
-            continue;
-        }
-        // ---- gather: inside = sum_s w_s (M - vhat . Phi) over the stencil
-        int idx[3];
-        double wd[3][3];
-#pragma unroll
-        for (int d = 0; d < 3; d++) shape3((pos[d] - g.plo[d]) * g.dxi[d], g.order[d], idx[d], wd[d]);
-        T inside[nh];
-#pragma unroll
-        for (int c = 0; c < nh; c++) inside[c] = T(0);
-#pragma unroll
-        for (int kk = 0; kk < WZ; kk++)
-#pragma unroll
-            for (int jj = 0; jj < WY; jj++)
-#pragma unroll
-                for (int ii = 0; ii < WX; ii++) {
-                    const int ox = WX == 1 ? 1 : ii, oy = WY == 1 ? 1 : jj, oz = WZ == 1 ? 1 : kk;
-                    const int ci = pmod(cx + ox - 1, g.n[0]), cj = pmod(cy + oy - 1, g.n[1]),
-                              ck = pmod(cz + oz - 1, g.n[2]);
-                    const double* G = a.gin + (((int64_t)ck * g.n[1] + cj) * g.n[0] + ci) * QG;
-                    const T w = T(wd[0][ox] * wd[1][oy] * wd[2][oz]);
-                    const T wx = w * vhat[0], wy = w * vhat[1], wz = w * vhat[2];
-#pragma unroll
-                    for (int c = 0; c < nh; c++)
-                        inside[c] += w * T(G[c]) - wx * T(G[nh + c]) - wy * T(G[2 * nh + c]) - wz * T(G[3 * nh + c]);
-                }
-        // ---- Hamiltonians / hbar (1/s): H = H_vac + coef*inside, Hbar = conj(H_vac - coef*inside)
-        T H[nh], Hb[nh];
-        const T inv_pupt = T(1) / pupt;
-#pragma unroll
-        for (int c = 0; c < nh; c++) {
-            const T hv = T(a.vac[c]) * inv_pupt;
-            const T hs = T(a.coef) * inside[c];
-            H[c] = hv + hs;
-            Hb[c] = is_im_comp<NF>(c) ? -(hv - hs) : hv - hs;
-        }
-        T k[2 * nh];
-        commutator_rhs<NF, T>(H, Y, k, T(1));
-        commutator_rhs<NF, T>(Hb, Y + nh, k + nh, T(1));
-        // ---- RK update
-        const T a_acc = T(dt * bw[s - 1]);
-        if (s < 4) {
-            const T a_next = T(dt * cnext[s - 1]);
-            T yn[2 * nh], Ynext[2 * nh];
-#pragma unroll
-            for (int c = 0; c < 2 * nh; c++) {
-                yn[c] = s == 1 ? Y[c] : a.f[(int64_t)c * a.np + ip];
-                const T acc_old = s == 1 ? yn[c] : a.acc[(int64_t)c * a.np + ip];
-                a.acc[(int64_t)c * a.np + ip] = a_acc * k[c] + acc_old;
-                Ynext[c] = a_next * k[c] + yn[c];
-                a.Y[(int64_t)c * a.np + ip] = Ynext[c];
-            }
-            deposit_particle<NF, T, WX, WY, WZ>(a.gout, a.aux, s == 3 && a.want_aux, g, pos, vhat, N, Nb, Ynext,
-                                                Ynext + nh);
-        } else {
-            // y_{n+1} = acc + (dt/6) k4, then the post-step hook (main.cpp:155-199)
-            double y[2 * nh];
-#pragma unroll
-            for (int c = 0; c < 2 * nh; c++) y[c] = double(a_acc * k[c] + a.acc[(int64_t)c * a.np + ip]);
-            int err = renorm_fp64<NF>(y, double(a.Ll[0][ip]), a.maxError);
-            if (err == EMU_OK) err = renorm_fp64<NF>(y + nh, double(a.Ll[1][ip]), a.maxError);
-            if (err != EMU_OK) atomicMin(a.err, ((unsigned long long)a.id[ip] << 3) | (unsigned long long)err);
-            T ynew[2 * nh];
-#pragma unroll
-            for (int c = 0; c < 2 * nh; c++) {
-                ynew[c] = T(y[c]);
-                a.f[(int64_t)c * a.np + ip] = ynew[c];
-            }
-            // positions: x += (dt b_i) c pup/pupt for i = 1..4, then the periodic shift
-            const double ai[4] = {dt * bw[0], dt * bw[1], dt * bw[2], dt * bw[3]};
-            double npos[3];
-            int nc[3];
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-                const double v = double(a.pup[d][ip]) / double(pupt) * phys::c;
-                npos[d] = wrap1(advance_pos(pos[d], v, ai), g.rlo[d], g.rhi[d], g.L[d]);
-                a.pos[d][ip] = npos[d];
-                nc[d] = bin1(npos[d], g.plo[d], g.dxi[d], g.n[d]);
-            }
-            const int newc = (nc[2] * g.n[1] + nc[1]) * g.n[0] + nc[0];
-            a.newcell[ip] = newc;
-            deposit_particle<NF, T, WX, WY, WZ>(a.gout, a.aux, false, g, npos, vhat, N, Nb, ynew, ynew + nh);
-        }
-    }
-}
-
 // ---------------------------------------------------------------------------
 // Stable counting sort by cell after each step.  A particle moves at most one
 // cell per dimension per step (c dt < dx, checked on the host), so its move
@@ -329,6 +56,13 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs<T> a)
 // particle's destination is base[old cell][e] + (its rank among the particles
 // of its old cell with the same e), which makes the order deterministic.
 // ---------------------------------------------------------------------------
+// FP32 copy of a deposited grid for the FP32-bulk gather.
+__global__ void k_grid_to_f32(const double* __restrict__ g, float* __restrict__ g32, int64_t n)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+        g32[t] = (float)g[t];
+}
+
 __device__ __forceinline__ int move1(int from, int to, int n, int* bad)
 {
     const int dd = pmod(to - from, n);
@@ -423,40 +157,7 @@ __global__ void k_bases(const int* __restrict__ m, const int* __restrict__ new_s
     }
 }
 
-template <typename T> struct Arrays {
-    double* pos[3];
-    T* pup[4];
-    T* Nn[2];
-    T* Ll[2];
-    T* f;
-    int64_t* id;
-};
-
-template <typename T>
-__global__ void k_scatter(Arrays<T> src, Arrays<T> dst, const int* __restrict__ cell_start,
-                          const int* __restrict__ base, const int* __restrict__ rank,
-                          const unsigned char* __restrict__ mv, int64_t np, int ncomp, unsigned long long* err)
-{
-    const int c = blockIdx.x;
-    const int beg = cell_start[c], end = cell_start[c + 1];
-    for (int ip = beg + threadIdx.x; ip < end; ip += blockDim.x) {
-        const int64_t j = base[(int64_t)c * 27 + mv[ip]] + rank[ip];
-        if (j < 0 || j >= np) { // cannot happen for a consistent sort; never write outside the arrays
-            atomicMin(err, (unsigned long long)EMU_ERR_MOVE | (~0ull << 3));
-            continue;
-        }
-#pragma unroll
-        for (int d = 0; d < 3; d++) dst.pos[d][j] = src.pos[d][ip];
-#pragma unroll
-        for (int d = 0; d < 4; d++) dst.pup[d][j] = src.pup[d][ip];
-        dst.Nn[0][j] = src.Nn[0][ip];
-        dst.Nn[1][j] = src.Nn[1][ip];
-        dst.Ll[0][j] = src.Ll[0][ip];
-        dst.Ll[1][j] = src.Ll[1][ip];
-        dst.id[j] = src.id[ip];
-        for (int q = 0; q < ncomp; q++) dst.f[(int64_t)q * np + j] = src.f[(int64_t)q * np + ip];
-    }
-}
+template <typename T> using Arrays = PArrays<T>;
 
 // compute_dt over the cells (Evolve.cpp:40-56) from the combined fields and
 // the auxiliaries.  HSI = N - conj(Nbar) + matter = M; F - Fbar keeps Re and the
@@ -711,6 +412,7 @@ struct emu_state {
     int* mbase = nullptr;          // [ncell][27] destination of the first of them
     unsigned char* mv = nullptr;   // move index per particle
     double* grid[2] = {nullptr, nullptr};
+    float* grid32 = nullptr;       // FP32 copy for the FP32-bulk gather
     double* aux = nullptr;
     double* scal = nullptr;        // [0] time [1] dt [2] dt_next
     int64_t* steps = nullptr;
@@ -721,7 +423,6 @@ struct emu_state {
     int gcur = 0;     // grid holding the deposit of the current state
     double Vvac_max = 0.0;
     double m00 = 0.0;
-    int blocksize = 64;
     // timing: an event pair around every profiled launch, by kernel class
     int profile = 0;
     struct Mark { int k; hipEvent_t a, b; };
@@ -778,52 +479,57 @@ int scan_counts(emu_state* st, int* out)
     return EMU_OK;
 }
 
+constexpr int kWPB = 2; // waves (cells in flight) per block of the stage kernel
+
 template <int NF, typename T, int WX, int WY, int WZ>
 void launch_stage_t(emu_state* st, StageArgs<T>& a)
 {
-    hipLaunchKernelGGL((k_stage<NF, T, WX, WY, WZ>), dim3((unsigned)st->ncell), dim3(st->blocksize), 0,
-                       st->ctx->stream, a);
+    constexpr int nh = NH<NF>;
+    const int aux = a.want_aux && (a.stage == 3 || a.stage == 0);
+    const size_t lds = (size_t)kWPB * 64 * (9 + (nh + 3) + (aux ? 2 * NF + NF * (NF - 1) / 2 : 0)) * sizeof(double);
+    const int64_t nblk = (st->ncell + kWPB - 1) / kWPB;
+    a.blocks_per_xcd = (int)((nblk + 7) / 8);
+    hipLaunchKernelGGL((k_stage<NF, T, WX, WY, WZ, kWPB>), dim3((unsigned)(8 * a.blocks_per_xcd)), dim3(64 * kWPB),
+                       lds, st->ctx->stream, a);
 }
 
+// Stencil shapes: 3 cells per dimension with shape order 1 or 2, 1 cell with
+// order 0 or in a 1-cell dimension.  (3,3,3), (1,1,3) and (1,1,1) have their
+// own kernels; other shapes run the (3,3,3) kernel, whose extra entries carry
+// zero weight.
 template <int NF, typename T> void launch_stage_shape(emu_state* st, StageArgs<T>& a)
 {
     const int* o = st->ctx->g.order;
     const int wx = o[0] > 0 ? 3 : 1, wy = o[1] > 0 ? 3 : 1, wz = o[2] > 0 ? 3 : 1;
-    const int key = (wx == 3) * 4 + (wy == 3) * 2 + (wz == 3);
-    switch (key) {
-    case 0: launch_stage_t<NF, T, 1, 1, 1>(st, a); break;
-    case 1: launch_stage_t<NF, T, 1, 1, 3>(st, a); break;
-    case 2: launch_stage_t<NF, T, 1, 3, 1>(st, a); break;
-    case 3: launch_stage_t<NF, T, 1, 3, 3>(st, a); break;
-    case 4: launch_stage_t<NF, T, 3, 1, 1>(st, a); break;
-    case 5: launch_stage_t<NF, T, 3, 1, 3>(st, a); break;
-    case 6: launch_stage_t<NF, T, 3, 3, 1>(st, a); break;
-    default: launch_stage_t<NF, T, 3, 3, 3>(st, a); break;
-    }
+    if (wx == 1 && wy == 1 && wz == 1) launch_stage_t<NF, T, 1, 1, 1>(st, a);
+    else if (wx == 1 && wy == 1) launch_stage_t<NF, T, 1, 1, 3>(st, a);
+    else launch_stage_t<NF, T, 3, 3, 3>(st, a);
 }
 
-template <typename T> StageArgs<T> stage_args(emu_state* st, int stage, const double* gin, double* gout, bool want_aux)
+template <typename T>
+StageArgs<T> stage_args(emu_state* st, int stage, const double* gin, double* gout, bool want_aux)
 {
     StageArgs<T> a;
-    Arrays<T> A = arrays_of<T>(st, st->cur);
-    for (int d = 0; d < 3; d++) a.pos[d] = A.pos[d];
-    for (int d = 0; d < 4; d++) a.pup[d] = A.pup[d];
-    for (int d = 0; d < 2; d++) { a.Nn[d] = A.Nn[d]; a.Ll[d] = A.Ll[d]; }
-    a.f = A.f;
-    a.id = A.id;
+    a.cur = arrays_of<T>(st, st->cur);
+    a.dst = arrays_of<T>(st, 1 - st->cur);
     a.Y = (T*)st->Y;
     a.acc = (T*)st->acc;
     a.newcell = st->newcell;
+    a.mbase = st->mbase;
     a.rank = st->rank;
+    a.mv = st->mv;
     a.cell_start = st->cell_start;
     a.gin = gin;
+    a.gin32 = st->grid32;
     a.gout = gout;
     a.aux = st->aux;
     a.dt = st->scal + 1;
     a.err = st->red + 2;
     a.np = st->np;
+    a.ncell = st->ncell;
     a.stage = stage;
     a.want_aux = want_aux ? 1 : 0;
+    a.blocks_per_xcd = 0;
     a.g = st->ctx->g;
     const emu_ctx* ctx = st->ctx;
     for (int c = 0; c < 9; c++) a.vac[c] = c < st->nh ? ctx->M2[c] * phys::c4 / 2.0 / phys::hbar : 0.0;
@@ -835,6 +541,9 @@ template <typename T> StageArgs<T> stage_args(emu_state* st, int stage, const do
 void launch_stage(emu_state* st, int stage, const double* gin, double* gout, bool want_aux)
 {
     if (st->precision == EMU_FP32_BULK) {
+        if (gin) // the FP32-bulk gather reads an FP32 copy of the grid
+            hipLaunchKernelGGL(k_grid_to_f32, dim3(grid_blocks(st->ncell * st->QG)), dim3(256), 0, st->ctx->stream,
+                               gin, st->grid32, st->ncell * st->QG);
         auto a = stage_args<float>(st, stage, gin, gout, want_aux);
         if (st->nf == 2) launch_stage_shape<2, float>(st, a);
         else launch_stage_shape<3, float>(st, a);
@@ -926,6 +635,8 @@ extern "C" int emu_state_create(emu_ctx* ctx, int64_t np, int precision, emu_sta
               hipMalloc(&st->grid[0], (size_t)st->ncell * st->QG * sizeof(double)) == hipSuccess &&
               hipMalloc(&st->grid[1], (size_t)st->ncell * st->QG * sizeof(double)) == hipSuccess &&
               hipMalloc(&st->aux, (size_t)st->ncell * st->naux * sizeof(double)) == hipSuccess &&
+              (precision != EMU_FP32_BULK ||
+               hipMalloc(&st->grid32, (size_t)st->ncell * st->QG * sizeof(float)) == hipSuccess) &&
               hipMalloc(&st->scal, 4 * sizeof(double)) == hipSuccess &&
               hipMalloc(&st->steps, sizeof(int64_t)) == hipSuccess &&
               hipMalloc(&st->red, 8 * sizeof(unsigned long long)) == hipSuccess;
@@ -933,9 +644,6 @@ extern "C" int emu_state_create(emu_ctx* ctx, int64_t np, int precision, emu_sta
         emu_state_destroy(st);
         return set_error(ctx, EMU_ERR_NOMEM, "state allocation failed");
     }
-    // waves per block from the average number of particles per cell
-    const double ppc = st->ncell > 0 ? (double)np / (double)st->ncell : 0.0;
-    st->blocksize = ppc > 192 ? 256 : ppc > 128 ? 192 : ppc > 64 ? 128 : 64;
     *out = st;
     return EMU_OK;
 }
@@ -944,7 +652,7 @@ extern "C" int emu_state_destroy(emu_state* st)
 {
     if (!st) return EMU_OK;
     void* ptrs[] = {st->block[0], st->block[1], st->Y, st->acc, st->newcell, st->rank, st->counts, st->cell_start,
-                    st->new_start, st->mcount, st->mbase, st->mv, st->grid[0], st->grid[1], st->aux, st->scal,
+                    st->new_start, st->mcount, st->mbase, st->mv, st->grid[0], st->grid[1], st->grid32, st->aux, st->scal,
                     st->steps, st->red, st->cub_tmp};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -1070,13 +778,6 @@ extern "C" int emu_state_store(emu_state* st, emu_particles* parts, int64_t* ids
     return EMU_OK;
 }
 
-template <typename T> static void scatter_t(emu_state* st)
-{
-    Arrays<T> src = arrays_of<T>(st, st->cur), dst = arrays_of<T>(st, 1 - st->cur);
-    hipLaunchKernelGGL(k_scatter<T>, dim3((unsigned)st->ncell), dim3(64), 0, st->ctx->stream, src, dst,
-                       st->cell_start, st->mbase, st->rank, st->mv, st->np, 2 * st->nh, st->red + 2);
-}
-
 extern "C" int emu_state_evolve(emu_state* st, int nsteps, double fixed_dt)
 {
     if (!st || nsteps < 0) return EMU_ERR_ARG;
@@ -1090,15 +791,26 @@ extern "C" int emu_state_evolve(emu_state* st, int nsteps, double fixed_dt)
             return set_error(ctx, EMU_ERR_MOVE, "fused step needs c*dt < dx (0 < cfl_factor < 1)");
     if (fixed_dt > 0) EMU_HIP(ctx, hipMemcpyAsync(st->scal + 1, &fixed_dt, sizeof(double), hipMemcpyHostToDevice, s));
     for (int step = 0; step < nsteps; step++) {
-        double* A = st->grid[st->gcur];
+        double* A = st->grid[st->gcur];     // holds the deposit of y_n
         double* B = st->grid[1 - st->gcur];
         const bool want_aux = ctx->p.flavor_cfl_factor > 0.0 && fixed_dt <= 0;
         ev_record(st, 1, true);
         fill_grid(st, B, false);
         ev_record(st, 1, false);
         ev_record(st, 0, true);
-        launch_stage(st, 1, A, B, false);
+        launch_stage(st, 1, A, B, false);   // also bins the end-of-step positions
         ev_record(st, 0, false);
+        // stable counting-sort plan by the new cell
+        ev_record(st, 2, true);
+        hipLaunchKernelGGL(k_move_ranks, dim3((unsigned)st->ncell), dim3(64), 0, s, st->newcell, st->cell_start,
+                           ctx->g, st->rank, st->mv, st->mcount, st->red + 2);
+        hipLaunchKernelGGL(k_new_counts, dim3(grid_blocks(st->ncell)), dim3(256), 0, s, st->mcount, ctx->g,
+                           st->ncell, st->counts);
+        int e = scan_counts(st, st->new_start);
+        if (e) return e;
+        hipLaunchKernelGGL(k_bases, dim3(grid_blocks(st->ncell)), dim3(256), 0, s, st->mcount, st->new_start, ctx->g,
+                           st->ncell, st->mbase);
+        ev_record(st, 2, false);
         ev_record(st, 1, true);
         fill_grid(st, A, false);
         ev_record(st, 1, false);
@@ -1112,31 +824,21 @@ extern "C" int emu_state_evolve(emu_state* st, int nsteps, double fixed_dt)
         launch_stage(st, 3, A, B, want_aux);
         ev_record(st, 0, false);
         ev_record(st, 3, true);
-        int e = dt_from_grid(st, B, fixed_dt);
+        e = dt_from_grid(st, B, fixed_dt);
         ev_record(st, 3, false);
         if (e) return e;
+        ev_record(st, 0, true);
+        launch_stage(st, 4, B, nullptr, false); // writes y_{n+1} into the other copy, new order
+        ev_record(st, 0, false);
+        st->cur = 1 - st->cur;
+        std::swap(st->cell_start, st->new_start);
+        // deposit of y_{n+1} for the next step's stage 1
         ev_record(st, 1, true);
         fill_grid(st, A, false);
         ev_record(st, 1, false);
         ev_record(st, 0, true);
-        launch_stage(st, 4, B, A, false);
+        launch_stage(st, 0, nullptr, A, false);
         ev_record(st, 0, false);
-        // stable counting sort by the new cell
-        ev_record(st, 2, true);
-        hipLaunchKernelGGL(k_move_ranks, dim3((unsigned)st->ncell), dim3(64), 0, s, st->newcell, st->cell_start,
-                           ctx->g, st->rank, st->mv, st->mcount, st->red + 2);
-        hipLaunchKernelGGL(k_new_counts, dim3(grid_blocks(st->ncell)), dim3(256), 0, s, st->mcount, ctx->g,
-                           st->ncell, st->counts);
-        e = scan_counts(st, st->new_start);
-        if (e) return e;
-        hipLaunchKernelGGL(k_bases, dim3(grid_blocks(st->ncell)), dim3(256), 0, s, st->mcount, st->new_start, ctx->g,
-                           st->ncell, st->mbase);
-        if (st->precision == EMU_FP32_BULK) scatter_t<float>(st);
-        else scatter_t<double>(st);
-        ev_record(st, 2, false);
-        st->cur = 1 - st->cur;
-        std::swap(st->cell_start, st->new_start);
-        // the stage-4 kernel deposited y_{n+1} into A for the next step
         hipLaunchKernelGGL(k_step_end, dim3(1), dim3(64), 0, s, st->scal, st->scal + 1, st->scal + 2, st->steps);
         if (fixed_dt > 0) EMU_HIP(ctx, hipMemcpyAsync(st->scal + 1, &fixed_dt, sizeof(double), hipMemcpyHostToDevice, s));
         EMU_LAUNCH_CHECK(ctx, "state_evolve");

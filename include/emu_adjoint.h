@@ -1,0 +1,53 @@
+/*
+ * emu_adjoint.h -- the flavour right-hand side and its vector-Jacobian product
+ * on particles kept in a fixed order, for reverse-mode gradients of the RK4
+ * evolution (no counterpart in the reference, which has no adjoint).
+ *
+ * Arrays are device SoA [component][np] in double precision:
+ *   pos [3][np], pup [4][np] (pupx, pupy, pupz, pupt), Nn [2][np] (N, Nbar),
+ *   Y / k / lam [2 NF^2][np] (nu components then nubar, PIdx order of f),
+ *   Ll [2][np] (L, Lbar).
+ * 'work' is caller-owned device scratch of emu_adj_work_doubles() doubles.
+ */
+#ifndef EMU_ADJOINT_H_
+#define EMU_ADJOINT_H_
+
+#include "emu_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+int64_t emu_adj_work_doubles(const emu_ctx* ctx, int64_t np);
+
+/* k = d(f, fbar)/dt of interpolate_rhs_from_mesh (Evolve.cpp:128-176) after
+ * deposit_to_mesh of Y (Evolve.cpp:91-126) at positions pos. */
+int emu_adj_rhs(emu_ctx* ctx, int64_t np, const double* pos, const double* pup, const double* Nn, const double* Y,
+                double* k, double* work);
+
+/* lam_Y = (dk/dY)^T lam_k (overwritten); lam_Nbar += (dk/dNbar)^T lam_k when
+ * lam_Nbar is not NULL. */
+int emu_adj_rhs_vjp(emu_ctx* ctx, int64_t np, const double* pos, const double* pup, const double* Nn,
+                    const double* Y, const double* lam_k, double* lam_Y, double* lam_Nbar, double* work);
+
+/* J = sum over cells of |N_01|^2, N_01 = sum_p w N f01 (the nu part of the
+ * N01 grid components); dJdY (optional) receives dJ/dY.  Synchronous. */
+int emu_adj_objective(emu_ctx* ctx, int64_t np, const double* pos, const double* Nn, const double* Y, double* J,
+                      double* dJdY, double* work);
+
+/* Positions after one RK4 step of length dt and the periodic shift (the same
+ * arithmetic as the fused step). */
+int emu_adj_advance_positions(emu_ctx* ctx, int64_t np, double* pos, const double* pup, double dt);
+
+/* Renormalize (FlavoredNeutrinoContainer.cpp:89-113) on the SoA state.
+ * Synchronous; returns the first error as emu_renormalize does. */
+int emu_adj_renormalize(emu_ctx* ctx, int64_t np, double* Y, const double* Ll, int64_t* bad_index);
+
+/* out = a x + y, elementwise over n doubles. */
+int emu_adj_axpy(emu_ctx* ctx, int64_t n, double a, const double* x, const double* y, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,112 @@
+"""The hand-written adjoint: forward RHS against the oracle, the VJP by the
+dot-product test, and whole-run gradients against central finite differences
+(1e-5 relative) through torch.autograd."""
+import numpy as np
+import pytest
+
+import cases
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def mods():
+    import torch
+    assert torch.cuda.is_available()
+    from emu_amd import adjoint, core
+    return torch, core, adjoint
+
+
+def setup(mods, oracle, nf, ncell=(4, 4, 4), nphi=4, amp=1e-4, **kw):
+    torch, core, adjoint = mods
+    d = cases.fast_flavor_dict(nf=nf, ncell=ncell, nphi=nphi, st4_amplitude=amp, maxError=1e-2, **kw)
+    deck = cases.oracle_deck(oracle, d)
+    parts = oracle.init_particles(deck)
+    emu = core.Emu(cases.product_params(d))
+    c = core.FlavoredNeutrinoContainer.from_aos(emu, parts)
+    return emu, c, deck, parts
+
+
+@pytest.mark.parametrize("nf", [2, 3])
+def test_adj_rhs_matches_oracle(mods, oracle, nf):
+    torch, core, adjoint = mods
+    emu, c, deck, parts = setup(mods, oracle, nf, rho_g_ccm=1e9, Ye=0.5)
+    prob = adjoint.AdjointProblem(emu, c, dt=1e-13, nsteps=1)
+    k = prob.rhs(prob.pos0, prob.Nn_base, prob.f0).cpu().numpy()
+    prm = deck.params()
+    g = oracle.Grid(nf, prm)
+    oracle.deposit_to_mesh(nf, prm, parts, g)
+    oracle.fill_boundary(nf, prm, g)
+    rhs = parts.copy()
+    oracle.interpolate_rhs_from_mesh(nf, prm, rhs, g)
+    K = emu.K
+    nh = nf * nf
+    want = np.concatenate([rhs["rdata"][:, K["f00_Re"]:K["f00_Re"] + nh],
+                           rhs["rdata"][:, K["f00_Rebar"]:K["f00_Rebar"] + nh]], axis=1)
+    assert cases.max_rel_err(k.T, want) < 1e-12
+
+
+@pytest.mark.parametrize("nf", [2, 3])
+def test_vjp_dot_product(mods, oracle, nf):
+    """F is quadratic in Y (H is affine in Y, the commutator bilinear), so the
+    central difference of lam . F is exact up to roundoff."""
+    torch, core, adjoint = mods
+    emu, c, deck, parts = setup(mods, oracle, nf, amp=1e-2)
+    prob = adjoint.AdjointProblem(emu, c, dt=1e-13, nsteps=1)
+    gen = torch.Generator(device="cpu").manual_seed(3)
+    Y = prob.f0
+    lam = torch.randn(Y.shape, generator=gen, dtype=torch.float64).to(Y.device)
+    dY = torch.randn(Y.shape, generator=gen, dtype=torch.float64).to(Y.device) * 1e-3
+    Nn = prob.Nn_base
+    prob.lam_nbar.zero_()
+    lamY = prob.vjp(prob.pos0, Nn, Y, lam)
+    h = 1.0
+    fp = prob.rhs(prob.pos0, Nn, Y + h * dY)
+    fm = prob.rhs(prob.pos0, Nn, Y - h * dY)
+    lhs = float((lam * (fp - fm)).sum() / (2 * h))
+    rhs_ = float((lamY * dY).sum())
+    assert rhs_ == pytest.approx(lhs, rel=1e-9)
+    # Nbar direction
+    dN = torch.zeros_like(Nn)
+    dN[1] = torch.rand(Nn.shape[1], generator=gen, dtype=torch.float64).to(Y.device) * Nn[1] * 1e-3
+    fp = prob.rhs(prob.pos0, Nn + dN, Y)
+    fm = prob.rhs(prob.pos0, Nn - dN, Y)
+    lhs = float((lam * (fp - fm)).sum() / 2)
+    rhs_ = float((prob.lam_nbar * dN[1]).sum())
+    assert rhs_ == pytest.approx(lhs, rel=1e-6)
+
+
+@pytest.mark.parametrize("nf", [2, 3])
+def test_adjoint_gradient_matches_finite_differences(mods, oracle, nf):
+    torch, core, adjoint = mods
+    emu, c, deck, parts = setup(mods, oracle, nf, amp=1e-6)
+    dt = 0.4 * 0.5 / 2.99792458e10 * 0.25
+    prob = adjoint.AdjointProblem(emu, c, dt=dt, nsteps=8, checkpoint_every=3)
+    f0 = prob.f0.clone().requires_grad_(True)
+    ratio = torch.tensor(1.0, dtype=torch.float64, device=f0.device, requires_grad=True)
+    J = adjoint.FlavorObjective.apply(f0, ratio, prob)
+    J.backward()
+    g = f0.grad
+    nh = nf * nf
+    K = emu.K
+    i01 = K["f01_Re"] - K["f00_Re"]
+    rng = np.random.default_rng(0)
+    picks = [(i01, p) for p in rng.choice(c.np, 4, replace=False)] + \
+            [(i01 + 1, p) for p in rng.choice(c.np, 2, replace=False)] + \
+            [(nh + i01, p) for p in rng.choice(c.np, 2, replace=False)]
+    for comp, p in picks:
+        base = float(prob.f0[comp, p])
+        h = 1e-4 * abs(base) if base != 0 else 1e-10
+        fp = prob.f0.clone()
+        fp[comp, p] += h
+        fm = prob.f0.clone()
+        fm[comp, p] -= h
+        Jp, _, _, _ = prob.run(fp, 1.0, keep_checkpoints=False)
+        Jm, _, _, _ = prob.run(fm, 1.0, keep_checkpoints=False)
+        fd = (Jp - Jm) / (2 * h)
+        assert float(g[comp, p]) == pytest.approx(fd, rel=1e-5, abs=1e-5 * abs(fd) + 1e-300), (comp, p)
+    h = 1e-4
+    Jp, _, _, _ = prob.run(prob.f0, 1.0 + h, keep_checkpoints=False)
+    Jm, _, _, _ = prob.run(prob.f0, 1.0 - h, keep_checkpoints=False)
+    fd = (Jp - Jm) / (2 * h)
+    assert float(ratio.grad) == pytest.approx(fd, rel=1e-5)
