@@ -47,7 +47,7 @@ def algorithmic_bytes(nf: int, ppc: float, precision: str) -> float:
     return part + 12 * G * 8 / ppc
 
 
-def run_precision(core, parms, parts, precision, steps, warmup, profile_steps):
+def run_precision(core, parms, parts, precision, steps, warmup):
     import torch
     emu = parts.emu
     st = core.FusedState(emu, parts.np, precision)
@@ -56,8 +56,12 @@ def run_precision(core, parms, parts, precision, steps, warmup, profile_steps):
         st.evolve(warmup)
     info0 = st.info()
     a0 = st.flavor_amplitude()
+    # the library launches on the context's stream, which core.Emu binds to
+    # torch's current stream; events on that stream bracket the timed steps
     stream = torch.cuda.current_stream()
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    # HIP events around every kernel launch of the timed steps, on the same stream
+    st.profile(True)
     torch.cuda.synchronize()
     t_wall0 = time.perf_counter()
     ev0.record(stream)
@@ -66,22 +70,19 @@ def run_precision(core, parms, parts, precision, steps, warmup, profile_steps):
     torch.cuda.synchronize()
     t_wall = time.perf_counter() - t_wall0
     ms = ev0.elapsed_time(ev1)
-    info1 = st.info()
-    a1 = st.flavor_amplitude()
-    rate = math.log(a1[0] / a0[0]) / (info1["time"] - info0["time"]) if a0[0] > 0 and a1[0] > 0 else float("nan")
-    # per-launch kernel times (separate short run so events do not perturb the timed loop)
-    st.profile(True)
-    st.evolve(profile_steps)
     stage_ms, n_stage = st.kernel_ms(0)
     fill_ms, _ = st.kernel_ms(1)
     sort_ms, _ = st.kernel_ms(2)
     dt_ms, _ = st.kernel_ms(3)
     st.profile(False)
-    st.info()
+    info1 = st.info()
+    a1 = st.flavor_amplitude()
+    rate = math.log(a1[0] / a0[0]) / (info1["time"] - info0["time"]) if a0[0] > 0 and a1[0] > 0 else float("nan")
     st.close()
     return {"ms_total": ms, "wall_s": t_wall, "ms_per_step": ms / steps, "updates_per_s": parts.np * steps / (ms * 1e-3),
             "growth_rate": rate, "time_window": [info0["time"], info1["time"]], "amp": [a0[0], a1[0]],
-            "stage_ms": stage_ms, "stage_launches": n_stage, "fill_ms": fill_ms, "sort_ms": sort_ms, "dt_ms": dt_ms}
+            "stage_ms": stage_ms, "stage_launches_per_step": n_stage / steps, "fill_ms": fill_ms, "sort_ms": sort_ms,
+            "dt_ms": dt_ms}
 
 
 def cpu_baseline(parms_dict, budget_s=15.0):
@@ -130,9 +131,11 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="ff64")
-    ap.add_argument("--profile-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--precision", default="both", choices=["both", "fp64", "fp32"])
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device (5 times): the known-size kernel that scales the "
+                         "FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this build runs on a single MI355X; "
@@ -140,6 +143,15 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.copy_calibration:
+        n = args.copy_calibration * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_each_way": n * 8, "copies": 5}), flush=True)
+        return
     from emu_amd import capi, configs, core
     capi.lib()  # fails loudly without the HIP library
     assert torch.cuda.is_available(), "bench.py needs a GPU"
@@ -152,14 +164,19 @@ def main():
     res = {}
     precs = ["fp64", "fp32"] if args.precision == "both" else [args.precision]
     for prec in precs:
-        res[prec] = run_precision(core, parms, parts, prec, args.steps, args.warmup, args.profile_steps)
+        res[prec] = run_precision(core, parms, parts, prec, args.steps, args.warmup)
         torch.cuda.empty_cache()
     head = "fp32" if "fp32" in res else "fp64"
     r = res[head]
     bpu = algorithmic_bytes(parms.nf, ppc, head)
-    # one stage-kernel launch processes every particle for one of the four stages
-    bytes_per_launch = parts.np * bpu / 4.0
-    achieved = bytes_per_launch / (r["stage_ms"] * 1e-3) / 1e9 if r["stage_ms"] > 0 else None
+    # a step is stage_launches_per_step launches of k_stage (four RK stages and the
+    # deposit of the step's result); the step's algorithmic bytes are spread over them
+    def achieved_gbs(rr, b):
+        if rr["stage_ms"] <= 0:
+            return None
+        return parts.np * b / rr["stage_launches_per_step"] / (rr["stage_ms"] * 1e-3) / 1e9
+
+    achieved = achieved_gbs(r, bpu)
     out = {
         "metric": "particle-updates/s FP32-bulk vs FP64 plain, 1 GPU, at <=1% growth err; %HBM peak",
         "value": r["updates_per_s"],
@@ -178,15 +195,15 @@ def main():
                    "particles": parts.np, "precision": head},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
-                     "kernel": "k_stage (gather + RHS + RK update + deposit, one launch per RK stage)",
-                     "algorithmic_bytes_per_update": bpu, "avg_launch_ms": r["stage_ms"]},
+                     "kernel": "k_stage (gather + RHS + RK update + deposit; four RK stages and the step-end deposit)",
+                     "algorithmic_bytes_per_update": bpu, "avg_launch_ms": r["stage_ms"],
+                     "launches_per_step": r["stage_launches_per_step"]},
     }
     if "fp64" in res:
         f64 = res["fp64"]
         b64 = algorithmic_bytes(parms.nf, ppc, "fp64")
         out["fp64"] = {"value": f64["updates_per_s"], "ms_per_step": f64["ms_per_step"], "stage_ms": f64["stage_ms"],
-                       "roofline_frac": (parts.np * b64 / 4.0 / (f64["stage_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS)
-                       if f64["stage_ms"] > 0 else None,
+                       "roofline_frac": (achieved_gbs(f64, b64) or 0.0) / HBM_PEAK_GBS,
                        "growth_rate": f64["growth_rate"]}
     if "fp32" in res and "fp64" in res:
         g32, g64 = res["fp32"]["growth_rate"], res["fp64"]["growth_rate"]

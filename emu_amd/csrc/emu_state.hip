@@ -284,7 +284,8 @@ __global__ void k_iota(int* p, int64_t n)
 
 // Load: sorted slot j takes input particle order[j] (stable radix sort by cell).
 template <int NF, typename T>
-__global__ void k_load(PView P, const int64_t* ids, const int* __restrict__ order, Arrays<T> dst, int64_t np)
+__global__ void k_load(PView P, const int64_t* ids, const int* __restrict__ order, Arrays<T> dst, int64_t np,
+                       double nscale)
 {
     using I = PIdx<NF>;
     constexpr int nh = NH<NF>;
@@ -292,8 +293,8 @@ __global__ void k_load(PView P, const int64_t* ids, const int* __restrict__ orde
         const int64_t ip = order[j];
         for (int d = 0; d < 3; d++) dst.pos[d][j] = P.pos[d][ip];
         for (int d = 0; d < 4; d++) dst.pup[d][j] = T(P.r[I::pupx + d][ip]);
-        dst.Nn[0][j] = T(P.r[I::N][ip]);
-        dst.Nn[1][j] = T(P.r[I::Nbar][ip]);
+        dst.Nn[0][j] = T(P.r[I::N][ip] * nscale);
+        dst.Nn[1][j] = T(P.r[I::Nbar][ip] * nscale);
         dst.Ll[0][j] = T(P.r[I::L][ip]);
         dst.Ll[1][j] = T(P.r[I::Lbar][ip]);
         dst.id[j] = ids ? ids[ip] : ip;
@@ -305,7 +306,7 @@ __global__ void k_load(PView P, const int64_t* ids, const int* __restrict__ orde
 }
 
 template <int NF, typename T>
-__global__ void k_store(Arrays<T> src, PView P, int64_t* ids, const double* time, int64_t np)
+__global__ void k_store(Arrays<T> src, PView P, int64_t* ids, const double* time, int64_t np, double nscale)
 {
     using I = PIdx<NF>;
     constexpr int nh = NH<NF>;
@@ -317,8 +318,8 @@ __global__ void k_store(Arrays<T> src, PView P, int64_t* ids, const double* time
         }
         P.r[I::time][ip] = t;
         for (int d = 0; d < 4; d++) P.r[I::pupx + d][ip] = double(src.pup[d][ip]);
-        P.r[I::N][ip] = double(src.Nn[0][ip]);
-        P.r[I::Nbar][ip] = double(src.Nn[1][ip]);
+        P.r[I::N][ip] = double(src.Nn[0][ip]) / nscale;
+        P.r[I::Nbar][ip] = double(src.Nn[1][ip]) / nscale;
         P.r[I::L][ip] = double(src.Ll[0][ip]);
         P.r[I::Lbar][ip] = double(src.Ll[1][ip]);
         for (int c = 0; c < nh; c++) {
@@ -327,6 +328,18 @@ __global__ void k_store(Arrays<T> src, PView P, int64_t* ids, const double* time
         }
         if (ids) ids[ip] = src.id[ip];
     }
+}
+
+// max(|N|, |Nbar|) as the bit pattern of a non-negative double (those order
+// like unsigned integers)
+__global__ void k_absmax2(const double* __restrict__ a, const double* __restrict__ b, int64_t np,
+                          unsigned long long* red)
+{
+    double v = 0.0;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < np; t += (int64_t)gridDim.x * blockDim.x)
+        v = fmax(v, fmax(fabs(a[t]), fabs(b[t])));
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+    if ((threadIdx.x & 63) == 0) atomicMax(red, (unsigned long long)__double_as_longlong(v));
 }
 
 __global__ void k_min_pupt_t(const double* __restrict__ pupt, int64_t np, unsigned long long* red)
@@ -423,6 +436,13 @@ struct emu_state {
     int gcur = 0;     // grid holding the deposit of the current state
     double Vvac_max = 0.0;
     double m00 = 0.0;
+    // The particle weights are held as N * nscale, nscale = 2^-k chosen at load
+    // so that the largest weight is in [1, 2): FP32 cannot hold N = n dV for
+    // large cells (1e54 for a (1e7 cm)^3 cell), and a power of two scales every
+    // product and sum exactly.  The deposited fields carry the same factor,
+    // which coef, the matter term and the dt estimate take back out.  The FP64
+    // state keeps nscale = 1.
+    double nscale = 1.0;
     // timing: an event pair around every profiled launch, by kernel class
     int profile = 0;
     struct Mark { int k; hipEvent_t a, b; };
@@ -533,7 +553,7 @@ StageArgs<T> stage_args(emu_state* st, int stage, const double* gin, double* gou
     a.g = st->ctx->g;
     const emu_ctx* ctx = st->ctx;
     for (int c = 0; c < 9; c++) a.vac[c] = c < st->nh ? ctx->M2[c] * phys::c4 / 2.0 / phys::hbar : 0.0;
-    a.coef = M_SQRT2 * phys::GF * ctx->g.inv_cell_volume / phys::hbar;
+    a.coef = M_SQRT2 * phys::GF * ctx->g.inv_cell_volume / phys::hbar / st->nscale;
     a.maxError = ctx->p.maxError;
     return a;
 }
@@ -557,7 +577,7 @@ void launch_stage(emu_state* st, int stage, const double* gin, double* gout, boo
 void fill_grid(emu_state* st, double* g, bool with_aux)
 {
     hipLaunchKernelGGL(k_fill_grid, dim3(grid_blocks(st->ncell * st->QG)), dim3(256), 0, st->ctx->stream, g,
-                       st->ncell, st->QG, st->m00, with_aux ? st->aux : nullptr, st->naux);
+                       st->ncell, st->QG, st->m00 * st->nscale, with_aux ? st->aux : nullptr, st->naux);
 }
 
 int dt_from_grid(emu_state* st, const double* g, double fixed_dt)
@@ -569,10 +589,10 @@ int dt_from_grid(emu_state* st, const double* g, double fixed_dt)
     if (use_flavor && fixed_dt <= 0) {
         if (st->nf == 2)
             hipLaunchKernelGGL(k_dt_reduce<2>, dim3(grid_blocks(st->ncell)), dim3(256), 0, ctx->stream, g, st->aux,
-                               st->ncell, ctx->g.cell_volume, st->m00, st->red);
+                               st->ncell, ctx->g.cell_volume * st->nscale, st->m00 * st->nscale, st->red);
         else
             hipLaunchKernelGGL(k_dt_reduce<3>, dim3(grid_blocks(st->ncell)), dim3(256), 0, ctx->stream, g, st->aux,
-                               st->ncell, ctx->g.cell_volume, st->m00, st->red);
+                               st->ncell, ctx->g.cell_volume * st->nscale, st->m00 * st->nscale, st->red);
     }
     DtParams dp;
     dp.dt_translation = p.cfl_factor > 0.0
@@ -675,6 +695,19 @@ template <typename T> static int load_t(emu_state* st, const PView& P, const int
     int e = scan_counts(st, st->cell_start);
     if (e) return e;
     if (np == 0) return EMU_OK;
+    st->nscale = 1.0;
+    if (st->precision == EMU_FP32_BULK) {
+        using I2 = PIdx<2>;
+        const int iN = I2::N, iNb = st->nf == 2 ? PIdx<2>::Nbar : PIdx<3>::Nbar;
+        hipLaunchKernelGGL(k_fill_u64v, dim3(1), dim3(64), 0, s, st->red + 4, 1, 0ull);
+        hipLaunchKernelGGL(k_absmax2, dim3(grid_blocks(np)), dim3(256), 0, s, P.r[iN], P.r[iNb], np, st->red + 4);
+        unsigned long long h = 0;
+        EMU_HIP(ctx, hipMemcpyAsync(&h, st->red + 4, 8, hipMemcpyDeviceToHost, s));
+        EMU_HIP(ctx, hipStreamSynchronize(s));
+        double nmax;
+        std::memcpy(&nmax, &h, 8);
+        if (nmax > 0.0 && std::isfinite(nmax)) st->nscale = std::ldexp(1.0, -std::ilogb(nmax));
+    }
     // stable radix sort of (cell, input index): sorted slot j <- input order[j]
     int *keys_out = nullptr, *order = nullptr;
     if (hipMalloc(&keys_out, np * sizeof(int)) != hipSuccess || hipMalloc(&order, np * sizeof(int)) != hipSuccess) {
@@ -694,9 +727,11 @@ template <typename T> static int load_t(emu_state* st, const PView& P, const int
     if (!e) {
         Arrays<T> A = arrays_of<T>(st, st->cur);
         if (st->nf == 2)
-            hipLaunchKernelGGL((k_load<2, T>), dim3(grid_blocks(np)), dim3(256), 0, s, P, ids, order, A, np);
+            hipLaunchKernelGGL((k_load<2, T>), dim3(grid_blocks(np)), dim3(256), 0, s, P, ids, order, A, np,
+                               st->nscale);
         else
-            hipLaunchKernelGGL((k_load<3, T>), dim3(grid_blocks(np)), dim3(256), 0, s, P, ids, order, A, np);
+            hipLaunchKernelGGL((k_load<3, T>), dim3(grid_blocks(np)), dim3(256), 0, s, P, ids, order, A, np,
+                               st->nscale);
         if (hipGetLastError() != hipSuccess) e = set_error(ctx, EMU_ERR_HIP, "state_load");
     }
     (void)hipStreamSynchronize(s);
@@ -765,12 +800,12 @@ extern "C" int emu_state_store(emu_state* st, emu_particles* parts, int64_t* ids
     if (st->np > 0) {
         if (st->precision == EMU_FP32_BULK) {
             Arrays<float> A = arrays_of<float>(st, st->cur);
-            if (st->nf == 2) hipLaunchKernelGGL((k_store<2, float>), dim3(grid_blocks(st->np)), dim3(256), 0, ctx->stream, A, P, ids, st->scal, st->np);
-            else hipLaunchKernelGGL((k_store<3, float>), dim3(grid_blocks(st->np)), dim3(256), 0, ctx->stream, A, P, ids, st->scal, st->np);
+            if (st->nf == 2) hipLaunchKernelGGL((k_store<2, float>), dim3(grid_blocks(st->np)), dim3(256), 0, ctx->stream, A, P, ids, st->scal, st->np, st->nscale);
+            else hipLaunchKernelGGL((k_store<3, float>), dim3(grid_blocks(st->np)), dim3(256), 0, ctx->stream, A, P, ids, st->scal, st->np, st->nscale);
         } else {
             Arrays<double> A = arrays_of<double>(st, st->cur);
-            if (st->nf == 2) hipLaunchKernelGGL((k_store<2, double>), dim3(grid_blocks(st->np)), dim3(256), 0, ctx->stream, A, P, ids, st->scal, st->np);
-            else hipLaunchKernelGGL((k_store<3, double>), dim3(grid_blocks(st->np)), dim3(256), 0, ctx->stream, A, P, ids, st->scal, st->np);
+            if (st->nf == 2) hipLaunchKernelGGL((k_store<2, double>), dim3(grid_blocks(st->np)), dim3(256), 0, ctx->stream, A, P, ids, st->scal, st->np, st->nscale);
+            else hipLaunchKernelGGL((k_store<3, double>), dim3(grid_blocks(st->np)), dim3(256), 0, ctx->stream, A, P, ids, st->scal, st->np, st->nscale);
         }
     }
     EMU_LAUNCH_CHECK(ctx, "state_store");
@@ -934,7 +969,7 @@ extern "C" int emu_state_flavor_amplitude(emu_state* st, double* sums)
     double h[2];
     EMU_HIP(ctx, hipMemcpyAsync(h, st->red + 4, sizeof h, hipMemcpyDeviceToHost, s));
     EMU_HIP(ctx, hipStreamSynchronize(s));
-    sums[0] = h[0];
-    sums[1] = h[1];
+    sums[0] = h[0] / st->nscale;
+    sums[1] = h[1] / st->nscale;
     return EMU_OK;
 }
