@@ -130,6 +130,12 @@ __device__ __forceinline__ int renorm_fp64(double* f, double L, double maxError)
     return EMU_OK;
 }
 
+// w[i] for a wave-uniform i in 0..2, without indexing a register array
+__device__ __forceinline__ double sel3(const double w[3], int i)
+{
+    return i == 0 ? w[0] : (i == 1 ? w[1] : w[2]);
+}
+
 // End-of-step position: the four stage saxpys, then the periodic shift.
 __device__ __forceinline__ void end_position(const double pos[3], const double v[3], double dt, const Geom& g,
                                              double npos[3], int& newcell)
@@ -144,8 +150,10 @@ __device__ __forceinline__ void end_position(const double pos[3], const double v
     newcell = (nc[2] * g.n[1] + nc[1]) * g.n[0] + nc[0];
 }
 
-template <int NF, typename T, int WX, int WY, int WZ, int WPB>
-__global__ __launch_bounds__(64 * WPB) void k_stage(StageArgs<T> a)
+// S: 0 = deposit only (the step's result, or the initial state), 1..4 = RK
+// stage; AUX: also deposit the compute_dt auxiliaries (S = 0 or 3).
+template <int NF, typename T, int WX, int WY, int WZ, int WPB, int S, bool AUX>
+__global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(2))) void k_stage(StageArgs<T> a)
 {
     constexpr int nh = NH<NF>;
     constexpr int QG = 4 * nh;
@@ -159,8 +167,8 @@ __global__ __launch_bounds__(64 * WPB) void k_stage(StageArgs<T> a)
     extern __shared__ double lds[];
 
     const Geom& g = a.g;
-    const int s = a.stage;
-    const bool aux = a.want_aux && (s == 3 || s == 0);
+    constexpr int s = S;
+    constexpr bool aux = AUX;
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int lane = threadIdx.x & 63;
     // blocks go round-robin over the 8 XCDs; give each XCD a contiguous run of cells
@@ -209,7 +217,8 @@ __global__ __launch_bounds__(64 * WPB) void k_stage(StageArgs<T> a)
 #pragma unroll
         for (int d = 0; d < 3; d++) pos[d] = a.cur.pos[d][ipl];
         const T pupt = a.cur.pup[3][ipl];
-        const T vhat[3] = {a.cur.pup[0][ipl] / pupt, a.cur.pup[1][ipl] / pupt, a.cur.pup[2][ipl] / pupt};
+        const T pupv[3] = {a.cur.pup[0][ipl], a.cur.pup[1][ipl], a.cur.pup[2][ipl]};
+        const T vhat[3] = {pupv[0] / pupt, pupv[1] / pupt, pupv[2] / pupt};
         const T N = a.cur.Nn[0][ipl], Nb = a.cur.Nn[1][ipl];
         int idx[3];
         double wd[3][3];
@@ -220,30 +229,30 @@ __global__ __launch_bounds__(64 * WPB) void k_stage(StageArgs<T> a)
         for (int c = 0; c < 2 * nh; c++)
             Y[c] = s <= 1 ? a.cur.f[(int64_t)c * a.np + ipl] : a.Y[(int64_t)c * a.np + ipl];
         T Dep[2 * nh]; // state deposited by this launch
-        if (s == 0) {
+        if constexpr (s == 0) {
 #pragma unroll
             for (int c = 0; c < 2 * nh; c++) Dep[c] = Y[c];
         } else {
-            // ---- gather over the stencil: the neighbours are wave-uniform.  The
-            // loops over z and y stay rolled (uniform trip counts) to keep the
-            // scalar loads of one row of neighbours in flight at a time.
+            // ---- gather over the stencil: the neighbours are wave-uniform, so
+            // their fields are loaded to scalar registers and used as FMA
+            // operands.  The loop stays rolled so that one neighbour's fields
+            // (4 NH values) are in scalar registers at a time.
             T inside[nh];
 #pragma unroll
             for (int c = 0; c < nh; c++) inside[c] = T(0);
 #pragma unroll 1
             for (int kk = 0; kk < WZ; kk++) {
-                const double wz_ = WZ == 1 ? wd[2][1] : (kk == 0 ? wd[2][0] : kk == 1 ? wd[2][1] : wd[2][2]);
+                const double wz_ = WZ == 1 ? wd[2][1] : sel3(wd[2], kk);
                 const int ck = WZ == 1 ? cz : pmod(cz + kk - 1, g.n[2]);
 #pragma unroll 1
                 for (int jj = 0; jj < WY; jj++) {
-                    const double wyz =
-                        (WY == 1 ? wd[1][1] : (jj == 0 ? wd[1][0] : jj == 1 ? wd[1][1] : wd[1][2])) * wz_;
+                    const double wyz = (WY == 1 ? wd[1][1] : sel3(wd[1], jj)) * wz_;
                     const int cj = WY == 1 ? cy : pmod(cy + jj - 1, g.n[1]);
-#pragma unroll
+#pragma unroll 1
                     for (int ii = 0; ii < WX; ii++) {
                         const int ci = WX == 1 ? cx : pmod(cx + ii - 1, g.n[0]);
                         const int64_t nb = ((int64_t)ck * g.n[1] + cj) * g.n[0] + ci;
-                        const T w = T(wd[0][WX == 1 ? 1 : ii] * wyz);
+                        const T w = T((WX == 1 ? wd[0][1] : sel3(wd[0], ii)) * wyz);
                         const T wx = w * vhat[0], wy = w * vhat[1], wz = w * vhat[2];
                         if constexpr (sizeof(T) == 8) {
                             // constant address space: the grid is read-only in this launch and
@@ -264,6 +273,18 @@ __global__ __launch_bounds__(64 * WPB) void k_stage(StageArgs<T> a)
                     }
                 }
             }
+            // ---- the RK registers this stage reads, as one batch of loads whose
+            // latency the Hamiltonian and commutator arithmetic covers
+            T yn[2 * nh], ao[2 * nh];
+            if constexpr (s == 1) {
+#pragma unroll
+                for (int c = 0; c < 2 * nh; c++) yn[c] = ao[c] = Y[c];
+            } else {
+#pragma unroll
+                for (int c = 0; c < 2 * nh; c++) yn[c] = s < 4 ? a.cur.f[(int64_t)c * a.np + ipl] : T(0);
+#pragma unroll
+                for (int c = 0; c < 2 * nh; c++) ao[c] = a.acc[(int64_t)c * a.np + ipl];
+            }
             // ---- Hamiltonians / hbar: H = H_vac + coef inside, Hbar = conj(H_vac - coef inside)
             T H[nh], Hb[nh];
             const T inv_pupt = T(1) / pupt;
@@ -278,22 +299,24 @@ __global__ __launch_bounds__(64 * WPB) void k_stage(StageArgs<T> a)
             commutator_rhs<NF, T>(H, Y, k, T(1));
             commutator_rhs<NF, T>(Hb, Y + nh, k + nh, T(1));
             const T a_acc = T(dt * bw[s - 1]);
-            if (s < 4) {
+            if constexpr (s < 4) {
                 const T a_next = T(dt * cnext[s - 1]);
 #pragma unroll
                 for (int c = 0; c < 2 * nh; c++) {
-                    const T yn = s == 1 ? Y[c] : a.cur.f[(int64_t)c * a.np + ipl];
-                    const T acc_old = s == 1 ? yn : a.acc[(int64_t)c * a.np + ipl];
-                    Dep[c] = a_next * k[c] + yn;
-                    if (act) {
-                        a.acc[(int64_t)c * a.np + ip] = a_acc * k[c] + acc_old;
+                    Dep[c] = a_next * k[c] + yn[c];
+                    ao[c] = a_acc * k[c] + ao[c];
+                }
+                if (act) {
+#pragma unroll
+                    for (int c = 0; c < 2 * nh; c++) {
+                        a.acc[(int64_t)c * a.np + ip] = ao[c];
                         a.Y[(int64_t)c * a.np + ip] = Dep[c];
                     }
                 }
                 if (s == 1 && act) { // the end-of-step cell, for the sort before stage 4
                     double v[3], npos[3];
 #pragma unroll
-                    for (int d = 0; d < 3; d++) v[d] = double(a.cur.pup[d][ip]) / double(pupt) * phys::c;
+                    for (int d = 0; d < 3; d++) v[d] = double(pupv[d]) / double(pupt) * phys::c;
                     int nc;
                     end_position(pos, v, dt, g, npos, nc);
                     a.newcell[ip] = nc;
@@ -303,7 +326,7 @@ __global__ __launch_bounds__(64 * WPB) void k_stage(StageArgs<T> a)
                 // periodic shift, Renormalize in FP64, and the move to the sorted slot
                 double y[2 * nh];
 #pragma unroll
-                for (int c = 0; c < 2 * nh; c++) y[c] = double(a_acc * k[c] + a.acc[(int64_t)c * a.np + ip]);
+                for (int c = 0; c < 2 * nh; c++) y[c] = double(a_acc * k[c] + ao[c]);
                 const T L = a.cur.Ll[0][ip], Lb = a.cur.Ll[1][ip];
                 const int64_t id = a.cur.id[ip];
                 int err = renorm_fp64<NF>(y, double(L), a.maxError);
@@ -311,7 +334,7 @@ __global__ __launch_bounds__(64 * WPB) void k_stage(StageArgs<T> a)
                 if (err != EMU_OK) atomicMin(a.err, ((unsigned long long)id << 3) | (unsigned long long)err);
                 double v[3], npos[3];
 #pragma unroll
-                for (int d = 0; d < 3; d++) v[d] = double(a.cur.pup[d][ip]) / double(pupt) * phys::c;
+                for (int d = 0; d < 3; d++) v[d] = double(pupv[d]) / double(pupt) * phys::c;
                 int nc;
                 end_position(pos, v, dt, g, npos, nc);
                 const int64_t j = (int64_t)a.mbase[(int64_t)cell * 27 + a.mv[ip]] + a.rank[ip];
@@ -319,7 +342,8 @@ __global__ __launch_bounds__(64 * WPB) void k_stage(StageArgs<T> a)
 #pragma unroll
                     for (int d = 0; d < 3; d++) a.dst.pos[d][j] = npos[d];
 #pragma unroll
-                    for (int d = 0; d < 4; d++) a.dst.pup[d][j] = a.cur.pup[d][ip];
+                    for (int d = 0; d < 3; d++) a.dst.pup[d][j] = pupv[d];
+                    a.dst.pup[3][j] = pupt;
                     a.dst.Nn[0][j] = N;
                     a.dst.Nn[1][j] = Nb;
                     a.dst.Ll[0][j] = L;

@@ -509,8 +509,21 @@ void launch_stage_t(emu_state* st, StageArgs<T>& a)
     const size_t lds = (size_t)kWPB * 64 * (9 + (nh + 3) + (aux ? 2 * NF + NF * (NF - 1) / 2 : 0)) * sizeof(double);
     const int64_t nblk = (st->ncell + kWPB - 1) / kWPB;
     a.blocks_per_xcd = (int)((nblk + 7) / 8);
-    hipLaunchKernelGGL((k_stage<NF, T, WX, WY, WZ, kWPB>), dim3((unsigned)(8 * a.blocks_per_xcd)), dim3(64 * kWPB),
-                       lds, st->ctx->stream, a);
+    const dim3 grid((unsigned)(8 * a.blocks_per_xcd)), block(64 * kWPB);
+    hipStream_t s = st->ctx->stream;
+    switch (a.stage) {
+    case 0:
+        if (aux) hipLaunchKernelGGL((k_stage<NF, T, WX, WY, WZ, kWPB, 0, true>), grid, block, lds, s, a);
+        else hipLaunchKernelGGL((k_stage<NF, T, WX, WY, WZ, kWPB, 0, false>), grid, block, lds, s, a);
+        break;
+    case 1: hipLaunchKernelGGL((k_stage<NF, T, WX, WY, WZ, kWPB, 1, false>), grid, block, lds, s, a); break;
+    case 2: hipLaunchKernelGGL((k_stage<NF, T, WX, WY, WZ, kWPB, 2, false>), grid, block, lds, s, a); break;
+    case 3:
+        if (aux) hipLaunchKernelGGL((k_stage<NF, T, WX, WY, WZ, kWPB, 3, true>), grid, block, lds, s, a);
+        else hipLaunchKernelGGL((k_stage<NF, T, WX, WY, WZ, kWPB, 3, false>), grid, block, lds, s, a);
+        break;
+    default: hipLaunchKernelGGL((k_stage<NF, T, WX, WY, WZ, kWPB, 4, false>), grid, block, lds, s, a); break;
+    }
 }
 
 // Stencil shapes: 3 cells per dimension with shape order 1 or 2, 1 cell with
