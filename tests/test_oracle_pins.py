@@ -12,6 +12,7 @@ import os
 import numpy as np
 import pytest
 
+import cases
 import kat
 
 NB = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "notebook_commutator_nf3.json")
@@ -488,3 +489,76 @@ def test_kat_msw(oracle):
 
 def test_kat_bipolar_runs(oracle):
     _run_kat(oracle, "inputs_bipolar_test", 40, every=lambda p, K: None)
+
+
+# ---------------------------------------------------------------------------
+# initial-condition helpers
+# ---------------------------------------------------------------------------
+def test_philox_known_answers(oracle):
+    """Philox4x32-10 against the known-answer vectors published with Random123
+    (Salmon et al., SC'11; kat_vectors: counter, key -> output)."""
+    kats = [((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+            ((0xFFFFFFFF,) * 4, (0xFFFFFFFF, 0xFFFFFFFF), (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+            ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+             (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1))]
+    for ctr, key, want in kats:
+        got = oracle.philox4x32(*ctr, *key)
+        assert [int(x) for x in got] == list(want)
+
+
+def test_uniform01_range_and_moments(oracle):
+    u = oracle.uniform01(99, np.arange(200000, dtype=np.uint64), 3)
+    assert u.min() >= 0.0 and u.max() < 1.0
+    assert abs(u.mean() - 0.5) < 3e-3 and abs(u.var() - 1.0 / 12.0) < 2e-3
+    # the two halves of one Philox block are different draws
+    v = oracle.uniform01(99, np.arange(1000, dtype=np.uint64), 2)
+    w = oracle.uniform01(99, np.arange(1000, dtype=np.uint64), 3)
+    assert not np.any(v == w)
+
+
+def test_minerbo_closure_solves_its_equation(oracle):
+    """Minerbo: the flux factor of the closure is coth(Z) - 1/Z
+    (FlavoredNeutrinoContainerInit.cpp:46-80); below 1e-3 the reference uses
+    the small-Z limit Z = 3 ff."""
+    for ff in (0.01, 0.1, 0.33, 0.6, 0.9):
+        Z = oracle.minerbo_Z(ff)
+        assert abs(1.0 / math.tanh(Z) - 1.0 / Z - ff) < 1e-6
+    assert oracle.minerbo_Z(5e-4) == 3 * 5e-4
+    assert oracle.minerbo_Z(0.0) == 0.0
+
+
+def test_uniform_sphere_counts_and_antipodes(oracle):
+    """uniform_sphere_xyz(16) is the 92-direction set of inputs_1d_fiducial;
+    every direction off the equator comes with its antipode and all are unit
+    vectors."""
+    for nphi, n in ((16, 92), (32, 378)):
+        xyz = oracle.uniform_sphere_xyz(nphi)
+        assert len(xyz) == n
+        np.testing.assert_allclose(np.linalg.norm(xyz, axis=1), 1.0, atol=1e-15)
+        off = xyz[np.abs(xyz[:, 2]) > 1e-12]
+        for v in off[:20]:
+            assert np.min(np.linalg.norm(off + v, axis=1)) < 1e-15
+
+
+def test_sync_location_and_apply_rhs(oracle):
+    """SyncLocation copies between pos and (x,y,z) (FlavoredNeutrinoContainer.cpp:8-41);
+    ApplyFlavoredNeutrinoRHS is p = a dF + p on every attribute (.H:33-47)."""
+    d = cases.oracle_deck(oracle, cases.fast_flavor_dict(nf=2, ncell=(2, 2, 2), nphi=4))
+    parts = oracle.init_particles(d)
+    K = {n: i for i, n in enumerate(oracle.attribute_names(2))}
+    # type 0 = CoordinateToPosition: pos <- (x, y, z)
+    p = parts.copy()
+    p["rdata"][:, K["x"]] = -7.0
+    oracle.sync_location(2, p, 0)
+    np.testing.assert_array_equal(p["pos"][:, 0], -7.0)
+    np.testing.assert_array_equal(p["pos"][:, 1:], parts["pos"][:, 1:])
+    # type 1 = PositionToCoordinate: (x, y, z) <- pos
+    q = parts.copy()
+    q["pos"] += 0.125
+    oracle.sync_location(2, q, 1)
+    np.testing.assert_array_equal(q["rdata"][:, K["x"]:K["z"] + 1], parts["pos"] + 0.125)
+    r = parts.copy()
+    dF = parts.copy()
+    dF["rdata"] = np.random.default_rng(1).standard_normal(dF["rdata"].shape)
+    oracle.apply_rhs(2, r, 0.25, dF)
+    np.testing.assert_array_equal(r["rdata"], 0.25 * dF["rdata"] + parts["rdata"])
