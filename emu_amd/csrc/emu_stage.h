@@ -5,10 +5,14 @@
 // One wave owns one cell at a time and takes the cell's particles 64 at a
 // time (lane = particle).
 //
-// Gather (VALU): the 27 neighbour cells' fields are the same for every lane of
-// the wave, so their addresses are wave-uniform and the loads go to scalar
-// registers; each particle then needs 27 x (4 + 4 NH) FMAs:
-//   inside_c = sum_t w_t (M_c - vx Phix_c - vy Phiy_c - vz Phiz_c).
+// Gather (MFMA): with s(p) = (1, -vx, -vy, -vz) and G_g(t, c) the four field
+// groups (M, Phix, Phiy, Phiz) of neighbour t,
+//   inside_c(p) = sum_t sum_g [w_t(p) s_g(p)] G_g(t, c),
+// a (16 particles x 4*27) x (4*27 x NH) product per tile of 16 particles: one
+// 16x16x4 step per neighbour t, its K = 4 slice being the four groups.  The
+// cell's 27 x 4 NH neighbour fields are staged in LDS once per cell with
+// coalesced loads and feed the B operand; A is the particle's weight times
+// s_g, read from the shape factors staged in LDS.
 // Deposit (MFMA): the contribution of the cell's particles p to target cell t
 // and field column q is D[t][q] = sum_p W[p][t] R[p][q], with W the shape
 // weights and R = (1, vx, vy, vz) x q_c(p), q = N f - conj(Nbar fbar).  Over a
@@ -24,6 +28,27 @@
 namespace emu {
 
 typedef double mf_d4 __attribute__((ext_vector_type(4)));
+typedef float mf_f4 __attribute__((ext_vector_type(4)));
+
+// One 16x16x4 matrix-core step in the precision of T.  Element r of lane l of
+// the result holds row 4r + l/16 (f64) or 4(l/16) + r (f32), column l%16.
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+    typedef mf_d4 acc_t;
+    static __device__ __forceinline__ acc_t step(double a, double b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int r, int lane) { return 4 * r + (lane >> 4); }
+};
+template <> struct Mfma<float> {
+    typedef mf_f4 acc_t;
+    static __device__ __forceinline__ acc_t step(float a, float b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int r, int lane) { return 4 * (lane >> 4) + r; }
+};
 
 template <typename T> struct PArrays {
     double* pos[3];
@@ -179,10 +204,36 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(2))) v
     const int beg = a.cell_start[cell], end = a.cell_start[cell + 1];
     if (beg >= end) return;
     const int cx = cell % g.n[0], cy = (cell / g.n[0]) % g.n[1], cz = cell / (g.n[0] * g.n[1]);
-    const int per_wave = 64 * (9 + NSRC + (aux ? NSAUX : 0));
-    double* Wf = lds + wave * per_wave;     // [64][9]   shape factors by dim
-    double* Src = Wf + 64 * 9;              // [64][NSRC]
+    constexpr int GS = s == 0 ? 0 : (NT * QG * (int)sizeof(T) + 7) / 8; // doubles of the staged neighbourhood
+    constexpr int per_wave = GS + 64 * (9 + NSRC + (aux ? NSAUX : 0));
+    T* Gs = (T*)(lds + wave * per_wave);    // [NT][QG] neighbour fields (gather stages)
+    double* Wf = lds + wave * per_wave + GS; // [64][9]   shape factors by dim
+    double* Src = Wf + 64 * 9;              // [64][NSRC] q_c, vhat (inside_c during the gather)
     double* Saux = Src + 64 * NSRC;         // [64][NSAUX]
+    // neighbour t of the cell (offsets -1, 0, +1 per dimension; a 3-wide
+    // stencil implies at least 2 cells in that dimension, so one wrap suffices)
+    auto wrap1i = [](int x, int n) { return x < 0 ? x + n : (x >= n ? x - n : x); };
+    auto neighbour = [&](int t) -> int64_t {
+        const int ii = t % WX, jj = (t / WX) % WY, kk = t / (WX * WY);
+        const int ci = WX == 1 ? cx : wrap1i(cx + ii - 1, g.n[0]);
+        const int cj = WY == 1 ? cy : wrap1i(cy + jj - 1, g.n[1]);
+        const int ck = WZ == 1 ? cz : wrap1i(cz + kk - 1, g.n[2]);
+        return ((int64_t)ck * g.n[1] + cj) * g.n[0] + ci;
+    };
+    if constexpr (s != 0) {
+        // stage the neighbourhood's fields: NT rows of QG values, one row per neighbour
+        const T* gsrc;
+        if constexpr (sizeof(T) == 8) gsrc = (const T*)a.gin;
+        else gsrc = (const T*)a.gin32;
+#pragma unroll
+        for (int e0 = 0; e0 < NT * QG; e0 += 64) {
+            const int e = e0 + lane;
+            if (e < NT * QG) {
+                const int t = e / QG, q = e - t * QG;
+                Gs[e] = gsrc[neighbour(t) * QG + q];
+            }
+        }
+    }
 
     const double dt = *a.dt;
     const double bw[4] = {1. / 6., 1. / 3., 1. / 3., 1. / 6.};
@@ -224,6 +275,17 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(2))) v
         double wd[3][3];
 #pragma unroll
         for (int d = 0; d < 3; d++) shape3((pos[d] - g.plo[d]) * g.dxi[d], g.order[d], idx[d], wd[d]);
+        // shape factors and directions of the chunk, for the matrix-core operands
+        // of the gather and the deposit (positions are frozen within a step)
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+#pragma unroll
+            for (int e = 0; e < 3; e++) Wf[lane * 9 + 3 * d + e] = act ? wd[d][e] : 0.0;
+#pragma unroll
+        for (int d = 0; d < 3; d++) Src[lane * NSRC + nh + d] = act ? double(vhat[d]) : 0.0;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         T Y[2 * nh];
 #pragma unroll
         for (int c = 0; c < 2 * nh; c++)
@@ -233,45 +295,40 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(2))) v
 #pragma unroll
             for (int c = 0; c < 2 * nh; c++) Dep[c] = Y[c];
         } else {
-            // ---- gather over the stencil: the neighbours are wave-uniform, so
-            // their fields are loaded to scalar registers and used as FMA
-            // operands.  The loop stays rolled so that one neighbour's fields
-            // (4 NH values) are in scalar registers at a time.
+            // ---- gather on the matrix cores, 16 particles per tile
             T inside[nh];
-#pragma unroll
-            for (int c = 0; c < nh; c++) inside[c] = T(0);
+            {
+                using MF = Mfma<T>;
+                const int g4 = lane >> 4, c16 = lane & 15;
+                const int ntile = (min(64, end - base) + 15) >> 4;
 #pragma unroll 1
-            for (int kk = 0; kk < WZ; kk++) {
-                const double wz_ = WZ == 1 ? wd[2][1] : sel3(wd[2], kk);
-                const int ck = WZ == 1 ? cz : pmod(cz + kk - 1, g.n[2]);
-#pragma unroll 1
-                for (int jj = 0; jj < WY; jj++) {
-                    const double wyz = (WY == 1 ? wd[1][1] : sel3(wd[1], jj)) * wz_;
-                    const int cj = WY == 1 ? cy : pmod(cy + jj - 1, g.n[1]);
-#pragma unroll 1
-                    for (int ii = 0; ii < WX; ii++) {
-                        const int ci = WX == 1 ? cx : pmod(cx + ii - 1, g.n[0]);
-                        const int64_t nb = ((int64_t)ck * g.n[1] + cj) * g.n[0] + ci;
-                        const T w = T((WX == 1 ? wd[0][1] : sel3(wd[0], ii)) * wyz);
-                        const T wx = w * vhat[0], wy = w * vhat[1], wz = w * vhat[2];
-                        if constexpr (sizeof(T) == 8) {
-                            // constant address space: the grid is read-only in this launch and
-                            // the address is wave-uniform, so the loads go to scalar registers
-                            const __attribute__((address_space(4))) double* G =
-                                (const __attribute__((address_space(4))) double*)(a.gin + nb * QG);
+                for (int m = 0; m < ntile; m++) {
+                    const int p = 16 * m + c16; // the particle of this lane's A rows
+                    double w3[9];
 #pragma unroll
-                            for (int c = 0; c < nh; c++)
-                                inside[c] += w * T(G[c]) - wx * T(G[nh + c]) - wy * T(G[2 * nh + c]) -
-                                             wz * T(G[3 * nh + c]);
-                        } else {
-                            const __attribute__((address_space(4))) float* G =
-                                (const __attribute__((address_space(4))) float*)(a.gin32 + nb * QG);
+                    for (int e = 0; e < 9; e++) w3[e] = Wf[p * 9 + e];
+                    const double* vp = Src + p * NSRC + nh;
+                    const double v0 = vp[0], v1 = vp[1], v2 = vp[2];
+                    const T sg = T(g4 == 0 ? 1.0 : (g4 == 1 ? -v0 : (g4 == 2 ? -v1 : -v2)));
+                    typename MF::acc_t accg = {T(0), T(0), T(0), T(0)};
 #pragma unroll
-                            for (int c = 0; c < nh; c++)
-                                inside[c] += w * G[c] - wx * G[nh + c] - wy * G[2 * nh + c] - wz * G[3 * nh + c];
-                        }
+                    for (int t = 0; t < NT; t++) {
+                        const int ii = t % WX, jj = (t / WX) % WY, kk = t / (WX * WY);
+                        const double w = w3[WX == 1 ? 1 : ii] * (w3[3 + (WY == 1 ? 1 : jj)] * w3[6 + (WZ == 1 ? 1 : kk)]);
+                        const T A = T(w) * sg;
+                        const T B = c16 < nh ? Gs[t * QG + g4 * nh + c16] : T(0);
+                        accg = MF::step(A, B, accg);
+                    }
+                    if (c16 < nh) {
+#pragma unroll
+                        for (int r = 0; r < 4; r++) Src[(16 * m + MF::row(r, lane)) * NSRC + c16] = double(accg[r]);
                     }
                 }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+                for (int c = 0; c < nh; c++) inside[c] = T(Src[lane * NSRC + c]);
             }
             // ---- the RK registers this stage reads, as one batch of loads whose
             // latency the Hamiltonian and commutator arithmetic covers
@@ -357,18 +414,12 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(2))) v
             }
         }
         if (!deposits) continue;
-        // ---- stage the chunk's MFMA operands in LDS (zeros for inactive lanes)
-#pragma unroll
-        for (int d = 0; d < 3; d++)
-#pragma unroll
-            for (int e = 0; e < 3; e++) Wf[lane * 9 + 3 * d + e] = act ? wd[d][e] : 0.0;
+        // ---- stage the chunk's deposit sources in LDS (zeros for inactive lanes)
 #pragma unroll
         for (int c = 0; c < nh; c++) {
             const T qc = is_im_comp<NF>(c) ? N * Dep[c] + Nb * Dep[nh + c] : N * Dep[c] - Nb * Dep[nh + c];
             Src[lane * NSRC + c] = act ? double(qc) : 0.0;
         }
-#pragma unroll
-        for (int d = 0; d < 3; d++) Src[lane * NSRC + nh + d] = act ? double(vhat[d]) : 0.0;
         if (aux) {
 #pragma unroll
             for (int x = 0; x < NF; x++) {
@@ -431,11 +482,7 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(2))) v
         for (int r = 0; r < 4; r++) {
             const int t = 16 * m + 4 * r + (lane >> 4);
             if (t >= NT) continue;
-            const int ii = t % WX, jj = (t / WX) % WY, kk = t / (WX * WY);
-            const int ci = WX == 1 ? cx : pmod(cx + ii - 1, g.n[0]);
-            const int cj = WY == 1 ? cy : pmod(cy + jj - 1, g.n[1]);
-            const int ck = WZ == 1 ? cz : pmod(cz + kk - 1, g.n[2]);
-            const int64_t tc = ((int64_t)ck * g.n[1] + cj) * g.n[0] + ci;
+            const int64_t tc = neighbour(t);
 #pragma unroll
             for (int n = 0; n < NTL; n++) {
                 const int q = 16 * n + (lane & 15);

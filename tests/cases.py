@@ -50,3 +50,24 @@ def max_rel_err(a, b):
     scale = np.abs(b).max(axis=0)
     scale = np.where(scale > 0, scale, 1.0)
     return float((np.abs(a - b) / scale).max())
+
+
+# keys Parameters.H:53-130 requires, with the values a deck dict may leave out
+_DECK_DEFAULTS = dict(nppc=(1, 1, 1), max_grid_size=64, nsteps=1, end_time=1.0, T_MeV=10.0, max_adaptive_speedup=0.0,
+                      write_plot_every=0, write_plot_particles_every=0, do_restart=0, restart_dir="",
+                      alpha1_degrees=0.0, alpha2_degrees=0.0, st4_phi=0.0, st4_phibar=0.0)
+
+
+def deck_text(d: dict) -> str:
+    """A ParmParse inputs file (key = value) for a deck dict."""
+    full = {**_DECK_DEFAULTS, **d}
+    lines = []
+    for k, v in full.items():
+        if isinstance(v, (tuple, list)):
+            v = "(" + ",".join(str(int(x)) for x in v) + ")"
+        elif isinstance(v, str):
+            v = f'"{v}"'
+        elif isinstance(v, float):
+            v = repr(v)
+        lines.append(f"{k} = {v}")
+    return "\n".join(lines) + "\n"
