@@ -47,6 +47,20 @@ def algorithmic_bytes(nf: int, ppc: float, precision: str) -> float:
     return part + 12 * G * 8 / ppc
 
 
+def counter_traffic(workload: str, precision: str):
+    """HBM bytes per k_stage launch measured with rocprofv3 --pmc (FETCH_SIZE
+    and WRITE_SIZE passes, scaled by a 1 GiB copy) and committed as a text
+    summary under profiles/; None if there is none for this workload."""
+    path = os.path.join(ROOT, "profiles", f"k_stage_{workload}_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    key = f"{precision} traffic_bytes_per_launch ="
+    for line in open(path):
+        if line.startswith(key):
+            return float(line.split("=", 1)[1].split()[0])
+    return None
+
+
 def run_precision(core, parms, parts, precision, steps, warmup):
     import torch
     emu = parts.emu
@@ -194,7 +208,8 @@ def main():
                                f"periodic cells, {round(ppc)} directions/cell, {parts.np} particles, fast-flavor",
                    "particles": parts.np, "precision": head},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+                     "traffic": counter_traffic(args.workload, head),
                      "kernel": "k_stage (gather + RHS + RK update + deposit; four RK stages and the step-end deposit)",
                      "algorithmic_bytes_per_update": bpu, "avg_launch_ms": r["stage_ms"],
                      "launches_per_step": r["stage_launches_per_step"]},
