@@ -221,17 +221,18 @@ def compute_dt(state: MultiFab, Vvac_max: float) -> float:
 
 
 class Integrator:
-    """The AMReX TimeIntegrator with classical RK4 plus the post-step hook of
-    main.cpp:155-199, on per-function calls (the plain drop-in path)."""
-
-    A = ((), (0.5,), (0.0, 0.5), (0.0, 0.0, 1.0))
-    B = (1. / 6., 1. / 3., 1. / 3., 1. / 6.)
+    """The AMReX TimeIntegrator plus the post-step hook of main.cpp:155-199,
+    on per-function calls (the plain drop-in path).  The tableau is the one
+    the deck's integration.type / integration.rk.type select (classical RK4
+    in every reference deck; forward Euler, trapezoid and SSPRK3 for the
+    convergence study, Scripts/tests/convergence.sh)."""
 
     def __init__(self, emu: Emu, old: FlavoredNeutrinoContainer):
         self.emu = emu
+        self.A, self.B = emu.parms.tableau()
         self.old = old
         self.new = FlavoredNeutrinoContainer(emu, old.np)
-        self.F = [FlavoredNeutrinoContainer(emu, old.np) for _ in range(4)]
+        self.F = [FlavoredNeutrinoContainer(emu, old.np) for _ in range(len(self.B))]
         self.state = MultiFab(emu)
         self.state.FillBoundary()
         self.vvac = old.Vvac_max()
@@ -248,13 +249,13 @@ class Integrator:
 
     def advance(self):
         dt = self.dt
-        for i in range(4):
+        for i in range(len(self.B)):
             self.new.copyParticles(self.old)
             for j, a in enumerate(self.A[i]):
                 self.new.apply_rhs(dt * a, self.F[j])
             self.rhs(self.F[i], self.new)
         self.new.copyParticles(self.old)
-        for i in range(4):
+        for i in range(len(self.B)):
             self.new.apply_rhs(dt * self.B[i], self.F[i])
         self.time += dt
         self.step_number += 1
@@ -279,6 +280,10 @@ class FusedState:
     """emu_state: the fused, cell-sorted RK4 step (plain FP64 or FP32-bulk)."""
 
     def __init__(self, emu: Emu, np_: int, precision: str = "fp64"):
+        from .params import TABLEAUS
+        if emu.parms.tableau() != TABLEAUS[4]:
+            raise ValueError("the fused step is classical RK4 (integration.type = 1, integration.rk.type = 4); "
+                             "use Integrator for the other tableaus")
         self.emu = emu
         self.np = int(np_)
         self.precision = {"fp64": capi.PLAIN_FP64, "fp32": capi.FP32_BULK}[precision]

@@ -147,6 +147,9 @@ void TestParams::Initialize(const std::map<std::string, std::string>& table)
     if (t.has("directions")) directions = t.str("directions");
     if (t.has("ndirections")) ndirections = t.integer("ndirections");
     if (t.has("seed")) seed = (uint64_t)std::strtoull(t.raw("seed").c_str(), nullptr, 10);
+    if (t.has("integration.type")) integration_type = t.integer("integration.type");
+    if (t.has("integration.rk.type")) integration_rk_type = t.integer("integration.rk.type");
+    (void)Tableau::preset(integration_type, integration_rk_type); // refuses unknown types
 
     simulation_type = t.integer("simulation_type");
     t.ivect("ncell", ncell);
@@ -481,11 +484,63 @@ double compute_dt(const Geometry& geom, double cfl_factor, const MultiFab& state
 // ---------------------------------------------------------------------------
 // TimeIntegrator, RK4
 // ---------------------------------------------------------------------------
-TimeIntegrator::TimeIntegrator(FlavoredNeutrinoContainer& old_data, FlavoredNeutrinoContainer& new_data, double time,
-                               int step)
-    : old_(&old_data), new_(&new_data), time_(time), step_(step)
+Tableau Tableau::preset(int integration_type, int rk_type)
 {
-    for (int i = 0; i < 4; i++) F_.push_back(new FlavoredNeutrinoContainer(old_data.geom()));
+    Tableau t{};
+    if (integration_type == 0) rk_type = 1; // the ForwardEuler integrator
+    else if (integration_type != 1) Abort("integration.type must be 0 or 1", EMU_ERR_UNSUPPORTED);
+    switch (rk_type) {
+    case 1: // forward Euler
+        t.nstages = 1;
+        t.b[0] = 1.0;
+        break;
+    case 2: // trapezoid
+        t.nstages = 2;
+        t.A[1][0] = 1.0;
+        t.b[0] = 0.5;
+        t.b[1] = 0.5;
+        t.c[1] = 1.0;
+        break;
+    case 3: // SSPRK3
+        t.nstages = 3;
+        t.A[1][0] = 1.0;
+        t.A[2][0] = 0.25;
+        t.A[2][1] = 0.25;
+        t.b[0] = 1. / 6.;
+        t.b[1] = 1. / 6.;
+        t.b[2] = 2. / 3.;
+        t.c[1] = 1.0;
+        t.c[2] = 0.5;
+        break;
+    case 4: // classical RK4
+        t.nstages = 4;
+        t.A[1][0] = 0.5;
+        t.A[2][1] = 0.5;
+        t.A[3][2] = 1.0;
+        t.b[0] = 1. / 6.;
+        t.b[1] = 1. / 3.;
+        t.b[2] = 1. / 3.;
+        t.b[3] = 1. / 6.;
+        t.c[1] = 0.5;
+        t.c[2] = 0.5;
+        t.c[3] = 1.0;
+        break;
+    default: Abort("integration.rk.type must be 1, 2, 3 or 4", EMU_ERR_UNSUPPORTED);
+    }
+    return t;
+}
+
+bool Tableau::is_rk4() const
+{
+    const Tableau r = preset(1, 4);
+    return std::memcmp(this, &r, sizeof r) == 0;
+}
+
+TimeIntegrator::TimeIntegrator(FlavoredNeutrinoContainer& old_data, FlavoredNeutrinoContainer& new_data, double time,
+                               int step, const Tableau& tableau)
+    : tab_(tableau), old_(&old_data), new_(&new_data), time_(time), step_(step)
+{
+    for (int i = 0; i < tab_.nstages; i++) F_.push_back(new FlavoredNeutrinoContainer(old_data.geom()));
 }
 
 TimeIntegrator::~TimeIntegrator()
@@ -495,16 +550,13 @@ TimeIntegrator::~TimeIntegrator()
 
 void TimeIntegrator::advance(double dt)
 {
-    static const double A[4][3] = {{0, 0, 0}, {0.5, 0, 0}, {0.0, 0.5, 0}, {0.0, 0.0, 1.0}};
-    static const double B[4] = {1. / 6., 1. / 3., 1. / 3., 1. / 6.};
-    static const double Cn[4] = {0.0, 0.5, 0.5, 1.0};
-    for (int i = 0; i < 4; i++) {
+    for (int i = 0; i < tab_.nstages; i++) {
         new_->copyParticles(*old_, true);
-        for (int j = 0; j < i; j++) new_->apply_rhs(dt * A[i][j], *F_[j]);
-        rhs_(*F_[i], *new_, time_ + Cn[i] * dt);
+        for (int j = 0; j < i; j++) new_->apply_rhs(dt * tab_.A[i][j], *F_[j]);
+        rhs_(*F_[i], *new_, time_ + tab_.c[i] * dt);
     }
     new_->copyParticles(*old_, true);
-    for (int i = 0; i < 4; i++) new_->apply_rhs(dt * B[i], *F_[i]);
+    for (int i = 0; i < tab_.nstages; i++) new_->apply_rhs(dt * tab_.b[i], *F_[i]);
     time_ += dt;
     step_ += 1;
 }
@@ -575,7 +627,8 @@ EvolveResult evolve_flavor(const TestParams* parms, const EvolveOptions& opt)
     double run_fom = 0.0;
 
     if (opt.path == "plain") {
-        TimeIntegrator integrator(neutrinos_old, neutrinos_new, 0.0, 0);
+        TimeIntegrator integrator(neutrinos_old, neutrinos_new, 0.0, 0,
+                                  Tableau::preset(parms->integration_type, parms->integration_rk_type));
         auto source_fun = [&](FlavoredNeutrinoContainer& neutrinos_rhs, const FlavoredNeutrinoContainer& neutrinos,
                               double /*time*/) {
             deposit_to_mesh(neutrinos, state, geom);
@@ -614,6 +667,9 @@ EvolveResult evolve_flavor(const TestParams* parms, const EvolveOptions& opt)
         res.time = integrator.get_time();
         if (!opt.dump_path.empty()) write_dump(opt.dump_path, integrator.get_old_data(), res.time);
     } else if (opt.path == "fused-fp64" || opt.path == "fused-fp32") {
+        if (!Tableau::preset(parms->integration_type, parms->integration_rk_type).is_rk4())
+            Abort("the fused step is classical RK4 (integration.type = 1, integration.rk.type = 4)",
+                  EMU_ERR_UNSUPPORTED);
         const int precision = opt.path == "fused-fp64" ? EMU_PLAIN_FP64 : EMU_FP32_BULK;
         emu_state* st = nullptr;
         emu_check(emu_state_create(geom.ctx(), res.np, precision, &st), "emu_state_create", geom.ctx());

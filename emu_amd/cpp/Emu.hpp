@@ -87,6 +87,11 @@ struct TestParams {
     std::string directions = "sphere";
     int ndirections = 0;
     uint64_t seed = 1234;
+    // the AMReX TimeIntegrator's ParmParse keys: integration.type (0 forward
+    // Euler, 1 Runge-Kutta) and integration.rk.type (1 Euler, 2 trapezoid,
+    // 3 SSPRK3, 4 RK4)
+    int integration_type = 1;
+    int integration_rk_type = 4;
 
     void Initialize(const std::string& inputs_file, const std::vector<std::string>& overrides = {});
     void Initialize(const std::map<std::string, std::string>& table);
@@ -192,15 +197,26 @@ double compute_dt(const Geometry& geom, double cfl_factor, const MultiFab& state
                   const FlavoredNeutrinoContainer& neutrinos, double flavor_cfl_factor, double max_adaptive_speedup);
 
 // ---------------------------------------------------------------------------
-// The AMReX TimeIntegrator with the classical RK4 tableau (integration.type =
-// 1, integration.rk.type = 4 in every deck): stage i copies the old data,
-// adds dt a_ij F_j, evaluates F_i; the new data is old + dt sum b_i F_i; then
-// the post-timestep hook runs and old/new swap roles.
+// The AMReX TimeIntegrator with an explicit preset tableau (classical RK4,
+// integration.type = 1 / integration.rk.type = 4, in every deck): stage i
+// copies the old data, adds dt a_ij F_j for j < i, evaluates F_i; the new data
+// is old + dt sum b_i F_i; then the post-timestep hook runs and old/new swap
+// roles.  integration.type = 0 (ForwardEuler) is the one-stage tableau.
 // ---------------------------------------------------------------------------
+struct Tableau {
+    int nstages;
+    double A[4][4];
+    double b[4];
+    double c[4];
+    static Tableau preset(int integration_type, int rk_type);
+    bool is_rk4() const;
+};
+
 class TimeIntegrator {
 public:
     using Rhs = std::function<void(FlavoredNeutrinoContainer& rhs, const FlavoredNeutrinoContainer& state, double time)>;
-    TimeIntegrator(FlavoredNeutrinoContainer& old_data, FlavoredNeutrinoContainer& new_data, double time, int step);
+    TimeIntegrator(FlavoredNeutrinoContainer& old_data, FlavoredNeutrinoContainer& new_data, double time, int step,
+                   const Tableau& tableau = Tableau::preset(1, 4));
     ~TimeIntegrator();
     void set_rhs(Rhs f) { rhs_ = std::move(f); }
     void set_post_timestep(std::function<void()> f) { post_ = std::move(f); }
@@ -213,6 +229,7 @@ public:
 
 private:
     void advance(double dt);
+    Tableau tab_;
     FlavoredNeutrinoContainer* old_;
     FlavoredNeutrinoContainer* new_;
     std::vector<FlavoredNeutrinoContainer*> F_;

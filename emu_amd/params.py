@@ -12,6 +12,15 @@ from dataclasses import dataclass, field, fields
 
 from . import capi
 
+# AMReX RungeKuttaTimeIntegrator preset tableaus by integration.rk.type:
+# (rows of A below the diagonal, b)
+TABLEAUS = {
+    1: (((),), (1.0,)),                                                          # forward Euler
+    2: (((), (1.0,)), (0.5, 0.5)),                                               # trapezoid
+    3: (((), (1.0,), (0.25, 0.25)), (1. / 6., 1. / 6., 2. / 3.)),               # SSPRK3
+    4: (((), (0.5,), (0.0, 0.5), (0.0, 0.0, 1.0)), (1. / 6., 1. / 3., 1. / 3., 1. / 6.)),  # RK4
+}
+
 EV = 1.60218e-12
 C = 2.99792458e10
 C2 = C * C
@@ -78,6 +87,8 @@ class TestParams:
     st5_fznux: float = 0.0
     st5_amplitude: float = 0.0
     seed: int = 1234
+    integration_type: int = 1        # integration.type: 0 = ForwardEuler, 1 = RungeKutta
+    integration_rk_type: int = 4     # integration.rk.type: 1 Euler, 2 trapezoid, 3 SSPRK3, 4 RK4
     extra: dict = field(default_factory=dict)  # keys this build does not use
 
     # ---- deck I/O -------------------------------------------------------
@@ -85,7 +96,9 @@ class TestParams:
     def from_dict(cls, d: dict, **overrides) -> "TestParams":
         names = {f.name for f in fields(cls)}
         kw, extra = {}, {}
+        dotted = {"integration.type": "integration_type", "integration.rk.type": "integration_rk_type"}
         for k, v in {**d, **overrides}.items():
+            k = dotted.get(k, k)
             if k in names:
                 kw[k] = tuple(v) if isinstance(v, list) else v
             else:
@@ -155,6 +168,16 @@ class TestParams:
         c.st5[:] = st5
         c.seed = self.seed
         return c
+
+    def tableau(self):
+        """The AMReX RungeKuttaTimeIntegrator preset tableau (A rows, b) the
+        deck's integration keys select; integration.type = 0 is forward
+        Euler, the one-stage tableau."""
+        rk = 1 if self.integration_type == 0 else self.integration_rk_type
+        if self.integration_type not in (0, 1) or rk not in TABLEAUS:
+            raise ValueError(f"integration.type={self.integration_type}, integration.rk.type="
+                             f"{self.integration_rk_type} is not supported")
+        return TABLEAUS[rk]
 
     def st5_dict(self) -> dict:
         return {k[4:]: getattr(self, k) for k in (f.name for f in fields(self)) if k.startswith("st5_")}

@@ -72,6 +72,9 @@ enum {
     int or_rk4_step_##NFX(const or_params* p, void* state, int64_t np, double* grid,     \
                           double dt, double Vvac_max, double* dt_next,                   \
                           int64_t* bad_index);                                           \
+    int or_rk_step_##NFX(const or_params* p, void* state, int64_t np, double* grid,       \
+                         double dt, double Vvac_max, int nstages, const double* A,       \
+                         const double* B, double* dt_next, int64_t* bad_index);          \
     int or_evolve_##NFX(const or_params* p, void* state, int64_t np, double* grid,       \
                         int nsteps, double end_time, double* time, double* dt,           \
                         double* dt_log);

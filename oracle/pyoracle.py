@@ -519,6 +519,8 @@ def lib():
                 "or_apply_rhs": (None, [vp, i64, ctypes.c_double, vp]),
                 "or_rk4_step": (ctypes.c_int, [P, vp, i64, dp, ctypes.c_double, ctypes.c_double, dp,
                                                ctypes.POINTER(i64)]),
+                "or_rk_step": (ctypes.c_int, [P, vp, i64, dp, ctypes.c_double, ctypes.c_double, ctypes.c_int, dp,
+                                              dp, dp, ctypes.POINTER(i64)]),
                 "or_evolve": (ctypes.c_int, [P, vp, i64, dp, ctypes.c_int, ctypes.c_double, dp, dp, dp]),
             }
             for name, (res, args) in sig.items():
@@ -652,6 +654,29 @@ def rk4_step(nf, params, parts, grid: Grid, dt: float, vvac: float):
     bad = ctypes.c_int64(-1)
     err = _f(nf, "or_rk4_step")(ctypes.byref(params), parts.ctypes.data, len(parts), _dp(grid.data), dt,
                                 vvac, ctypes.byref(dtn), ctypes.byref(bad))
+    return err, dtn.value, bad.value
+
+
+# AMReX RungeKuttaTimeIntegrator preset tableaus by integration.rk.type (A row-major, b);
+# integration.type = 0 (the ForwardEuler integrator) is the 1-stage tableau.
+TABLEAUS = {
+    1: ([[0.0]], [1.0]),                                                     # forward Euler
+    2: ([[0.0, 0.0], [1.0, 0.0]], [0.5, 0.5]),                               # trapezoid
+    3: ([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.25, 0.25, 0.0]], [1. / 6., 1. / 6., 2. / 3.]),  # SSPRK3
+    4: ([[0.0, 0.0, 0.0, 0.0], [0.5, 0.0, 0.0, 0.0], [0.0, 0.5, 0.0, 0.0], [0.0, 0.0, 1.0, 0.0]],
+        [1. / 6., 1. / 3., 1. / 3., 1. / 6.]),                                # RK4
+}
+
+
+def rk_step(nf, params, parts, grid: Grid, dt: float, vvac: float, rk_type: int):
+    A, b = TABLEAUS[rk_type]
+    n = len(b)
+    Aa = np.ascontiguousarray(np.array(A, dtype=np.float64).reshape(n * n))
+    ba = np.ascontiguousarray(np.array(b, dtype=np.float64))
+    dtn = ctypes.c_double()
+    bad = ctypes.c_int64(-1)
+    err = _f(nf, "or_rk_step")(ctypes.byref(params), parts.ctypes.data, len(parts), _dp(grid.data), dt, vvac, n,
+                               _dp(Aa), _dp(ba), ctypes.byref(dtn), ctypes.byref(bad))
     return err, dtn.value, bad.value
 
 

@@ -47,7 +47,7 @@ def test_deck_parsing_and_units(tmp_path):
                     "theta12_degrees = 33.82\nst3_amplitude = 1e-6\nintegration.rk.type = 4\n")
     p = TestParams.from_file(str(deck), nf=2)
     assert p.simulation_type == 3 and p.ncell == (1, 1, 50) and p.st3_amplitude == 1e-6
-    assert p.extra["integration.rk.type"] == 4
+    assert p.integration_type == 1 and p.integration_rk_type == 4
     e = p.emu_params()
     # Parameters.H:81-85: mass *= eV/c^2, theta *= pi/180
     assert e.mass[0] == -0.008596511 * (1.60218e-12 / (2.99792458e10 * 2.99792458e10))

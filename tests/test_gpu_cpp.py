@@ -94,3 +94,32 @@ def test_cpp_renormalize_abort_is_a_status(cpp, tmp_path):
     deck_path.write_text(cases.deck_text(d))
     code, out, err = run(cpp, deck_path, "plain", None)
     assert code in (1, 2, 3) and "Renormalize" in err
+
+
+def test_cpp_ssprk3_plain_and_fused_refusal(cpp, oracle, tmp_path):
+    """integration.rk.type = 3 through the C++ TimeIntegrator against the
+    oracle's SSPRK3 step; the fused path (RK4 only) refuses the deck."""
+    nf, nsteps = 2, 3
+    d = cases.fast_flavor_dict(nf=nf, ncell=(4, 4, 4), nphi=4, nsteps=nsteps, end_time=1.0)
+    d["integration.type"], d["integration.rk.type"] = 1, 3
+    deck_path = tmp_path / "inputs"
+    deck_path.write_text(cases.deck_text(d))
+    dump = tmp_path / "final.bin"
+    code, out, err = run(cpp, deck_path, "plain", dump)
+    assert code == 0, err
+    deck = cases.oracle_deck(oracle, d)
+    prm = deck.params()
+    parts = oracle.init_particles(deck)
+    g = oracle.Grid(nf, prm)
+    vv = oracle.vvac_max(nf, prm, parts)
+    oracle.deposit_to_mesh(nf, prm, parts, g)
+    e, dt = oracle.compute_dt(nf, prm, g, vv)
+    for _ in range(nsteps):
+        e, dt, bad = oracle.rk_step(nf, prm, parts, g, dt, vv, 3)
+        assert e == 0
+    ids, pos, rdata, time = read_dump(dump, nf)
+    want = parts[np.argsort(parts["id"])]
+    np.testing.assert_allclose(pos, want["pos"], rtol=1e-13, atol=0)
+    assert cases.max_rel_err(rdata, want["rdata"]) < 1e-12
+    code, out, err = run(cpp, deck_path, "fused-fp64", None)
+    assert code == 12 and "RK4" in err

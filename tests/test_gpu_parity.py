@@ -179,6 +179,34 @@ def test_integrator_parity(core, oracle, nf):
     assert cases.max_rel_err(out["rdata"], parts["rdata"]) < 1e-12
 
 
+@pytest.mark.parametrize("itype,rk", [(0, 1), (1, 2), (1, 3)])
+def test_integrator_tableaus_parity(core, oracle, itype, rk):
+    """Forward Euler (integration.type = 0), trapezoid and SSPRK3 on the
+    per-function path against the oracle's tableau step, cfl_factor = -1 as
+    in the reference's convergence study."""
+    nf = 3
+    d = cases.fast_flavor_dict(nf=nf, ncell=(4, 4, 4), nphi=4, cfl_factor=-1.0)
+    d["integration.type"], d["integration.rk.type"] = itype, rk
+    deck, prm, emu = make(core, oracle, d)
+    parts = oracle.init_particles(deck)
+    c = core.FlavoredNeutrinoContainer.from_aos(emu, parts)
+    integ = core.Integrator(emu, c)
+    g = oracle.Grid(nf, prm)
+    vv = oracle.vvac_max(nf, prm, parts)
+    oracle.deposit_to_mesh(nf, prm, parts, g)
+    err, dt = oracle.compute_dt(nf, prm, g, vv)
+    assert err == 0 and integ.dt == pytest.approx(dt, rel=1e-13)
+    for _ in range(3):
+        err, dt, bad = oracle.rk_step(nf, prm, parts, g, dt, vv, rk)
+        assert err == 0
+    out = integ.integrate(3).to_aos()
+    assert integ.dt == pytest.approx(dt, rel=1e-13)
+    np.testing.assert_allclose(out["pos"], parts["pos"], rtol=1e-14, atol=0)
+    assert cases.max_rel_err(out["rdata"], parts["rdata"]) < 1e-12
+    with pytest.raises(ValueError):
+        core.FusedState(emu, c.np, "fp64")
+
+
 @pytest.mark.parametrize("nf", [2, 3])
 @pytest.mark.parametrize("ncell", [(4, 4, 4), (1, 1, 8)])
 def test_fused_fp64_parity(core, oracle, nf, ncell):

@@ -435,6 +435,42 @@ def test_rk4_fourth_order(oracle):
     assert errs[0] / errs[1] == pytest.approx(16.0, rel=0.15)
 
 
+@pytest.mark.parametrize("rk_type,order,n0", [(1, 1, 400), (2, 2, 40), (3, 3, 20), (4, 4, 20)])
+def test_rk_tableau_orders(oracle, rk_type, order, n0):
+    """Each preset tableau of integration.rk.type (the reference's convergence
+    study, Scripts/tests/convergence.sh) converges at its order on MSW."""
+    deck = kat.reference_deck("inputs_msw_test")
+    d = oracle.Deck(nf=2, simulation_type=0, ncell=(1, 1, 1), nphi_equator=1, rho_g_ccm=deck["rho_g_ccm"],
+                    Ye=1.0, mass1_eV=deck["mass1_eV"], mass2_eV=0.0, theta12_degrees=deck["theta12_degrees"],
+                    maxError=1.0)
+    K = {nm: i for i, nm in enumerate(oracle.attribute_names(2))}
+    T = 2.0e-11
+    errs = []
+    for nsteps in (n0, 2 * n0):
+        parts = oracle.init_particles(d)
+        p = d.params()
+        g = oracle.Grid(2, p)
+        for _ in range(nsteps):
+            oracle.rk_step(2, p, parts, g, T / nsteps, 0.0, rk_type)
+        r = parts["rdata"][0]
+        e = kat.msw_errors([r[K["time"]]], [r[K["f00_Re"]]], [r[K["f11_Re"]]], [r[K["f00_Rebar"]]],
+                           [r[K["f11_Rebar"]]])
+        errs.append(max(e["ee"], e["eebar"]))
+    assert errs[0] / errs[1] == pytest.approx(2.0 ** order, rel=0.2)
+
+
+def test_rk_step_rk4_tableau_is_rk4_step(oracle):
+    d = cases.oracle_deck(oracle, cases.fast_flavor_dict(nf=3, ncell=(2, 2, 2), nphi=4, st4_amplitude=1e-3))
+    a = oracle.init_particles(d)
+    b = a.copy()
+    p = d.params()
+    ga, gb = oracle.Grid(3, p), oracle.Grid(3, p)
+    ra = oracle.rk4_step(3, p, a, ga, 1e-12, 0.0)
+    rb = oracle.rk_step(3, p, b, gb, 1e-12, 0.0, 4)
+    assert ra == rb
+    np.testing.assert_array_equal(a["rdata"], b["rdata"])
+
+
 # --------------------------------------------------------------------------
 # The reference's analytic acceptance tests (Scripts/tests, .travis.yml:32-36)
 # --------------------------------------------------------------------------
