@@ -352,30 +352,50 @@ __global__ void k_min_pupt_t(const double* __restrict__ pupt, int64_t np, unsign
 }
 
 // Diagnostic: per cell N_01 = sum_p w N f01 and Nbar_01 (Re, Im), deposited
-// with the particle shape factors, then the cell sums of |N_01| and |Nbar_01|
-// and of N_00, Nbar_00.
+// with the particle shape factors, then the cell sums of |N_01| and |Nbar_01|.
+// One wave per source cell takes its particles 64 at a time; for each of the
+// 27 target cells the four weighted values are summed across the wave and
+// lane 0 adds them to the target (108 atomics per chunk, not per particle).
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
 template <int NF, typename T>
-__global__ void k_diag_deposit(Arrays<T> A, const int* __restrict__ cell_start, Geom g, int64_t np, double* out4)
+__global__ __launch_bounds__(64) void k_diag_deposit(Arrays<T> A, const int* __restrict__ cell_start, Geom g,
+                                                     int64_t np, double* out4)
 {
     const int cell = blockIdx.x;
-    for (int ip = cell_start[cell] + threadIdx.x; ip < cell_start[cell + 1]; ip += blockDim.x) {
+    const int lane = threadIdx.x;
+    const int beg = cell_start[cell], end = cell_start[cell + 1];
+    for (int base = beg; base < end; base += 64) {
+        const int ip = base + lane;
+        const bool act = ip < end;
+        const int ipl = act ? ip : beg;
         int idx[3];
         double wd[3][3];
-        for (int d = 0; d < 3; d++) shape3((A.pos[d][ip] - g.plo[d]) * g.dxi[d], g.order[d], idx[d], wd[d]);
-        const double N = A.Nn[0][ip], Nb = A.Nn[1][ip];
-        const double v[4] = {N * (double)A.f[(int64_t)herm_re<NF>(0, 1) * np + ip],
-                             N * (double)A.f[(int64_t)herm_im<NF>(0, 1) * np + ip],
-                             Nb * (double)A.f[(int64_t)(NH<NF> + herm_re<NF>(0, 1)) * np + ip],
-                             Nb * (double)A.f[(int64_t)(NH<NF> + herm_im<NF>(0, 1)) * np + ip]};
-        for (int kk = 0; kk < 3; kk++)
-            for (int jj = 0; jj < 3; jj++)
-                for (int ii = 0; ii < 3; ii++) {
-                    const double w = wd[0][ii] * wd[1][jj] * wd[2][kk];
-                    if (w == 0.0) continue;
-                    const int64_t c = ((int64_t)pmod(idx[2] + kk - 1, g.n[2]) * g.n[1] + pmod(idx[1] + jj - 1, g.n[1])) *
-                                          g.n[0] + pmod(idx[0] + ii - 1, g.n[0]);
-                    for (int q = 0; q < 4; q++) unsafeAtomicAdd(&out4[c * 4 + q], w * v[q]);
-                }
+        for (int d = 0; d < 3; d++) shape3((A.pos[d][ipl] - g.plo[d]) * g.dxi[d], g.order[d], idx[d], wd[d]);
+        const double N = act ? double(A.Nn[0][ipl]) : 0.0, Nb = act ? double(A.Nn[1][ipl]) : 0.0;
+        const double v[4] = {N * (double)A.f[(int64_t)herm_re<NF>(0, 1) * np + ipl],
+                             N * (double)A.f[(int64_t)herm_im<NF>(0, 1) * np + ipl],
+                             Nb * (double)A.f[(int64_t)(NH<NF> + herm_re<NF>(0, 1)) * np + ipl],
+                             Nb * (double)A.f[(int64_t)(NH<NF> + herm_im<NF>(0, 1)) * np + ipl]};
+        // every particle of the cell has the same first stencil cell per dimension
+        // (the stencil is centred on the particle's own cell)
+        const int cx = cell % g.n[0], cy = (cell / g.n[0]) % g.n[1], cz = cell / (g.n[0] * g.n[1]);
+        for (int t = 0; t < 27; t++) {
+            const int ii = t % 3, jj = (t / 3) % 3, kk = t / 9;
+            const double w = wd[0][ii] * wd[1][jj] * wd[2][kk];
+            double s4[4];
+            for (int q = 0; q < 4; q++) s4[q] = wave_sum(w * v[q]);
+            if (lane == 0 && (s4[0] != 0.0 || s4[1] != 0.0 || s4[2] != 0.0 || s4[3] != 0.0)) {
+                const int64_t c = ((int64_t)pmod(cz + kk - 1, g.n[2]) * g.n[1] + pmod(cy + jj - 1, g.n[1])) * g.n[0] +
+                                  pmod(cx + ii - 1, g.n[0]);
+                for (int q = 0; q < 4; q++) unsafeAtomicAdd(&out4[c * 4 + q], s4[q]);
+            }
+        }
+        (void)idx;
     }
 }
 
