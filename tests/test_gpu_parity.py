@@ -179,6 +179,28 @@ def test_integrator_parity(core, oracle, nf):
     assert cases.max_rel_err(out["rdata"], parts["rdata"]) < 1e-12
 
 
+@pytest.mark.parametrize("nf", [2, 3])
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 1e-5)])
+def test_flavor_amplitude_diagnostic(core, oracle, nf, prec, tol):
+    """emu_state_flavor_amplitude: sums over cells of |N_01| and |Nbar_01| of
+    the deposit (the growth-rate diagnostic) against the oracle's fab."""
+    d = cases.fast_flavor_dict(nf=nf, ncell=(4, 4, 4), nphi=4, st4_amplitude=1e-3)
+    deck, prm, emu = make(core, oracle, d)
+    parts = oracle.init_particles(deck)
+    c = core.FlavoredNeutrinoContainer.from_aos(emu, parts)
+    st = core.FusedState(emu, c.np, prec)
+    st.load(c)
+    got = st.flavor_amplitude()
+    st.close()
+    g = oracle.Grid(nf, prm)
+    oracle.deposit_to_mesh(nf, prm, parts, g)
+    K = {n: i for i, n in enumerate(oracle.grid_names(nf))}
+    v = g.valid()
+    a = np.sqrt(v[K["N01_Re"]] ** 2 + v[K["N01_Im"]] ** 2).sum()
+    b = np.sqrt(v[K["N01_Rebar"]] ** 2 + v[K["N01_Imbar"]] ** 2).sum()
+    assert got[0] == pytest.approx(a, rel=tol) and got[1] == pytest.approx(b, rel=tol)
+
+
 @pytest.mark.parametrize("itype,rk", [(0, 1), (1, 2), (1, 3)])
 def test_integrator_tableaus_parity(core, oracle, itype, rk):
     """Forward Euler (integration.type = 0), trapezoid and SSPRK3 on the
