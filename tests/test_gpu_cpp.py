@@ -80,7 +80,9 @@ def test_cpp_evolve_flavor_matches_oracle(cpp, oracle, tmp_path, nf, path, tol):
     want = parts[np.argsort(parts["id"])]
     np.testing.assert_array_equal(ids, want["id"])
     assert time == pytest.approx(t, rel=1e-13)
-    np.testing.assert_allclose(pos, want["pos"], rtol=1e-13, atol=0)
+    # FP32-bulk keeps positions in FP64 but the momenta in FP32, so the
+    # velocity carries FP32 rounding
+    np.testing.assert_allclose(pos, want["pos"], rtol=1e-13 if tol < 1e-10 else 1e-7, atol=0)
     got = want.copy()
     got["rdata"] = rdata
     assert cases.max_rel_err(got["rdata"], want["rdata"]) < tol
