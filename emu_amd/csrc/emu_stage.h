@@ -184,11 +184,16 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(2))) v
     constexpr int QG = 4 * nh;
     constexpr int NT = WX * WY * WZ;        // target cells of a source cell
     constexpr int MT = (NT + 15) / 16;       // MFMA row tiles
-    constexpr int NTL = (QG + 15) / 16;      // MFMA column tiles
+    constexpr int QGA = QG + (AUX ? NF + 3 * (NF * (NF - 1) / 2) : 0); // deposited columns
+    constexpr int NTL = (QGA + 15) / 16;     // MFMA column tiles
     constexpr int NPAIR = NF * (NF - 1) / 2;
-    constexpr int NAUX = 2 * NF + 3 * NPAIR; // compute_dt auxiliaries per cell
+    // compute_dt auxiliaries per cell: N_ii + Nbar_ii (i < NF), then Nbar Im(fbar_ab) v_d
+    // per pair a < b and direction d.  N_ii and Nbar_ii separately follow from the sum
+    // and M_ii = N_ii - Nbar_ii (+ matter on 00).  They ride in the deposit's spare
+    // MFMA columns (QG .. QG + NAUX - 1): at NF = 3, 36 + 12 = 3 column tiles exactly.
+    constexpr int NAUX = NF + 3 * NPAIR;
     constexpr int NSRC = nh + 3;             // q_c and vhat per particle
-    constexpr int NSAUX = 2 * NF + NPAIR;    // N f_ii, Nbar fbar_ii, Nbar Im(fbar_ab)
+    constexpr int NSAUX = NF + NPAIR;        // N f_ii + Nbar fbar_ii, Nbar Im(fbar_ab)
     extern __shared__ double lds[];
 
     const Geom& g = a.g;
@@ -251,13 +256,10 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(2))) v
         tdx[m][2] = 6 + (WZ == 1 ? 1 : kk);
     }
     mf_d4 acc[MT][NTL];
-    mf_d4 acca[MT];
 #pragma unroll
-    for (int m = 0; m < MT; m++) {
-        acca[m] = mf_d4{0.0, 0.0, 0.0, 0.0};
+    for (int m = 0; m < MT; m++)
 #pragma unroll
         for (int n = 0; n < NTL; n++) acc[m][n] = mf_d4{0.0, 0.0, 0.0, 0.0};
-    }
     const bool deposits = s <= 3;
 
     for (int base = beg; base < end; base += 64) {
@@ -422,11 +424,10 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(2))) v
         }
         if (aux) {
 #pragma unroll
-            for (int x = 0; x < NF; x++) {
-                Saux[lane * NSAUX + x] = act ? double(N * Dep[herm_re<NF>(x, x)]) : 0.0;
-                Saux[lane * NSAUX + NF + x] = act ? double(Nb * Dep[nh + herm_re<NF>(x, x)]) : 0.0;
-            }
-            int o = 2 * NF;
+            for (int x = 0; x < NF; x++)
+                Saux[lane * NSAUX + x] =
+                    act ? double(N * Dep[herm_re<NF>(x, x)]) + double(Nb * Dep[nh + herm_re<NF>(x, x)]) : 0.0;
+            int o = NF;
 #pragma unroll
             for (int x = 0; x < NF; x++)
 #pragma unroll
@@ -451,21 +452,17 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(2))) v
 #pragma unroll
             for (int n = 0; n < NTL; n++) {
                 const int q = 16 * n + (lane & 15);
-                const int f = q / nh, c = q % nh;
-                const double B = q < QG ? sr[c] * (f == 0 ? 1.0 : sr[nh + f - 1]) : 0.0;
-#pragma unroll
-                for (int m = 0; m < MT; m++) acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[m], B, acc[m][n], 0, 0, 0);
-            }
-            if (aux) {
-                const int q = lane & 15;
                 double B = 0.0;
-                if (q < 2 * NF) B = Saux[p * NSAUX + q];
-                else if (q < NAUX) {
-                    const int pr = (q - 2 * NF) / 3, d = (q - 2 * NF) % 3;
-                    B = Saux[p * NSAUX + 2 * NF + pr] * sr[nh + d];
+                if (q < QG) {
+                    const int f = q / nh, c = q % nh;
+                    B = sr[c] * (f == 0 ? 1.0 : sr[nh + f - 1]);
+                } else if (aux && q < QGA) {
+                    const int x = q - QG;
+                    if (x < NF) B = Saux[p * NSAUX + x];
+                    else B = Saux[p * NSAUX + NF + (x - NF) / 3] * sr[nh + (x - NF) % 3];
                 }
 #pragma unroll
-                for (int m = 0; m < MT; m++) acca[m] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[m], B, acca[m], 0, 0, 0);
+                for (int m = 0; m < MT; m++) acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[m], B, acc[m][n], 0, 0, 0);
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -487,8 +484,8 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(2))) v
             for (int n = 0; n < NTL; n++) {
                 const int q = 16 * n + (lane & 15);
                 if (q < QG) unsafeAtomicAdd(&a.gout[tc * QG + q], acc[m][n][r]);
+                else if (aux && q < QGA) unsafeAtomicAdd(&a.aux[tc * NAUX + (q - QG)], acc[m][n][r]);
             }
-            if (aux && (lane & 15) < NAUX) unsafeAtomicAdd(&a.aux[tc * NAUX + (lane & 15)], acca[m][r]);
         }
 }
 

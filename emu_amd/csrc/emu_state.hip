@@ -161,14 +161,16 @@ template <typename T> using Arrays = PArrays<T>;
 
 // compute_dt over the cells (Evolve.cpp:40-56) from the combined fields and
 // the auxiliaries.  HSI = N - conj(Nbar) + matter = M; F - Fbar keeps Re and the
-// diagonal of Phi and takes Im(F - Fbar) = Im(Phi) - 2 Im(Fbar).
+// diagonal of Phi and takes Im(F - Fbar) = Im(Phi) - 2 Im(Fbar).  The
+// auxiliaries hold S_ii = N_ii + Nbar_ii, so N_ii = (S_ii + M_ii - m00 d_i0) / 2
+// and Nbar_ii = (S_ii - M_ii + m00 d_i0) / 2, then Im(Fbar_ab) per direction.
 template <int NF>
 __global__ void k_dt_reduce(const double* __restrict__ grid, const double* __restrict__ aux, int64_t ncell,
                             double cell_volume, double m00, unsigned long long* red)
 {
     constexpr int nh = NH<NF>;
     constexpr int QG = 4 * nh;
-    constexpr int NAUX = 2 * NF + 3 * (NF * (NF - 1) / 2);
+    constexpr int NAUX = NF + 3 * (NF * (NF - 1) / 2);
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     double Vad = 0.0, Vst = 0.0;
     if (t < ncell) {
@@ -182,7 +184,7 @@ __global__ void k_dt_reduce(const double* __restrict__ grid, const double* __res
         for (int d = 0; d < 3; d++) {
 #pragma unroll
             for (int c = 0; c < nh; c++) h[c] = G[(d + 1) * nh + c];
-            int o = 2 * NF;
+            int o = NF;
 #pragma unroll
             for (int x = 0; x < NF; x++)
 #pragma unroll
@@ -195,8 +197,10 @@ __global__ void k_dt_reduce(const double* __restrict__ grid, const double* __res
         Vad = sqrt(V2) * M_SQRT2 * phys::GF / cell_volume;
 #pragma unroll
         for (int x = 0; x < NF; x++) {
-            Vst = Vst > A[x] ? Vst : A[x];
-            Vst = Vst > A[NF + x] ? Vst : A[NF + x];
+            const double Mxx = G[herm_re<NF>(x, x)] - (x == 0 ? m00 : 0.0);
+            const double Nxx = 0.5 * (A[x] + Mxx), Nbxx = 0.5 * (A[x] - Mxx);
+            Vst = Vst > Nxx ? Vst : Nxx;
+            Vst = Vst > Nbxx ? Vst : Nbxx;
         }
         Vst = Vst > m00 ? Vst : m00;
         Vst *= 2.0 * NF * M_SQRT2 * phys::GF / cell_volume;
@@ -529,7 +533,7 @@ void launch_stage_t(emu_state* st, StageArgs<T>& a)
     // per wave: the staged neighbourhood (gather stages), shape factors, sources, auxiliaries
     const size_t gs = a.stage == 0 ? 0 : ((size_t)WX * WY * WZ * 4 * nh * sizeof(T) + 7) / 8;
     const size_t lds =
-        (size_t)kWPB * (gs + 64 * (9 + (nh + 3) + (aux ? 2 * NF + NF * (NF - 1) / 2 : 0))) * sizeof(double);
+        (size_t)kWPB * (gs + 64 * (9 + (nh + 3) + (aux ? NF + NF * (NF - 1) / 2 : 0))) * sizeof(double);
     const int64_t nblk = (st->ncell + kWPB - 1) / kWPB;
     a.blocks_per_xcd = (int)((nblk + 7) / 8);
     const dim3 grid((unsigned)(8 * a.blocks_per_xcd)), block(64 * kWPB);
@@ -671,7 +675,7 @@ extern "C" int emu_state_create(emu_ctx* ctx, int64_t np, int precision, emu_sta
     st->nf = ctx->p.nf;
     st->nh = st->nf * st->nf;
     st->QG = 4 * st->nh;
-    st->naux = 2 * st->nf + 3 * (st->nf * (st->nf - 1) / 2);
+    st->naux = st->nf + 3 * (st->nf * (st->nf - 1) / 2);
     st->ncell = (int64_t)ctx->g.n[0] * ctx->g.n[1] * ctx->g.n[2];
     st->tsize = precision == EMU_FP32_BULK ? sizeof(float) : sizeof(double);
     st->m00 = ctx->p.rho * ctx->p.Ye / phys::Mp / ctx->g.inv_cell_volume;
