@@ -178,7 +178,7 @@ __device__ __forceinline__ void end_position(const double pos[3], const double v
 // S: 0 = deposit only (the step's result, or the initial state), 1..4 = RK
 // stage; AUX: also deposit the compute_dt auxiliaries (S = 0 or 3).
 template <int NF, typename T, int WX, int WY, int WZ, int WPB, int S, bool AUX>
-__global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(2))) void k_stage(StageArgs<T> a)
+__global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 2))) void k_stage(StageArgs<T> a)
 {
     constexpr int nh = NH<NF>;
     constexpr int QG = 4 * nh;
@@ -210,11 +210,13 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(2))) v
     if (beg >= end) return;
     const int cx = cell % g.n[0], cy = (cell / g.n[0]) % g.n[1], cz = cell / (g.n[0] * g.n[1]);
     constexpr int GS = s == 0 ? 0 : (NT * QG * (int)sizeof(T) + 7) / 8; // doubles of the staged neighbourhood
-    constexpr int per_wave = GS + 64 * (9 + NSRC + (aux ? NSAUX : 0));
+    constexpr int per_wave = GS + 64 * 9 + 64 * NSRC * (int)sizeof(T) / 8 + (aux ? 64 * NSAUX : 0); // doubles
     T* Gs = (T*)(lds + wave * per_wave);    // [NT][QG] neighbour fields (gather stages)
     double* Wf = lds + wave * per_wave + GS; // [64][9]   shape factors by dim
-    double* Src = Wf + 64 * 9;              // [64][NSRC] q_c, vhat (inside_c during the gather)
-    double* Saux = Src + 64 * NSRC;         // [64][NSAUX]
+    // q_c, vhat (inside_c during the gather): values of the working precision T,
+    // so FP32-bulk keeps them as float (lossless) and the LDS per wave shrinks
+    T* Src = (T*)(Wf + 64 * 9);             // [64][NSRC]
+    double* Saux = (double*)(Src + 64 * NSRC); // [64][NSAUX]
     // neighbour t of the cell (offsets -1, 0, +1 per dimension; a 3-wide
     // stencil implies at least 2 cells in that dimension, so one wrap suffices)
     auto wrap1i = [](int x, int n) { return x < 0 ? x + n : (x >= n ? x - n : x); };
@@ -284,7 +286,7 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(2))) v
 #pragma unroll
             for (int e = 0; e < 3; e++) Wf[lane * 9 + 3 * d + e] = act ? wd[d][e] : 0.0;
 #pragma unroll
-        for (int d = 0; d < 3; d++) Src[lane * NSRC + nh + d] = act ? double(vhat[d]) : 0.0;
+        for (int d = 0; d < 3; d++) Src[lane * NSRC + nh + d] = act ? vhat[d] : T(0);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -309,9 +311,9 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(2))) v
                     double w3[9];
 #pragma unroll
                     for (int e = 0; e < 9; e++) w3[e] = Wf[p * 9 + e];
-                    const double* vp = Src + p * NSRC + nh;
-                    const double v0 = vp[0], v1 = vp[1], v2 = vp[2];
-                    const T sg = T(g4 == 0 ? 1.0 : (g4 == 1 ? -v0 : (g4 == 2 ? -v1 : -v2)));
+                    const T* vp = Src + p * NSRC + nh;
+                    const T v0 = vp[0], v1 = vp[1], v2 = vp[2];
+                    const T sg = g4 == 0 ? T(1) : (g4 == 1 ? -v0 : (g4 == 2 ? -v1 : -v2));
                     typename MF::acc_t accg = {T(0), T(0), T(0), T(0)};
 #pragma unroll
                     for (int t = 0; t < NT; t++) {
@@ -323,14 +325,14 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(2))) v
                     }
                     if (c16 < nh) {
 #pragma unroll
-                        for (int r = 0; r < 4; r++) Src[(16 * m + MF::row(r, lane)) * NSRC + c16] = double(accg[r]);
+                        for (int r = 0; r < 4; r++) Src[(16 * m + MF::row(r, lane)) * NSRC + c16] = accg[r];
                     }
                 }
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
-                for (int c = 0; c < nh; c++) inside[c] = T(Src[lane * NSRC + c]);
+                for (int c = 0; c < nh; c++) inside[c] = Src[lane * NSRC + c];
             }
             // ---- the RK registers this stage reads, as one batch of loads whose
             // latency the Hamiltonian and commutator arithmetic covers
@@ -420,7 +422,7 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(2))) v
 #pragma unroll
         for (int c = 0; c < nh; c++) {
             const T qc = is_im_comp<NF>(c) ? N * Dep[c] + Nb * Dep[nh + c] : N * Dep[c] - Nb * Dep[nh + c];
-            Src[lane * NSRC + c] = act ? double(qc) : 0.0;
+            Src[lane * NSRC + c] = act ? qc : T(0);
         }
         if (aux) {
 #pragma unroll
@@ -442,7 +444,7 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(2))) v
         for (int kap = 0; kap < nk; kap++) {
             const int p = 4 * kap + (lane >> 4);
             const double* wf = Wf + p * 9;
-            const double* sr = Src + p * NSRC;
+            const T* sr = Src + p * NSRC;
             double A[MT];
 #pragma unroll
             for (int m = 0; m < MT; m++) {
@@ -455,11 +457,11 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(2))) v
                 double B = 0.0;
                 if (q < QG) {
                     const int f = q / nh, c = q % nh;
-                    B = sr[c] * (f == 0 ? 1.0 : sr[nh + f - 1]);
+                    B = double(sr[c]) * (f == 0 ? 1.0 : double(sr[nh + f - 1]));
                 } else if (aux && q < QGA) {
                     const int x = q - QG;
                     if (x < NF) B = Saux[p * NSAUX + x];
-                    else B = Saux[p * NSAUX + NF + (x - NF) / 3] * sr[nh + (x - NF) % 3];
+                    else B = Saux[p * NSAUX + NF + (x - NF) / 3] * double(sr[nh + (x - NF) % 3]);
                 }
 #pragma unroll
                 for (int m = 0; m < MT; m++) acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[m], B, acc[m][n], 0, 0, 0);

@@ -533,7 +533,8 @@ void launch_stage_t(emu_state* st, StageArgs<T>& a)
     // per wave: the staged neighbourhood (gather stages), shape factors, sources, auxiliaries
     const size_t gs = a.stage == 0 ? 0 : ((size_t)WX * WY * WZ * 4 * nh * sizeof(T) + 7) / 8;
     const size_t lds =
-        (size_t)kWPB * (gs + 64 * (9 + (nh + 3) + (aux ? NF + NF * (NF - 1) / 2 : 0))) * sizeof(double);
+        (size_t)kWPB * ((gs + 64 * (9 + (aux ? NF + NF * (NF - 1) / 2 : 0))) * sizeof(double) +
+                        64 * (nh + 3) * sizeof(T));
     const int64_t nblk = (st->ncell + kWPB - 1) / kWPB;
     a.blocks_per_xcd = (int)((nblk + 7) / 8);
     const dim3 grid((unsigned)(8 * a.blocks_per_xcd)), block(64 * kWPB);
