@@ -192,7 +192,7 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
     // and M_ii = N_ii - Nbar_ii (+ matter on 00).  They ride in the deposit's spare
     // MFMA columns (QG .. QG + NAUX - 1): at NF = 3, 36 + 12 = 3 column tiles exactly.
     constexpr int NAUX = NF + 3 * NPAIR;
-    constexpr int NSRC = nh + 3;             // q_c and vhat per particle
+    constexpr int NSRC = nh + 4;             // q_c, vhat and the constant 1 per particle
     constexpr int NSAUX = NF + NPAIR;        // N f_ii + Nbar fbar_ii, Nbar Im(fbar_ab)
     extern __shared__ double lds[];
 
@@ -246,16 +246,42 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
     const double bw[4] = {1. / 6., 1. / 3., 1. / 3., 1. / 6.};
     const double cnext[4] = {0.5, 0.5, 1.0, 0.0};
 
-    // lane constants for the MFMA operands: target (row) and field column of
-    // this lane in each tile
+    // lane constants of the deposit's MFMA operands.  A (row t = 16m + l%16):
+    // the three shape factors of target t, clamped to a valid target beyond
+    // NT and masked.  B (column q = 16n + l%16): the product of two entries of
+    // the particle's source row, the constant 1 standing in where a column
+    // has one factor; the auxiliary columns read the auxiliary row first.
     int tdx[MT][3];
+    bool tok[MT];
 #pragma unroll
     for (int m = 0; m < MT; m++) {
         const int t = 16 * m + (lane & 15);
-        const int ii = t % WX, jj = (t / WX) % WY, kk = t / (WX * WY);
+        tok[m] = t < NT;
+        const int tc = t < NT ? t : NT - 1;
+        const int ii = tc % WX, jj = (tc / WX) % WY, kk = tc / (WX * WY);
         tdx[m][0] = WX == 1 ? 1 : ii;
         tdx[m][1] = 3 + (WY == 1 ? 1 : jj);
         tdx[m][2] = 6 + (WZ == 1 ? 1 : kk);
+    }
+    int bi1[NTL], bi2[NTL], bkind[NTL]; // kind: 0 main, 1 auxiliary, 2 zero
+#pragma unroll
+    for (int n = 0; n < NTL; n++) {
+        const int q = 16 * n + (lane & 15);
+        if (q < QG) {
+            const int f = q / nh, c = q % nh;
+            bkind[n] = 0;
+            bi1[n] = c;
+            bi2[n] = f == 0 ? nh + 3 : nh + f - 1;
+        } else if (aux && q < QGA) {
+            const int x = q - QG;
+            bkind[n] = 1;
+            bi1[n] = x < NF ? x : NF + (x - NF) / 3;
+            bi2[n] = x < NF ? nh + 3 : nh + (x - NF) % 3;
+        } else {
+            bkind[n] = 2;
+            bi1[n] = 0;
+            bi2[n] = nh + 3;
+        }
     }
     mf_d4 acc[MT][NTL];
 #pragma unroll
@@ -287,6 +313,7 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
             for (int e = 0; e < 3; e++) Wf[lane * 9 + 3 * d + e] = act ? wd[d][e] : 0.0;
 #pragma unroll
         for (int d = 0; d < 3; d++) Src[lane * NSRC + nh + d] = act ? vhat[d] : T(0);
+        Src[lane * NSRC + nh + 3] = T(1);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -445,24 +472,19 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
             const int p = 4 * kap + (lane >> 4);
             const double* wf = Wf + p * 9;
             const T* sr = Src + p * NSRC;
+            const double* sa = Saux + p * NSAUX;
             double A[MT];
 #pragma unroll
             for (int m = 0; m < MT; m++) {
-                const int t = 16 * m + (lane & 15);
-                A[m] = t < NT ? wf[tdx[m][0]] * wf[tdx[m][1]] * wf[tdx[m][2]] : 0.0;
+                const double w = wf[tdx[m][0]] * wf[tdx[m][1]] * wf[tdx[m][2]];
+                A[m] = tok[m] ? w : 0.0;
             }
 #pragma unroll
             for (int n = 0; n < NTL; n++) {
-                const int q = 16 * n + (lane & 15);
-                double B = 0.0;
-                if (q < QG) {
-                    const int f = q / nh, c = q % nh;
-                    B = double(sr[c]) * (f == 0 ? 1.0 : double(sr[nh + f - 1]));
-                } else if (aux && q < QGA) {
-                    const int x = q - QG;
-                    if (x < NF) B = Saux[p * NSAUX + x];
-                    else B = Saux[p * NSAUX + NF + (x - NF) / 3] * double(sr[nh + (x - NF) % 3]);
-                }
+                const double f2 = double(sr[bi2[n]]);
+                double B = double(sr[bi1[n]]) * f2;
+                if (aux) B = bkind[n] == 1 ? sa[bi1[n]] * f2 : B;
+                B = bkind[n] == 2 ? 0.0 : B;
 #pragma unroll
                 for (int m = 0; m < MT; m++) acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[m], B, acc[m][n], 0, 0, 0);
             }
