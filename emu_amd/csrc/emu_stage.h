@@ -175,6 +175,17 @@ __device__ __forceinline__ void end_position(const double pos[3], const double v
     newcell = (nc[2] * g.n[1] + nc[1]) * g.n[0] + nc[0];
 }
 
+// LDS of one wave of k_stage, in doubles: the staged neighbourhood (gather
+// stages, NT x 4 NH values of T), 9 shape factors per particle (double), the
+// source row (NH + 4 values of T) and, with AUX, NF + NF(NF-1)/2 auxiliary
+// sources (double).  The host sizes the launch with this same function.
+template <int NF, typename T, int WX, int WY, int WZ, int S, bool AUX> constexpr int stage_lds_doubles_per_wave()
+{
+    constexpr int nh = NF * NF;
+    constexpr int gs = S == 0 ? 0 : (WX * WY * WZ * 4 * nh * (int)sizeof(T) + 7) / 8;
+    return gs + 64 * 9 + 64 * (nh + 4) * (int)sizeof(T) / 8 + (AUX ? 64 * (NF + NF * (NF - 1) / 2) : 0);
+}
+
 // S: 0 = deposit only (the step's result, or the initial state), 1..4 = RK
 // stage; AUX: also deposit the compute_dt auxiliaries (S = 0 or 3).
 template <int NF, typename T, int WX, int WY, int WZ, int WPB, int S, bool AUX>
@@ -210,7 +221,9 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
     if (beg >= end) return;
     const int cx = cell % g.n[0], cy = (cell / g.n[0]) % g.n[1], cz = cell / (g.n[0] * g.n[1]);
     constexpr int GS = s == 0 ? 0 : (NT * QG * (int)sizeof(T) + 7) / 8; // doubles of the staged neighbourhood
-    constexpr int per_wave = GS + 64 * 9 + 64 * NSRC * (int)sizeof(T) / 8 + (aux ? 64 * NSAUX : 0); // doubles
+    constexpr int per_wave = stage_lds_doubles_per_wave<NF, T, WX, WY, WZ, S, AUX>();
+    static_assert(per_wave == GS + 64 * 9 + 64 * NSRC * (int)sizeof(T) / 8 + (aux ? 64 * NSAUX : 0),
+                  "LDS layout and its size function disagree");
     T* Gs = (T*)(lds + wave * per_wave);    // [NT][QG] neighbour fields (gather stages)
     double* Wf = lds + wave * per_wave + GS; // [64][9]   shape factors by dim
     // q_c, vhat (inside_c during the gather): values of the working precision T,

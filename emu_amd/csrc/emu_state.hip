@@ -525,32 +525,32 @@ int scan_counts(emu_state* st, int* out)
 
 constexpr int kWPB = 2; // waves (cells in flight) per block of the stage kernel
 
+template <int NF, typename T, int WX, int WY, int WZ, int S, bool AUX>
+void launch_stage_k(emu_state* st, StageArgs<T>& a, dim3 grid, dim3 block)
+{
+    const size_t lds = (size_t)kWPB * stage_lds_doubles_per_wave<NF, T, WX, WY, WZ, S, AUX>() * sizeof(double);
+    hipLaunchKernelGGL((k_stage<NF, T, WX, WY, WZ, kWPB, S, AUX>), grid, block, lds, st->ctx->stream, a);
+}
+
 template <int NF, typename T, int WX, int WY, int WZ>
 void launch_stage_t(emu_state* st, StageArgs<T>& a)
 {
-    constexpr int nh = NH<NF>;
     const int aux = a.want_aux && (a.stage == 3 || a.stage == 0);
-    // per wave: the staged neighbourhood (gather stages), shape factors, sources, auxiliaries
-    const size_t gs = a.stage == 0 ? 0 : ((size_t)WX * WY * WZ * 4 * nh * sizeof(T) + 7) / 8;
-    const size_t lds =
-        (size_t)kWPB * ((gs + 64 * (9 + (aux ? NF + NF * (NF - 1) / 2 : 0))) * sizeof(double) +
-                        64 * (nh + 3) * sizeof(T));
     const int64_t nblk = (st->ncell + kWPB - 1) / kWPB;
     a.blocks_per_xcd = (int)((nblk + 7) / 8);
     const dim3 grid((unsigned)(8 * a.blocks_per_xcd)), block(64 * kWPB);
-    hipStream_t s = st->ctx->stream;
     switch (a.stage) {
     case 0:
-        if (aux) hipLaunchKernelGGL((k_stage<NF, T, WX, WY, WZ, kWPB, 0, true>), grid, block, lds, s, a);
-        else hipLaunchKernelGGL((k_stage<NF, T, WX, WY, WZ, kWPB, 0, false>), grid, block, lds, s, a);
+        if (aux) launch_stage_k<NF, T, WX, WY, WZ, 0, true>(st, a, grid, block);
+        else launch_stage_k<NF, T, WX, WY, WZ, 0, false>(st, a, grid, block);
         break;
-    case 1: hipLaunchKernelGGL((k_stage<NF, T, WX, WY, WZ, kWPB, 1, false>), grid, block, lds, s, a); break;
-    case 2: hipLaunchKernelGGL((k_stage<NF, T, WX, WY, WZ, kWPB, 2, false>), grid, block, lds, s, a); break;
+    case 1: launch_stage_k<NF, T, WX, WY, WZ, 1, false>(st, a, grid, block); break;
+    case 2: launch_stage_k<NF, T, WX, WY, WZ, 2, false>(st, a, grid, block); break;
     case 3:
-        if (aux) hipLaunchKernelGGL((k_stage<NF, T, WX, WY, WZ, kWPB, 3, true>), grid, block, lds, s, a);
-        else hipLaunchKernelGGL((k_stage<NF, T, WX, WY, WZ, kWPB, 3, false>), grid, block, lds, s, a);
+        if (aux) launch_stage_k<NF, T, WX, WY, WZ, 3, true>(st, a, grid, block);
+        else launch_stage_k<NF, T, WX, WY, WZ, 3, false>(st, a, grid, block);
         break;
-    default: hipLaunchKernelGGL((k_stage<NF, T, WX, WY, WZ, kWPB, 4, false>), grid, block, lds, s, a); break;
+    default: launch_stage_k<NF, T, WX, WY, WZ, 4, false>(st, a, grid, block); break;
     }
 }
 

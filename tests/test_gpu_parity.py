@@ -272,6 +272,33 @@ def test_fused_fp64_computed_dt(core, oracle, nf):
     assert cases.max_rel_err(got["rdata"], want["rdata"]) < 1e-11
 
 
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-11), ("fp32", 2e-5)])
+@pytest.mark.parametrize("nf", [2, 3])
+def test_fused_full_chunks(core, oracle, nf, prec, tol):
+    """92 particles per cell (uniform_sphere_xyz(16), the benchmark's direction
+    set): every cell fills one 64-particle chunk and part of a second, so each
+    row of the per-wave LDS staging is used; computed dt, so the auxiliary
+    deposit runs too."""
+    d = cases.fast_flavor_dict(nf=nf, ncell=(4, 4, 4), nphi=16, st4_amplitude=1e-3)
+    deck, prm, emu = make(core, oracle, d)
+    parts = oracle.init_particles(deck)
+    assert len(parts) == 64 * 92
+    c = core.FlavoredNeutrinoContainer.from_aos(emu, parts)
+    st = core.FusedState(emu, c.np, prec)
+    st.load(c)
+    dt0 = st.info()["dt"]
+    dts, dt_next = oracle_steps(oracle, nf, prm, parts, 4)
+    assert dt0 == pytest.approx(dts[0], rel=1e-14 if prec == "fp64" else 1e-6)
+    st.evolve(4)
+    st.store(c)
+    assert st.info()["dt"] == pytest.approx(dt_next, rel=1e-13 if prec == "fp64" else 1e-5)
+    st.close()
+    got = cases.by_id(c.to_aos())
+    want = cases.by_id(parts)
+    np.testing.assert_allclose(got["pos"], want["pos"], rtol=1e-14 if prec == "fp64" else 1e-7)
+    assert cases.max_rel_err(got["rdata"], want["rdata"]) < tol
+
+
 @pytest.mark.parametrize("nf", [2, 3])
 def test_fused_fp32_bulk_tolerance(core, oracle, nf):
     """FP32 bulk (FP64 positions, deposit accumulation, Renormalize, dt): fields
