@@ -223,7 +223,10 @@ def main():
     if "fp32" in res and "fp64" in res:
         g32, g64 = res["fp32"]["growth_rate"], res["fp64"]["growth_rate"]
         out["fp32_vs_fp64"] = {"speedup": res["fp32"]["updates_per_s"] / res["fp64"]["updates_per_s"],
-                               "growth_rate_rel_err": abs(g32 - g64) / abs(g64) if g64 else None}
+                               "growth_rate_rel_err": abs(g32 - g64) / abs(g64) if g64 else None,
+                               "growth_rate_fp32": g32, "growth_rate_fp64": g64,
+                               "window_s_fp32": res["fp32"]["time_window"], "window_s_fp64": res["fp64"]["time_window"],
+                               "amplitude_fp32": res["fp32"]["amp"], "amplitude_fp64": res["fp64"]["amp"]}
     out["kernels_ms"] = {p: {k: res[p][k] for k in ("stage_ms", "fill_ms", "sort_ms", "dt_ms")} for p in res}
     if not args.no_cpu:
         d = {k: getattr(parms, k) for k in parms.__dataclass_fields__ if k != "extra"}

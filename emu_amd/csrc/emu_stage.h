@@ -479,27 +479,37 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         // ---- D += W^T R, four particles per MFMA
+        // two k-steps (8 particles) per iteration, so the operand reads of one
+        // overlap the matrix work of the other; rows past the chunk's particles
+        // hold zero sources (inactive lanes), so rounding the count up is exact
+        // (the deposit-only launch keeps one k-step per iteration: there the
+        // registers of the second would cost a wave of occupancy)
+        constexpr int KU = s == 0 ? 1 : 2;
         const int nact = min(64, end - base);
-        const int nk = (nact + 3) >> 2;
-        for (int kap = 0; kap < nk; kap++) {
-            const int p = 4 * kap + (lane >> 4);
-            const double* wf = Wf + p * 9;
-            const T* sr = Src + p * NSRC;
-            const double* sa = Saux + p * NSAUX;
-            double A[MT];
+        const int nkg = (nact + 4 * KU - 1) / (4 * KU);
+        for (int kp = 0; kp < nkg; kp++) {
 #pragma unroll
-            for (int m = 0; m < MT; m++) {
-                const double w = wf[tdx[m][0]] * wf[tdx[m][1]] * wf[tdx[m][2]];
-                A[m] = tok[m] ? w : 0.0;
-            }
+            for (int h = 0; h < KU; h++) {
+                const int p = 4 * KU * kp + 4 * h + (lane >> 4);
+                const double* wf = Wf + p * 9;
+                const T* sr = Src + p * NSRC;
+                const double* sa = Saux + p * NSAUX;
+                double A[MT];
 #pragma unroll
-            for (int n = 0; n < NTL; n++) {
-                const double f2 = double(sr[bi2[n]]);
-                double B = double(sr[bi1[n]]) * f2;
-                if (aux) B = bkind[n] == 1 ? sa[bi1[n]] * f2 : B;
-                B = bkind[n] == 2 ? 0.0 : B;
+                for (int m = 0; m < MT; m++) {
+                    const double w = wf[tdx[m][0]] * wf[tdx[m][1]] * wf[tdx[m][2]];
+                    A[m] = tok[m] ? w : 0.0;
+                }
 #pragma unroll
-                for (int m = 0; m < MT; m++) acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[m], B, acc[m][n], 0, 0, 0);
+                for (int n = 0; n < NTL; n++) {
+                    const double f2 = double(sr[bi2[n]]);
+                    double B = double(sr[bi1[n]]) * f2;
+                    if (aux) B = bkind[n] == 1 ? sa[bi1[n]] * f2 : B;
+                    B = bkind[n] == 2 ? 0.0 : B;
+#pragma unroll
+                    for (int m = 0; m < MT; m++)
+                        acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[m], B, acc[m][n], 0, 0, 0);
+                }
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
