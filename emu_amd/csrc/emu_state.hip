@@ -741,7 +741,7 @@ template <typename T> static int load_t(emu_state* st, const PView& P, const int
         using I2 = PIdx<2>;
         const int iN = I2::N, iNb = st->nf == 2 ? PIdx<2>::Nbar : PIdx<3>::Nbar;
         hipLaunchKernelGGL(k_fill_u64v, dim3(1), dim3(64), 0, s, st->red + 4, 1, 0ull);
-        hipLaunchKernelGGL(k_absmax2, dim3(grid_blocks(np)), dim3(256), 0, s, P.r[iN], P.r[iNb], np, st->red + 4);
+        hipLaunchKernelGGL(k_absmax2, dim3(std::min(grid_blocks(np), 1024u)), dim3(256), 0, s, P.r[iN], P.r[iNb], np, st->red + 4);
         unsigned long long h = 0;
         EMU_HIP(ctx, hipMemcpyAsync(&h, st->red + 4, 8, hipMemcpyDeviceToHost, s));
         EMU_HIP(ctx, hipStreamSynchronize(s));
@@ -799,7 +799,7 @@ extern "C" int emu_state_load(emu_state* st, const emu_particles* parts, const i
     // Vvac_max from the minimum energy (FlavoredNeutrinoContainerInit.cpp:657-660)
     hipLaunchKernelGGL(k_fill_u64v, dim3(1), dim3(64), 0, s, st->red, 4, ~0ull >> 1);
     if (st->np > 0)
-        hipLaunchKernelGGL(k_min_pupt_t, dim3(grid_blocks(st->np)), dim3(256), 0, s, parts->rdata[PIdx<2>::pupt], st->np,
+        hipLaunchKernelGGL(k_min_pupt_t, dim3(std::min(grid_blocks(st->np), 1024u)), dim3(256), 0, s, parts->rdata[PIdx<2>::pupt], st->np,
                            st->red + 3);
     unsigned long long h;
     EMU_HIP(ctx, hipMemcpyAsync(&h, st->red + 3, 8, hipMemcpyDeviceToHost, s));
