@@ -175,20 +175,60 @@ __device__ __forceinline__ void end_position(const double pos[3], const double v
     newcell = (nc[2] * g.n[1] + nc[1]) * g.n[0] + nc[0];
 }
 
-// LDS of one wave of k_stage, in doubles: the staged neighbourhood (gather
-// stages, NT x 4 NH values of T), 9 shape factors per particle (double), the
-// source row (NH + 4 values of T) and, with AUX, NF + NF(NF-1)/2 auxiliary
-// sources (double).  The host sizes the launch with this same function.
-template <int NF, typename T, int WX, int WY, int WZ, int S, bool AUX> constexpr int stage_lds_doubles_per_wave()
+// Two ways to map waves to cells.  PM (patch mode): a workgroup of WPB = 4
+// waves takes a patch of 4 cells, 2 x 2 in x, y when the stencil is 3 wide
+// there, else 4 along z; the patch's neighbourhood box (PX + WX - 1) x
+// (PY + WY - 1) x (PZ + WZ - 1) is staged once for the four gathers, and the
+// four deposits are summed over the same box in LDS before one set of global
+// atomics (56% fewer at the 3-D stencil).  Without PM every wave is its own
+// patch of one cell (box = its 27 neighbours) and flushes directly, with no
+// workgroup barrier: the barriers cost more than the merge saves in the RK
+// stages, so PM serves the deposit-heavy launches.
+template <int WX, int WY, int WZ, int WPB, bool PM> struct Patch {
+    static constexpr int PX = PM ? (WX == 3 ? 2 : 1) : 1;
+    static constexpr int PY = PM ? (WY == 3 ? 2 : 1) : 1;
+    static constexpr int PZ = PM ? WPB / (PX * PY) : 1;
+    static constexpr int CELLS = PM ? WPB : 1; // cells of one patch
+    static_assert(PX * PY * PZ == CELLS, "a patch holds one cell per wave");
+    static constexpr int NBX = PX + WX - 1, NBY = PY + WY - 1, NBZ = PZ + WZ - 1;
+    static constexpr int NB = NBX * NBY * NBZ;
+};
+
+// LDS of one wave's own region, in doubles: 9 shape factors per particle
+// (double), the source row (NH + 4 values of T) and, with AUX, NF + NF(NF-1)/2
+// auxiliary sources (double).
+template <int NF, typename T, bool AUX> constexpr int stage_lds_doubles_per_wave()
 {
     constexpr int nh = NF * NF;
-    constexpr int gs = S == 0 ? 0 : (WX * WY * WZ * 4 * nh * (int)sizeof(T) + 7) / 8;
-    return gs + 64 * 9 + 64 * (nh + 4) * (int)sizeof(T) / 8 + (AUX ? 64 * (NF + NF * (NF - 1) / 2) : 0);
+    return 64 * 9 + 64 * (nh + 4) * (int)sizeof(T) / 8 + (AUX ? 64 * (NF + NF * (NF - 1) / 2) : 0);
+}
+
+// LDS of a workgroup, in doubles.  PM: the staged neighbourhood box (gather
+// stages, NB x 4 NH values of T) and the waves' regions; the deposit's merge
+// buffer (NB x deposited columns, double) reuses the same space once every
+// wave is done with it.  Without PM: per wave, its own box and its region.
+// The host sizes the launch with this function.
+template <int NF, typename T, int WX, int WY, int WZ, int WPB, int S, bool AUX, bool PM>
+constexpr int stage_lds_doubles_per_block()
+{
+    constexpr int nh = NF * NF;
+    constexpr int NB = Patch<WX, WY, WZ, WPB, PM>::NB;
+    constexpr int gsh = S == 0 ? 0 : (NB * 4 * nh * (int)sizeof(T) + 7) / 8;
+    if (!PM) return WPB * (gsh + stage_lds_doubles_per_wave<NF, T, AUX>());
+    constexpr int work = gsh + WPB * stage_lds_doubles_per_wave<NF, T, AUX>();
+    constexpr int merge = S <= 3 ? NB * (4 * nh + (AUX ? NF + 3 * (NF * (NF - 1) / 2) : 0)) : 0;
+    return work > merge ? work : merge;
+}
+
+// the launches that take the patch mode
+template <typename T, int S, bool AUX> constexpr bool stage_patch_mode()
+{
+    return S == 0 || (AUX && sizeof(T) == 8);
 }
 
 // S: 0 = deposit only (the step's result, or the initial state), 1..4 = RK
 // stage; AUX: also deposit the compute_dt auxiliaries (S = 0 or 3).
-template <int NF, typename T, int WX, int WY, int WZ, int WPB, int S, bool AUX>
+template <int NF, typename T, int WX, int WY, int WZ, int WPB, int S, bool AUX, bool PM>
 __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 2))) void k_stage(StageArgs<T> a)
 {
     constexpr int nh = NH<NF>;
@@ -210,48 +250,78 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
     const Geom& g = a.g;
     constexpr int s = S;
     constexpr bool aux = AUX;
+    using PT = Patch<WX, WY, WZ, WPB, PM>;
+    constexpr int NB = PT::NB;
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int lane = threadIdx.x & 63;
-    // blocks go round-robin over the 8 XCDs; give each XCD a contiguous run of cells
+    // blocks go round-robin over the 8 XCDs; give each XCD a contiguous run of patches
+    const int npx = (g.n[0] + PT::PX - 1) / PT::PX, npy = (g.n[1] + PT::PY - 1) / PT::PY;
+    const int npz = (g.n[2] + PT::PZ - 1) / PT::PZ;
     const int b = blockIdx.x;
-    const int cblock = (b & 7) * a.blocks_per_xcd + (b >> 3);
-    const int cell = __builtin_amdgcn_readfirstlane(cblock * WPB + wave);
-    if (cell >= a.ncell) return;
-    const int beg = a.cell_start[cell], end = a.cell_start[cell + 1];
-    if (beg >= end) return;
-    const int cx = cell % g.n[0], cy = (cell / g.n[0]) % g.n[1], cz = cell / (g.n[0] * g.n[1]);
-    constexpr int GS = s == 0 ? 0 : (NT * QG * (int)sizeof(T) + 7) / 8; // doubles of the staged neighbourhood
-    constexpr int per_wave = stage_lds_doubles_per_wave<NF, T, WX, WY, WZ, S, AUX>();
-    static_assert(per_wave == GS + 64 * 9 + 64 * NSRC * (int)sizeof(T) / 8 + (aux ? 64 * NSAUX : 0),
+    const int bgroup = (b & 7) * a.blocks_per_xcd + (b >> 3);
+    // PM: the block's patch; otherwise this wave's one-cell patch, WPB consecutive per block
+    const int patch = __builtin_amdgcn_readfirstlane(PM ? bgroup : bgroup * WPB + wave);
+    if (patch >= npx * npy * npz) return; // uniform over the workgroup (PM) or the wave
+    const int bx = (patch % npx) * PT::PX, by = ((patch / npx) % npy) * PT::PY, bz = (patch / (npx * npy)) * PT::PZ;
+    const int pw = PM ? wave : 0; // this wave's cell within the patch
+    const int ox = pw % PT::PX, oy = (pw / PT::PX) % PT::PY, oz = pw / (PT::PX * PT::PY);
+    const int cx = bx + ox, cy = by + oy, cz = bz + oz;
+    // a wave whose cell lies outside the domain (a partial patch) or holds no
+    // particle has nothing to do, but it stays for the workgroup's barriers
+    const bool inside_domain = cx < g.n[0] && cy < g.n[1] && cz < g.n[2];
+    const int cell = inside_domain ? (cz * g.n[1] + cy) * g.n[0] + cx : 0;
+    const int beg = inside_domain ? a.cell_start[cell] : 0;
+    const int end = inside_domain ? a.cell_start[cell + 1] : 0;
+    // cell i of the neighbourhood box, box index (i, j, k) <-> the patch base
+    // minus one in each 3-wide dimension, plus (i, j, k), periodic
+    // (one conditional wrap covers a single cell's box: offsets are -1..+1 and a
+    // 3-wide stencil implies at least 2 cells in that dimension)
+    auto wrapc = [](int x, int n) { return PM ? pmod(x, n) : (x < 0 ? x + n : (x >= n ? x - n : x)); };
+    auto box_cell = [&](int e) -> int64_t {
+        const int i = e % PT::NBX, j = (e / PT::NBX) % PT::NBY, k = e / (PT::NBX * PT::NBY);
+        const int ci = wrapc(bx - (WX == 3 ? 1 : 0) + i, g.n[0]);
+        const int cj = wrapc(by - (WY == 3 ? 1 : 0) + j, g.n[1]);
+        const int ck = wrapc(bz - (WZ == 3 ? 1 : 0) + k, g.n[2]);
+        return ((int64_t)ck * g.n[1] + cj) * g.n[0] + ci;
+    };
+    // box index of this wave's stencil target t
+    auto box_of = [&](int t) -> int {
+        const int ii = t % WX, jj = (t / WX) % WY, kk = t / (WX * WY);
+        return ((oz + kk) * PT::NBY + (oy + jj)) * PT::NBX + (ox + ii);
+    };
+    constexpr int GSH = s == 0 ? 0 : (NB * QG * (int)sizeof(T) + 7) / 8; // doubles of the staged box
+    constexpr int per_wave = stage_lds_doubles_per_wave<NF, T, AUX>();
+    static_assert(per_wave == 64 * 9 + 64 * NSRC * (int)sizeof(T) / 8 + (aux ? 64 * NSAUX : 0),
                   "LDS layout and its size function disagree");
-    T* Gs = (T*)(lds + wave * per_wave);    // [NT][QG] neighbour fields (gather stages)
-    double* Wf = lds + wave * per_wave + GS; // [64][9]   shape factors by dim
+    // [NB][QG] neighbourhood fields (gather stages): the block's, or this wave's
+    T* Gs = (T*)(PM ? lds : lds + wave * (GSH + per_wave));
+    double* Wf = PM ? lds + GSH + wave * per_wave : lds + wave * (GSH + per_wave) + GSH; // [64][9] shape factors
     // q_c, vhat (inside_c during the gather): values of the working precision T,
     // so FP32-bulk keeps them as float (lossless) and the LDS per wave shrinks
     T* Src = (T*)(Wf + 64 * 9);             // [64][NSRC]
     double* Saux = (double*)(Src + 64 * NSRC); // [64][NSAUX]
-    // neighbour t of the cell (offsets -1, 0, +1 per dimension; a 3-wide
-    // stencil implies at least 2 cells in that dimension, so one wrap suffices)
-    auto wrap1i = [](int x, int n) { return x < 0 ? x + n : (x >= n ? x - n : x); };
-    auto neighbour = [&](int t) -> int64_t {
-        const int ii = t % WX, jj = (t / WX) % WY, kk = t / (WX * WY);
-        const int ci = WX == 1 ? cx : wrap1i(cx + ii - 1, g.n[0]);
-        const int cj = WY == 1 ? cy : wrap1i(cy + jj - 1, g.n[1]);
-        const int ck = WZ == 1 ? cz : wrap1i(cz + kk - 1, g.n[2]);
-        return ((int64_t)ck * g.n[1] + cj) * g.n[0] + ci;
-    };
     if constexpr (s != 0) {
-        // stage the neighbourhood's fields: NT rows of QG values, one row per neighbour
+        // stage the box's fields, all waves together: NB rows of QG values
         const T* gsrc;
         if constexpr (sizeof(T) == 8) gsrc = (const T*)a.gin;
         else gsrc = (const T*)a.gin32;
+        // a compile-time trip count, unrolled, so that every load is in flight at once
+        constexpr int STRIDE = PM ? 64 * WPB : 64;
+        const int tid = PM ? (int)threadIdx.x : lane;
 #pragma unroll
-        for (int e0 = 0; e0 < NT * QG; e0 += 64) {
-            const int e = e0 + lane;
-            if (e < NT * QG) {
-                const int t = e / QG, q = e - t * QG;
-                Gs[e] = gsrc[neighbour(t) * QG + q];
+        for (int e0 = 0; e0 < NB * QG; e0 += STRIDE) {
+            const int e = e0 + tid;
+            if (e < NB * QG) {
+                const int r = e / QG, q = e - r * QG;
+                Gs[e] = gsrc[box_cell(r) * QG + q];
             }
+        }
+        if constexpr (PM) {
+            __syncthreads();
+        } else {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
     }
 
@@ -360,7 +430,7 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
                         const int ii = t % WX, jj = (t / WX) % WY, kk = t / (WX * WY);
                         const double w = w3[WX == 1 ? 1 : ii] * (w3[3 + (WY == 1 ? 1 : jj)] * w3[6 + (WZ == 1 ? 1 : kk)]);
                         const T A = T(w) * sg;
-                        const T B = c16 < nh ? Gs[t * QG + g4 * nh + c16] : T(0);
+                        const T B = c16 < nh ? Gs[box_of(t) * QG + g4 * nh + c16] : T(0);
                         accg = MF::step(A, B, accg);
                     }
                     if (c16 < nh) {
@@ -517,23 +587,57 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
     if (!deposits) return;
-    // ---- flush: for the f64 16x16x4 product, element r of lane l holds
-    // D[16m + 4r + l/16][16n + l%16] (rows interleaved across the four
-    // 16-lane groups, unlike the f32 form's 4(l/16) + r)
+    // For the f64 16x16x4 product, element r of lane l holds
+    // D[16m + 4r + l/16][16n + l%16] (rows interleaved across the four 16-lane
+    // groups, unlike the f32 form's 4(l/16) + r).
+    if constexpr (!PM) {
+        // ---- flush this wave's deposit straight to the grid
+        if (end <= beg) return;
 #pragma unroll
-    for (int m = 0; m < MT; m++)
+        for (int m = 0; m < MT; m++)
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int t = 16 * m + 4 * r + (lane >> 4);
-            if (t >= NT) continue;
-            const int64_t tc = neighbour(t);
+            for (int r = 0; r < 4; r++) {
+                const int t = 16 * m + 4 * r + (lane >> 4);
+                if (t >= NT) continue;
+                const int64_t tc = box_cell(box_of(t));
 #pragma unroll
-            for (int n = 0; n < NTL; n++) {
-                const int q = 16 * n + (lane & 15);
-                if (q < QG) unsafeAtomicAdd(&a.gout[tc * QG + q], acc[m][n][r]);
-                else if (aux && q < QGA) unsafeAtomicAdd(&a.aux[tc * NAUX + (q - QG)], acc[m][n][r]);
+                for (int n = 0; n < NTL; n++) {
+                    const int q = 16 * n + (lane & 15);
+                    if (q < QG) unsafeAtomicAdd(&a.gout[tc * QG + q], acc[m][n][r]);
+                    else if (aux && q < QGA) unsafeAtomicAdd(&a.aux[tc * NAUX + (q - QG)], acc[m][n][r]);
+                }
             }
-        }
+        return;
+    }
+    // ---- merge the four waves' deposits over the box in LDS, then flush
+    double* Mb = lds; // [NB][QGA], over the staged box and the waves' regions
+    __syncthreads();
+    for (int e = threadIdx.x; e < NB * QGA; e += 64 * WPB) Mb[e] = 0.0;
+    __syncthreads();
+    if (end > beg) {
+#pragma unroll
+        for (int m = 0; m < MT; m++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int t = 16 * m + 4 * r + (lane >> 4);
+                if (t >= NT) continue;
+                const int bi = box_of(t);
+#pragma unroll
+                for (int n = 0; n < NTL; n++) {
+                    const int q = 16 * n + (lane & 15);
+                    if (q < QGA) unsafeAtomicAdd(&Mb[bi * QGA + q], acc[m][n][r]);
+                }
+            }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < NB * QGA; e += 64 * WPB) {
+        const double v = Mb[e];
+        if (v == 0.0) continue;
+        const int bi = e / QGA, q = e - bi * QGA;
+        const int64_t tc = box_cell(bi);
+        if (q < QG) unsafeAtomicAdd(&a.gout[tc * QG + q], v);
+        else unsafeAtomicAdd(&a.aux[tc * NAUX + (q - QG)], v);
+    }
 }
 
 } // namespace emu

@@ -523,34 +523,39 @@ int scan_counts(emu_state* st, int* out)
     return EMU_OK;
 }
 
-constexpr int kWPB = 2; // waves (cells in flight) per block of the stage kernel
 
 template <int NF, typename T, int WX, int WY, int WZ, int S, bool AUX>
-void launch_stage_k(emu_state* st, StageArgs<T>& a, dim3 grid, dim3 block)
+void launch_stage_k(emu_state* st, StageArgs<T>& a)
 {
-    const size_t lds = (size_t)kWPB * stage_lds_doubles_per_wave<NF, T, WX, WY, WZ, S, AUX>() * sizeof(double);
-    hipLaunchKernelGGL((k_stage<NF, T, WX, WY, WZ, kWPB, S, AUX>), grid, block, lds, st->ctx->stream, a);
+    constexpr bool PM = stage_patch_mode<T, S, AUX>();
+    constexpr int WPB = PM ? 4 : 2; // waves per workgroup
+    using PT = Patch<WX, WY, WZ, WPB, PM>;
+    const Geom& g = st->ctx->g;
+    const int64_t npatch = (int64_t)((g.n[0] + PT::PX - 1) / PT::PX) * ((g.n[1] + PT::PY - 1) / PT::PY) *
+                           ((g.n[2] + PT::PZ - 1) / PT::PZ);
+    const int64_t nblk = PM ? npatch : (npatch + WPB - 1) / WPB;
+    a.blocks_per_xcd = (int)((nblk + 7) / 8);
+    const size_t lds = (size_t)stage_lds_doubles_per_block<NF, T, WX, WY, WZ, WPB, S, AUX, PM>() * sizeof(double);
+    hipLaunchKernelGGL((k_stage<NF, T, WX, WY, WZ, WPB, S, AUX, PM>), dim3((unsigned)(8 * a.blocks_per_xcd)),
+                       dim3(64 * WPB), lds, st->ctx->stream, a);
 }
 
 template <int NF, typename T, int WX, int WY, int WZ>
 void launch_stage_t(emu_state* st, StageArgs<T>& a)
 {
     const int aux = a.want_aux && (a.stage == 3 || a.stage == 0);
-    const int64_t nblk = (st->ncell + kWPB - 1) / kWPB;
-    a.blocks_per_xcd = (int)((nblk + 7) / 8);
-    const dim3 grid((unsigned)(8 * a.blocks_per_xcd)), block(64 * kWPB);
     switch (a.stage) {
     case 0:
-        if (aux) launch_stage_k<NF, T, WX, WY, WZ, 0, true>(st, a, grid, block);
-        else launch_stage_k<NF, T, WX, WY, WZ, 0, false>(st, a, grid, block);
+        if (aux) launch_stage_k<NF, T, WX, WY, WZ, 0, true>(st, a);
+        else launch_stage_k<NF, T, WX, WY, WZ, 0, false>(st, a);
         break;
-    case 1: launch_stage_k<NF, T, WX, WY, WZ, 1, false>(st, a, grid, block); break;
-    case 2: launch_stage_k<NF, T, WX, WY, WZ, 2, false>(st, a, grid, block); break;
+    case 1: launch_stage_k<NF, T, WX, WY, WZ, 1, false>(st, a); break;
+    case 2: launch_stage_k<NF, T, WX, WY, WZ, 2, false>(st, a); break;
     case 3:
-        if (aux) launch_stage_k<NF, T, WX, WY, WZ, 3, true>(st, a, grid, block);
-        else launch_stage_k<NF, T, WX, WY, WZ, 3, false>(st, a, grid, block);
+        if (aux) launch_stage_k<NF, T, WX, WY, WZ, 3, true>(st, a);
+        else launch_stage_k<NF, T, WX, WY, WZ, 3, false>(st, a);
         break;
-    default: launch_stage_k<NF, T, WX, WY, WZ, 4, false>(st, a, grid, block); break;
+    default: launch_stage_k<NF, T, WX, WY, WZ, 4, false>(st, a); break;
     }
 }
 

@@ -230,8 +230,11 @@ def test_integrator_tableaus_parity(core, oracle, itype, rk):
 
 
 @pytest.mark.parametrize("nf", [2, 3])
-@pytest.mark.parametrize("ncell", [(4, 4, 4), (1, 1, 8)])
+@pytest.mark.parametrize("ncell", [(4, 4, 4), (1, 1, 8), (3, 5, 2), (2, 3, 7), (1, 1, 5)])
 def test_fused_fp64_parity(core, oracle, nf, ncell):
+    """Cells are taken four to a workgroup (2 x 2 in x, y, or 4 along z): odd
+    counts leave partial patches, 2-cell dimensions make a patch's
+    neighbourhood hold periodic images twice."""
     d = cases.fast_flavor_dict(nf=nf, ncell=ncell, nphi=8)
     deck, prm, emu = make(core, oracle, d)
     parts = oracle.init_particles(deck)
