@@ -220,10 +220,13 @@ constexpr int stage_lds_doubles_per_block()
     return work > merge ? work : merge;
 }
 
-// the launches that take the patch mode
+// the launches that take the patch mode (measured per launch kind at
+// 64^3 x 92, NF = 3: the deposit-only launch in both precisions, FP32-bulk
+// stage 4, the FP64 auxiliary stage; the other RK stages run faster as
+// independent waves)
 template <typename T, int S, bool AUX> constexpr bool stage_patch_mode()
 {
-    return S == 0 || (AUX && sizeof(T) == 8);
+    return S == 0 || (S == 4 && sizeof(T) == 4) || (AUX && sizeof(T) == 8);
 }
 
 // S: 0 = deposit only (the step's result, or the initial state), 1..4 = RK
