@@ -450,9 +450,18 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
             // ---- the RK registers this stage reads, as one batch of loads whose
             // latency the Hamiltonian and commutator arithmetic covers
             T yn[2 * nh], ao[2 * nh];
+            // The accumulator after stage 1, y_n + dt/6 k1, is not stored: with
+            // Y2 = y_n + dt/2 k1 it equals y_n + (Y2 - y_n)/3, which stage 2
+            // rebuilds from the two arrays it reads anyway (one write and one
+            // read of the flavour state saved per step; rounding-level change).
             if constexpr (s == 1) {
 #pragma unroll
                 for (int c = 0; c < 2 * nh; c++) yn[c] = ao[c] = Y[c];
+            } else if constexpr (s == 2) {
+#pragma unroll
+                for (int c = 0; c < 2 * nh; c++) yn[c] = a.cur.f[(int64_t)c * a.np + ipl];
+#pragma unroll
+                for (int c = 0; c < 2 * nh; c++) ao[c] = yn[c] + (Y[c] - yn[c]) / T(3);
             } else {
 #pragma unroll
                 for (int c = 0; c < 2 * nh; c++) yn[c] = s < 4 ? a.cur.f[(int64_t)c * a.np + ipl] : T(0);
@@ -483,7 +492,7 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
                 if (act) {
 #pragma unroll
                     for (int c = 0; c < 2 * nh; c++) {
-                        a.acc[(int64_t)c * a.np + ip] = ao[c];
+                        if (s != 1) a.acc[(int64_t)c * a.np + ip] = ao[c];
                         a.Y[(int64_t)c * a.np + ip] = Dep[c];
                     }
                 }
