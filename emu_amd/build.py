@@ -93,7 +93,7 @@ def build_cpp(verbose: bool = False):
     cxx = os.environ.get("CXX", "g++")
     os.makedirs(os.path.dirname(OUT_MAIN), exist_ok=True)
     objs = []
-    for src in ("Emu.cpp", "emu_cpp.cpp"):
+    for src in ("Emu.cpp", "PlotFile.cpp", "emu_cpp.cpp"):
         obj = os.path.join(BUILD, src.replace(".cpp", ".o"))
         _run([cxx, *CXXFLAGS, "-c", os.path.join(CPP, src), "-o", obj], verbose)
         objs.append(obj)

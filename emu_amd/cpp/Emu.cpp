@@ -435,6 +435,20 @@ void FlavoredNeutrinoContainer::apply_rhs(double a, const FlavoredNeutrinoContai
     emu_check(emu_apply_rhs(geom_.ctx(), &view_, a, &dFdt.view_), "ApplyFlavoredNeutrinoRHS", geom_.ctx());
 }
 
+void FlavoredNeutrinoContainer::from_host(int64_t np, const std::vector<double>& pos, const std::vector<double>& rdata,
+                                          const std::vector<int64_t>& ids)
+{
+    const int na = geom_.nattribs();
+    if ((int64_t)pos.size() != 3 * np || (int64_t)rdata.size() != na * np || (int64_t)ids.size() != np)
+        Abort("FlavoredNeutrinoContainer::from_host: array sizes");
+    resize(np);
+    if (np == 0) return;
+    hip_check(hipMemcpy(block_, pos.data(), pos.size() * sizeof(double), hipMemcpyHostToDevice), "from_host pos");
+    hip_check(hipMemcpy(block_ + (size_t)3 * np, rdata.data(), rdata.size() * sizeof(double), hipMemcpyHostToDevice),
+              "from_host rdata");
+    hip_check(hipMemcpy(ids_, ids.data(), ids.size() * sizeof(int64_t), hipMemcpyHostToDevice), "from_host ids");
+}
+
 void FlavoredNeutrinoContainer::to_host(std::vector<double>& pos, std::vector<double>& rdata,
                                         std::vector<int64_t>& ids) const
 {
@@ -558,15 +572,17 @@ void TimeIntegrator::advance(double dt)
     new_->copyParticles(*old_, true);
     for (int i = 0; i < tab_.nstages; i++) new_->apply_rhs(dt * tab_.b[i], *F_[i]);
     time_ += dt;
-    step_ += 1;
 }
 
 void TimeIntegrator::integrate(double start_dt, double end_time, int nsteps)
 {
+    // as AMReX's loop: the post-timestep hook sees the number of the step just
+    // taken counted from the start step (0 for the first step of a new run)
     dt_ = start_dt;
     for (int n = 0; n < nsteps && time_ < end_time; n++) {
         advance(dt_);
         if (post_) post_();
+        step_ += 1;
         std::swap(old_, new_);
     }
 }
@@ -612,22 +628,40 @@ void write_dump(const std::string& path, const FlavoredNeutrinoContainer& c, dou
 // ---------------------------------------------------------------------------
 EvolveResult evolve_flavor(const TestParams* parms, const EvolveOptions& opt)
 {
-    if (parms->do_restart) Abort("restart from a checkpoint is not supported", EMU_ERR_UNSUPPORTED);
     Geometry geom(*parms);
     MultiFab state(geom);
     FlavoredNeutrinoContainer neutrinos_old(geom);
     FlavoredNeutrinoContainer neutrinos_new(geom);
     EvolveResult res;
     if (!opt.quiet) std::cout << "Initializing particles... ";
-    neutrinos_old.InitParticles(parms);
+    double initial_time = 0.0;
+    int initial_step = 0;
+    if (parms->do_restart) {
+        RecoverParticles(parms->restart_dir, neutrinos_old, initial_time, initial_step);
+        // the reference sets Vvac_max only in InitParticles, so a restarted run
+        // (a fresh process there) uses its zero initial value
+        FlavoredNeutrinoContainer::Vvac_max = 0.0;
+        if (!opt.quiet)
+            std::cout << "Restarting after time step: " << initial_step - 1 << " t = " << initial_time
+                      << " s.  ct = " << kC * initial_time << " cm" << std::endl;
+    } else {
+        neutrinos_old.InitParticles(parms);
+    }
     neutrinos_new.copyParticles(neutrinos_old, true);
     deposit_to_mesh(neutrinos_old, state, geom);
+    if (opt.write_plotfiles && !parms->do_restart) WritePlotFile(state, neutrinos_old, geom, initial_time, initial_step, 1);
     if (!opt.quiet) std::cout << "Done. " << std::endl;
     res.np = neutrinos_old.TotalNumberOfParticles();
     double run_fom = 0.0;
+    // main.cpp:181-191 (a write_plot_every of 0 writes no grid-only plotfiles)
+    auto plot_due = [&](int next, int& with_particles) {
+        with_particles = parms->write_plot_particles_every > 0 && next % parms->write_plot_particles_every == 0;
+        return opt.write_plotfiles &&
+               ((parms->write_plot_every > 0 && next % parms->write_plot_every == 0) || with_particles);
+    };
 
     if (opt.path == "plain") {
-        TimeIntegrator integrator(neutrinos_old, neutrinos_new, 0.0, 0,
+        TimeIntegrator integrator(neutrinos_old, neutrinos_new, initial_time, initial_step,
                                   Tableau::preset(parms->integration_type, parms->integration_rk_type));
         auto source_fun = [&](FlavoredNeutrinoContainer& neutrinos_rhs, const FlavoredNeutrinoContainer& neutrinos,
                               double /*time*/) {
@@ -648,6 +682,8 @@ EvolveResult evolve_flavor(const TestParams* parms, const EvolveOptions& opt)
                 std::cout << "Completed time step: " << step << " t = " << time << " s.  ct = " << kC * time << " cm"
                           << std::endl;
             run_fom += (double)neutrinos.TotalNumberOfParticles();
+            int wpp = 0;
+            if (plot_due(step + 1, wpp)) WritePlotFile(state, neutrinos, geom, time, step + 1, wpp);
             const double dt = compute_dt(geom, parms->cfl_factor, state, neutrinos, parms->flavor_cfl_factor,
                                          parms->max_adaptive_speedup);
             integrator.set_timestep(dt);
@@ -663,7 +699,7 @@ EvolveResult evolve_flavor(const TestParams* parms, const EvolveOptions& opt)
         integrator.integrate(starting_dt, parms->end_time, parms->nsteps);
         geom.sync();
         res.run_time_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        res.steps = integrator.get_step_number();
+        res.steps = integrator.get_step_number() - initial_step;
         res.time = integrator.get_time();
         if (!opt.dump_path.empty()) write_dump(opt.dump_path, integrator.get_old_data(), res.time);
     } else if (opt.path == "fused-fp64" || opt.path == "fused-fp32") {
@@ -683,14 +719,25 @@ EvolveResult evolve_flavor(const TestParams* parms, const EvolveOptions& opt)
         int64_t steps = 0, bad = -1;
         geom.sync();
         const auto t0 = std::chrono::steady_clock::now();
+        time = initial_time;
         for (int n = 0; n < parms->nsteps && time < parms->end_time; n++) {
             emu_check(emu_state_evolve(st, 1, 0.0), "emu_state_evolve", geom.ctx());
             const int code = emu_state_info(st, &time, &dt, &steps, &bad);
             if (code != EMU_OK) emu_check(code, "Renormalize", geom.ctx(), bad);
             run_fom += (double)res.np;
+            const int step = initial_step + (int)steps - 1;
             if (!opt.quiet)
-                std::cout << "Completed time step: " << steps << " t = " << time << " s.  ct = " << kC * time << " cm"
+                std::cout << "Completed time step: " << step << " t = " << time << " s.  ct = " << kC * time << " cm"
                           << std::endl;
+            int wpp = 0;
+            if (plot_due(step + 1, wpp)) {
+                // the fused state's grid holds combined fields: deposit the step's
+                // result in the reference's layout for the plotfile
+                emu_check(emu_state_store(st, neutrinos_new.view(), neutrinos_new.ids()), "emu_state_store",
+                          geom.ctx());
+                deposit_to_mesh(neutrinos_new, state, geom);
+                WritePlotFile(state, neutrinos_new, geom, time, step + 1, wpp);
+            }
         }
         geom.sync();
         res.run_time_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

@@ -177,8 +177,10 @@ public:
 
     // ApplyFlavoredNeutrinoRHS over the container (FlavoredNeutrinoContainer.H:33-47)
     void apply_rhs(double a, const FlavoredNeutrinoContainer& dFdt);
-    // host copies (SoA, particle order), for output
+    // host copies (SoA, particle order), for output and restart
     void to_host(std::vector<double>& pos, std::vector<double>& rdata, std::vector<int64_t>& ids) const;
+    void from_host(int64_t np, const std::vector<double>& pos, const std::vector<double>& rdata,
+                   const std::vector<int64_t>& ids);
     void resize(int64_t np);
 
 private:
@@ -240,15 +242,44 @@ private:
 };
 
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// IO.cpp: plotfiles in the AMReX layout (the valid cells of `state` as a
+// single-level plotfile, the particles as an AMReX particle checkpoint named
+// "neutrinos") and the restart from one.  PlotFile.cpp holds the format.
+// ---------------------------------------------------------------------------
+struct PlotFileSpec {
+    int nf;
+    int ncell[3];
+    double plo[3], phi[3];
+    double time;
+    int step;
+};
+std::vector<std::string> grid_component_names(int nf);      // GIdx::names
+std::vector<std::string> particle_attribute_names(int nf);  // PIdx names
+void write_plotfile_grid(const std::string& dir, const PlotFileSpec& sp, const double* valid);
+void write_plotfile_particles(const std::string& dir, const std::string& name, const PlotFileSpec& sp, int64_t np,
+                              const int64_t* ids, const double* pos, const double* rdata);
+void read_plotfile_header(const std::string& dir, double& time, int& step);
+int64_t read_plotfile_particles(const std::string& dir, const std::string& name, int nf, std::vector<int64_t>& ids,
+                                std::vector<double>& pos, std::vector<double>& rdata);
+void WritePlotFile(const MultiFab& state, const FlavoredNeutrinoContainer& neutrinos, const Geometry& geom,
+                   double time, int step, int write_plot_particles); // IO.cpp:11-36
+void RecoverParticles(const std::string& dir, FlavoredNeutrinoContainer& neutrinos, double& time,
+                      int& step);                                   // IO.cpp:38-63
+
 // evolve_flavor (main.cpp:37-227).  path = "plain": the reference's loop on
 // the per-function calls above; "fused-fp64" / "fused-fp32": the same loop as
-// whole RK4 steps of the cell-sorted state (emu_state_*).  Plotfiles are not
-// written (out of scope); dump_path, if set, receives the final particles.
+// whole RK4 steps of the cell-sorted state (emu_state_*).  Plotfiles follow
+// write_plot_every / write_plot_particles_every as main.cpp:181-191 (the
+// fused path deposits the step's result for their grid); do_restart reads
+// the particles of restart_dir.  dump_path, if set, receives the final
+// particles.
 // ---------------------------------------------------------------------------
 struct EvolveOptions {
     std::string path = "plain";
     bool quiet = false;
     std::string dump_path;
+    bool write_plotfiles = true; // plt<step> directories in the working directory, as main.cpp
 };
 
 struct EvolveResult {

@@ -48,7 +48,7 @@ extern "C" int emu_cpp_read_params(const char* inputs_file, const char* const* o
 }
 
 extern "C" int emu_cpp_evolve_flavor(const char* inputs_file, const char* const* overrides, int noverrides,
-                                     const char* path, const char* dump_path, int quiet, double out[6], char* errbuf,
+                                     const char* path, const char* dump_path, int flags, double out[6], char* errbuf,
                                      int errlen)
 {
     if (!inputs_file) return EMU_ERR_ARG;
@@ -59,7 +59,8 @@ extern "C" int emu_cpp_evolve_flavor(const char* inputs_file, const char* const*
         emu::EvolveOptions opt;
         if (path) opt.path = path;
         if (dump_path) opt.dump_path = dump_path;
-        opt.quiet = quiet != 0;
+        opt.quiet = (flags & 1) != 0;
+        opt.write_plotfiles = (flags & 2) != 0;
         const emu::EvolveResult r = emu::evolve_flavor(&p, opt);
         if (out) {
             out[0] = r.steps;
@@ -68,6 +69,66 @@ extern "C" int emu_cpp_evolve_flavor(const char* inputs_file, const char* const*
             out[3] = r.run_time_s;
             out[4] = r.fom;
             out[5] = (double)r.np;
+        }
+        put(errbuf, errlen, "");
+        return EMU_OK;
+    } catch (const emu::Error& e) {
+        put(errbuf, errlen, e.what());
+        return e.code ? e.code : EMU_ERR_ARG;
+    } catch (const std::exception& e) {
+        put(errbuf, errlen, e.what());
+        return EMU_ERR_ARG;
+    }
+}
+
+extern "C" int emu_cpp_write_plotfile(const char* dir, int nf, const int ncell[3], const double plo[3],
+                                      const double phi[3], double time, int step, const double* valid_grid,
+                                      int64_t np, const int64_t* ids, const double* pos, const double* rdata,
+                                      char* errbuf, int errlen)
+{
+    if (!dir || !ncell || !plo || !phi || !valid_grid || (nf != 2 && nf != 3) || np < 0 ||
+        (np > 0 && (!ids || !pos || !rdata)))
+        return EMU_ERR_ARG;
+    emu::set_throw_on_error(true);
+    try {
+        emu::PlotFileSpec sp{};
+        sp.nf = nf;
+        for (int d = 0; d < 3; d++) {
+            sp.ncell[d] = ncell[d];
+            sp.plo[d] = plo[d];
+            sp.phi[d] = phi[d];
+        }
+        sp.time = time;
+        sp.step = step;
+        emu::write_plotfile_grid(dir, sp, valid_grid);
+        if (np > 0) emu::write_plotfile_particles(dir, "neutrinos", sp, np, ids, pos, rdata);
+        put(errbuf, errlen, "");
+        return EMU_OK;
+    } catch (const emu::Error& e) {
+        put(errbuf, errlen, e.what());
+        return e.code ? e.code : EMU_ERR_ARG;
+    } catch (const std::exception& e) {
+        put(errbuf, errlen, e.what());
+        return EMU_ERR_ARG;
+    }
+}
+
+extern "C" int emu_cpp_read_plotfile(const char* dir, int nf, double* time, int* step, int64_t* np, int64_t capacity,
+                                     int64_t* ids, double* pos, double* rdata, char* errbuf, int errlen)
+{
+    if (!dir || !time || !step || !np || (nf != 2 && nf != 3)) return EMU_ERR_ARG;
+    emu::set_throw_on_error(true);
+    try {
+        emu::read_plotfile_header(dir, *time, *step);
+        std::vector<int64_t> vi;
+        std::vector<double> vp, vr;
+        const int64_t n = emu::read_plotfile_particles(dir, "neutrinos", nf, vi, vp, vr);
+        *np = n;
+        if (capacity >= n && n > 0) {
+            if (!ids || !pos || !rdata) return EMU_ERR_ARG;
+            std::memcpy(ids, vi.data(), vi.size() * sizeof(int64_t));
+            std::memcpy(pos, vp.data(), vp.size() * sizeof(double));
+            std::memcpy(rdata, vr.data(), vr.size() * sizeof(double));
         }
         put(errbuf, errlen, "");
         return EMU_OK;

@@ -21,10 +21,25 @@ int emu_cpp_read_params(const char* inputs_file, const char* const* overrides, i
 
 /* evolve_flavor (main.cpp:37-227) through the C++ layer.  path: "plain",
  * "fused-fp64" or "fused-fp32"; dump_path (may be NULL): final particles in
- * the EMUDUMP1 format of Emu.hpp.  out[6] = steps, time, last dt, run time
- * (s), particles advanced per microsecond, particle count. */
+ * the EMUDUMP1 format of Emu.hpp; flags: 1 = quiet, 2 = write the plotfiles
+ * (plt<step> in the working directory).  out[6] = steps taken, time, last dt,
+ * run time (s), particles advanced per microsecond, particle count. */
 int emu_cpp_evolve_flavor(const char* inputs_file, const char* const* overrides, int noverrides, const char* path,
-                          const char* dump_path, int quiet, double out[6], char* errbuf, int errlen);
+                          const char* dump_path, int flags, double out[6], char* errbuf, int errlen);
+
+/* WritePlotFile's format from host arrays (IO.cpp:11-36): `dir` receives the
+ * valid cells valid_grid[GIdx::ncomp][nz][ny][nx] as a single-level AMReX
+ * plotfile and, if np > 0, the particles (ids[np], pos[3][np],
+ * rdata[PIdx::nattribs][np]) as the "neutrinos" particle checkpoint. */
+int emu_cpp_write_plotfile(const char* dir, int nf, const int ncell[3], const double plo[3], const double phi[3],
+                           double time, int step, const double* valid_grid, int64_t np, const int64_t* ids,
+                           const double* pos, const double* rdata, char* errbuf, int errlen);
+
+/* The time, step and "neutrinos" particles of a plotfile into host arrays
+ * (RecoverParticles, IO.cpp:38-63).  *np receives the count; the arrays are
+ * filled only when capacity >= *np. */
+int emu_cpp_read_plotfile(const char* dir, int nf, double* time, int* step, int64_t* np, int64_t capacity,
+                          int64_t* ids, double* pos, double* rdata, char* errbuf, int errlen);
 
 #ifdef __cplusplus
 }

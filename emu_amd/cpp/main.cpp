@@ -1,7 +1,7 @@
 // emu_main -- the reference's executable (main.cpp:229-262) on this build:
 //
 //   emu_main <inputs_file> [key=value ...] [--path plain|fused-fp64|fused-fp32]
-//            [--dump <file>] [--quiet] [--print-params]
+//            [--dump <file>] [--quiet] [--no-plotfiles] [--print-params]
 //
 // reads the deck like TestParams::Initialize, runs evolve_flavor and prints
 // the figure of merit (particles advanced per microsecond).
@@ -17,7 +17,7 @@ int main(int argc, char* argv[])
     if (argc < 2) {
         std::fprintf(stderr,
                      "usage: %s <inputs_file> [key=value ...] [--path plain|fused-fp64|fused-fp32] [--dump file] "
-                     "[--quiet] [--print-params]\n",
+                     "[--quiet] [--no-plotfiles] [--print-params]\n",
                      argv[0]);
         return 2;
     }
@@ -29,6 +29,7 @@ int main(int argc, char* argv[])
         if (a == "--path" && i + 1 < argc) opt.path = argv[++i];
         else if (a == "--dump" && i + 1 < argc) opt.dump_path = argv[++i];
         else if (a == "--quiet") opt.quiet = true;
+        else if (a == "--no-plotfiles") opt.write_plotfiles = false;
         else if (a == "--print-params") print_params = true;
         else if (a.find('=') != std::string::npos) overrides.push_back(a);
         else {
