@@ -275,6 +275,62 @@ def test_fused_fp64_computed_dt(core, oracle, nf):
     assert cases.max_rel_err(got["rdata"], want["rdata"]) < 1e-11
 
 
+@pytest.mark.parametrize("speedup", [4.0, 1e9])
+@pytest.mark.parametrize("path", ["integrator", "fp64", "fp32"])
+@pytest.mark.parametrize("nf", [2, 3])
+def test_adaptive_dt_parity(core, oracle, nf, path, speedup):
+    """max_adaptive_speedup > 1 (Evolve.cpp:68-74): dt = min(speedup dt_stupid,
+    dt_adaptive).  2 cm cells keep the translation bound (cfl_factor 0.9, as
+    the fused sort needs c dt < dx) above both, and the antineutrino flux makes
+    |F - Fbar| the adaptive bound: a speedup of 4 is capped by dt_stupid, 1e9
+    by dt_adaptive.  Every computed dt of 3 steps matches."""
+    ncell = (4, 3, 5)
+    d = cases.fast_flavor_dict(nf=nf, ncell=ncell, L=tuple(2.0 * n for n in ncell), nphi=4, st4_amplitude=1e-3,
+                               cfl_factor=0.9, max_adaptive_speedup=speedup)
+    deck, prm, emu = make(core, oracle, d)
+    parts = oracle.init_particles(deck)
+    c = core.FlavoredNeutrinoContainer.from_aos(emu, parts)
+    probe = oracle.Deck(**{**deck.__dict__, "max_adaptive_speedup": 0.0}).params()
+    g = oracle.Grid(nf, prm)
+    oracle.deposit_to_mesh(nf, prm, parts, g)
+    vv = oracle.vvac_max(nf, prm, parts)
+    _, dt_stupid = oracle.compute_dt(nf, probe, g, vv)
+    dts, dt_next = oracle_steps(oracle, nf, prm, parts, 3)
+    if speedup == 4.0:
+        assert dts[0] == pytest.approx(4.0 * dt_stupid, rel=1e-13)
+    else:
+        assert dts[0] > 4.0 * dt_stupid and dts[0] < 0.9 * 2.0 / oracle.clight
+    ftol = 1e-13 if path != "fp32" else 1e-5
+    if path == "integrator":
+        integ = core.Integrator(emu, c)
+        assert integ.dt == pytest.approx(dts[0], rel=1e-14)
+        out = integ.integrate(3)
+        assert integ.dt == pytest.approx(dt_next, rel=ftol)
+        got = cases.by_id(out.to_aos())
+    else:
+        st = core.FusedState(emu, c.np, path)
+        st.load(c)
+        assert st.info()["dt"] == pytest.approx(dts[0], rel=1e-14 if path == "fp64" else 1e-6)
+        st.evolve(3)
+        st.store(c)
+        assert st.info()["dt"] == pytest.approx(dt_next, rel=ftol)
+        st.close()
+        got = cases.by_id(c.to_aos())
+    want = cases.by_id(parts)
+    if path != "fp32":
+        assert cases.max_rel_err(got["rdata"], want["rdata"]) < 1e-11
+        return
+    # At the adaptive step (31x the stupid one) the oracle itself moves 5-8% of the
+    # largest |f11| (~1e-5) when its input is perturbed by 6e-8 relative, so
+    # FP32-bulk is held to the FP32 bar against the trace Tr f = 1 of each
+    # density matrix, and per attribute elsewhere.
+    K = emu.K
+    fcols = [K[n] for n in K if n.startswith("f")]
+    other = [i for i in range(want["rdata"].shape[1]) if i not in fcols]
+    assert cases.max_rel_err(got["rdata"][:, other], want["rdata"][:, other]) < 2e-5
+    assert np.abs(got["rdata"][:, fcols] - want["rdata"][:, fcols]).max() < 2e-5
+
+
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-11), ("fp32", 2e-5)])
 @pytest.mark.parametrize("nf", [2, 3])
 def test_fused_full_chunks(core, oracle, nf, prec, tol):

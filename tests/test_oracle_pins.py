@@ -345,6 +345,69 @@ def test_compute_dt_uniform_closed_form(oracle):
     assert dt == pytest.approx(min(dt_fl, dt_tr), rel=1e-12)
 
 
+def _herm(vals, names, tmpl, nf):
+    """Complex NF x NF Hermitian matrix from its stored real components."""
+    X = np.zeros((nf, nf), dtype=complex)
+    for i in range(nf):
+        for j in range(i, nf):
+            re = vals[names.index(tmpl.format(i, j, "Re"))]
+            im = vals[names.index(tmpl.format(i, j, "Im"))] if j > i else 0.0
+            X[i, j] = re + 1j * im
+            X[j, i] = re - 1j * im
+    return X
+
+
+def _su2(X):
+    """|X|^2 in the SU(N) generator basis = Tr(X0^2)/2, X0 the traceless part."""
+    X0 = X - np.trace(X) / X.shape[0] * np.eye(X.shape[0])
+    return 0.5 * np.trace(X0 @ X0).real
+
+
+@pytest.mark.parametrize("nf", [2, 3])
+@pytest.mark.parametrize("speedup", [0.0, 1.0, 4.0, 1e9])
+def test_compute_dt_adaptive_closed_form(oracle, nf, speedup):
+    """compute_dt's two flavor bounds from matrix algebra (Evolve.cpp:36-74,
+    generate_code.py:258-294): V_adaptive = |N - conj(Nbar) + matter|_SU and
+    |F_d - Fbar_d|_SU (no conjugate on the flux), times sqrt2 GF / dV;
+    V_stupid = 2 NF sqrt2 GF / dV times the largest diagonal or matter term;
+    max_adaptive_speedup > 1 picks min(speedup dt_stupid, dt_adaptive).  Every
+    cell of the fab, ghosts included, is reduced, so one cell carries the
+    largest values."""
+    d = oracle.Deck(nf=nf, ncell=(3, 2, 2), Lx=1.5, Ly=2.0, Lz=1.0, cfl_factor=-1.0, flavor_cfl_factor=0.3,
+                    max_adaptive_speedup=speedup)
+    p = d.params()
+    g = oracle.Grid(nf, p)
+    names = oracle.grid_names(nf)
+    rng = np.random.default_rng(7 + nf)
+    fab = g.fab()
+    vals = rng.uniform(-1.0, 1.0, size=len(names)) * 1e30
+    for t in ("", "bar"):
+        for i in range(nf):
+            vals[names.index("N{0}{0}_Re{1}".format(i, t))] = abs(vals[names.index("N{0}{0}_Re{1}".format(i, t))]) * 3
+    vals[names.index("rho")], vals[names.index("T")], vals[names.index("Ye")] = 2e6, 10.0, 0.4
+    fab[:] = 0.25 * vals[:, None, None, None]
+    fab[:, 0, 1, 2] = vals  # a ghost cell (ng >= 1 with shape order 2)
+    dV = (1.5 / 3) * (2.0 / 2) * (1.0 / 2)
+    matter = 2e6 * 0.4 / oracle.Mp * dV
+    HSI = _herm(vals, names, "N{}{}_{}", nf) - _herm(vals, names, "N{}{}_{}bar", nf).conj()
+    HSI[0, 0] += matter
+    V2 = _su2(HSI)
+    for dd in "xyz":
+        V2 += _su2(_herm(vals, names, "F" + dd + "{}{}_{}", nf) - _herm(vals, names, "F" + dd + "{}{}_{}bar", nf))
+    Vad = math.sqrt(V2) * math.sqrt(2) * oracle.GF / dV
+    diag = [vals[names.index("N{0}{0}_Re{1}".format(i, t))] for i in range(nf) for t in ("", "bar")]
+    Vst = max(max(diag), matter) * 2 * nf * math.sqrt(2) * oracle.GF / dV
+    vv = 3e-19
+    dt_ad = oracle.hbar / (Vad + vv) * 0.3
+    dt_st = oracle.hbar / (Vst + vv) * 0.3
+    want = min(dt_st * speedup, dt_ad) if speedup > 1 else dt_st
+    err, dt = oracle.compute_dt(nf, p, g, vv)
+    assert err == 0
+    assert dt == pytest.approx(want, rel=1e-12)
+    if speedup == 1e9:
+        assert dt == pytest.approx(dt_ad, rel=1e-12) and dt_ad > 1.5 * dt_st
+
+
 def test_compute_dt_error(oracle):
     d = oracle.Deck(nf=2, cfl_factor=0.0, flavor_cfl_factor=0.0)
     p = d.params()
