@@ -175,18 +175,20 @@ __device__ __forceinline__ void end_position(const double pos[3], const double v
     newcell = (nc[2] * g.n[1] + nc[1]) * g.n[0] + nc[0];
 }
 
-// Two ways to map waves to cells.  PM (patch mode): a workgroup of WPB = 4
-// waves takes a patch of 4 cells, 2 x 2 in x, y when the stencil is 3 wide
-// there, else 4 along z; the patch's neighbourhood box (PX + WX - 1) x
-// (PY + WY - 1) x (PZ + WZ - 1) is staged once for the four gathers, and the
-// four deposits are summed over the same box in LDS before one set of global
-// atomics (56% fewer at the 3-D stencil).  Without PM every wave is its own
-// patch of one cell (box = its 27 neighbours) and flushes directly, with no
-// workgroup barrier: the barriers cost more than the merge saves in the RK
-// stages, so PM serves the deposit-heavy launches.
+// Three ways to map waves to cells (MODE).
+//  0: every wave is its own patch of one cell (box = its 27 neighbours), stages
+//     its own box and flushes its deposit directly: no workgroup barrier.
+//  1: the WPB waves of a workgroup take a patch of WPB cells (2 along x, then
+//     2 along y, when the stencil is 3 wide there, else along z); the patch's
+//     neighbourhood box (PX + WX - 1) x (PY + WY - 1) x (PZ + WZ - 1) is staged
+//     once for all of them (one barrier), and each wave flushes directly.
+//  2: as 1, and the deposits are also summed over the box in LDS before one
+//     set of global atomics (56% fewer for a 2 x 2 patch at the 3-D stencil),
+//     at the cost of two more barriers.
+// PM (a shared patch) is MODE >= 1.
 template <int WX, int WY, int WZ, int WPB, bool PM> struct Patch {
-    static constexpr int PX = PM ? (WX == 3 ? 2 : 1) : 1;
-    static constexpr int PY = PM ? (WY == 3 ? 2 : 1) : 1;
+    static constexpr int PX = PM && WX == 3 ? 2 : 1;
+    static constexpr int PY = PM && WY == 3 && WPB >= 2 * PX ? 2 : 1;
     static constexpr int PZ = PM ? WPB / (PX * PY) : 1;
     static constexpr int CELLS = PM ? WPB : 1; // cells of one patch
     static_assert(PX * PY * PZ == CELLS, "a patch holds one cell per wave");
@@ -196,44 +198,57 @@ template <int WX, int WY, int WZ, int WPB, bool PM> struct Patch {
 
 // LDS of one wave's own region, in doubles: 9 shape factors per particle
 // (double), the source row (NH + 4 values of T) and, with AUX, NF + NF(NF-1)/2
-// auxiliary sources (double).
+// auxiliary sources (T).
 template <int NF, typename T, bool AUX> constexpr int stage_lds_doubles_per_wave()
 {
     constexpr int nh = NF * NF;
-    return 64 * 9 + 64 * (nh + 4) * (int)sizeof(T) / 8 + (AUX ? 64 * (NF + NF * (NF - 1) / 2) : 0);
+    return 64 * 9 + 64 * (nh + 4) * (int)sizeof(T) / 8 + (AUX ? 64 * (NF + NF * (NF - 1) / 2) * (int)sizeof(T) / 8 : 0);
 }
 
-// LDS of a workgroup, in doubles.  PM: the staged neighbourhood box (gather
-// stages, NB x 4 NH values of T) and the waves' regions; the deposit's merge
-// buffer (NB x deposited columns, double) reuses the same space once every
-// wave is done with it.  Without PM: per wave, its own box and its region.
-// The host sizes the launch with this function.
-template <int NF, typename T, int WX, int WY, int WZ, int WPB, int S, bool AUX, bool PM>
+// LDS of the table of the neighbourhood box's cell indices (NB ints), in
+// doubles: one per workgroup (PM) or per wave.
+template <int WX, int WY, int WZ, int WPB, bool PM> constexpr int stage_lds_ctab_doubles()
+{
+    return (Patch<WX, WY, WZ, WPB, PM>::NB + 1) / 2;
+}
+
+// LDS of a workgroup, in doubles.  The cell tables come first.  PM: then the
+// staged neighbourhood box (gather stages, NB x 4 NH values of T) and the
+// waves' regions; with the merge (MODE 2) the deposit's merge buffer (NB x
+// deposited columns, double) reuses that space once every wave is done with
+// it.  Without PM: per wave, its own box and its region.  The host sizes the
+// launch with this function.
+template <int NF, typename T, int WX, int WY, int WZ, int WPB, int S, bool AUX, int MODE>
 constexpr int stage_lds_doubles_per_block()
 {
+    constexpr bool PM = MODE >= 1;
     constexpr int nh = NF * NF;
     constexpr int NB = Patch<WX, WY, WZ, WPB, PM>::NB;
+    constexpr int ct = (PM ? 1 : WPB) * stage_lds_ctab_doubles<WX, WY, WZ, WPB, PM>();
     constexpr int gsh = S == 0 ? 0 : (NB * 4 * nh * (int)sizeof(T) + 7) / 8;
-    if (!PM) return WPB * (gsh + stage_lds_doubles_per_wave<NF, T, AUX>());
+    if (!PM) return ct + WPB * (gsh + stage_lds_doubles_per_wave<NF, T, AUX>());
     constexpr int work = gsh + WPB * stage_lds_doubles_per_wave<NF, T, AUX>();
-    constexpr int merge = S <= 3 ? NB * (4 * nh + (AUX ? NF + 3 * (NF * (NF - 1) / 2) : 0)) : 0;
-    return work > merge ? work : merge;
+    constexpr int merge = MODE == 2 && S <= 3 ? NB * (4 * nh + (AUX ? NF + 3 * (NF * (NF - 1) / 2) : 0)) : 0;
+    return ct + (work > merge ? work : merge);
 }
 
-// the launches that take the patch mode (measured per launch kind at
-// 64^3 x 92, NF = 3: the deposit-only launch in both precisions, FP32-bulk
-// stage 4, the FP64 auxiliary stage; the other RK stages run faster as
-// independent waves)
-template <typename T, int S, bool AUX> constexpr bool stage_patch_mode()
+// the mapping of each launch kind (measured per launch kind at 64^3 x 92,
+// NF = 3): the deposit-only launch in both precisions and FP32-bulk stage 4
+// merge their deposits (MODE 2); the other RK stages run faster with
+// independent waves
+template <typename T, int S, bool AUX> constexpr int stage_mode()
 {
-    return S == 0 || (S == 4 && sizeof(T) == 4) || (AUX && sizeof(T) == 8);
+    return S == 0 || (S == 4 && sizeof(T) == 4) ? 2 : 0;
 }
+template <int MODE> constexpr int stage_waves_per_block() { return MODE == 2 ? 4 : 2; }
 
 // S: 0 = deposit only (the step's result, or the initial state), 1..4 = RK
 // stage; AUX: also deposit the compute_dt auxiliaries (S = 0 or 3).
-template <int NF, typename T, int WX, int WY, int WZ, int WPB, int S, bool AUX, bool PM>
+template <int NF, typename T, int WX, int WY, int WZ, int WPB, int S, bool AUX, int MODE>
 __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 2))) void k_stage(StageArgs<T> a)
 {
+    constexpr bool PM = MODE >= 1;    // the workgroup shares one patch and its staged box
+    constexpr bool MERGE = MODE == 2; // and merges its deposits in LDS
     constexpr int nh = NH<NF>;
     constexpr int QG = 4 * nh;
     constexpr int NT = WX * WY * WZ;        // target cells of a source cell
@@ -248,6 +263,7 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
     constexpr int NAUX = NF + 3 * NPAIR;
     constexpr int NSRC = nh + 4;             // q_c, vhat and the constant 1 per particle
     constexpr int NSAUX = NF + NPAIR;        // N f_ii + Nbar fbar_ii, Nbar Im(fbar_ab)
+    constexpr int NSR = NSRC + (AUX ? NSAUX : 0); // a particle's source row: main, then auxiliary
     extern __shared__ double lds[];
 
     const Geom& g = a.g;
@@ -273,8 +289,9 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
     // particle has nothing to do, but it stays for the workgroup's barriers
     const bool inside_domain = cx < g.n[0] && cy < g.n[1] && cz < g.n[2];
     const int cell = inside_domain ? (cz * g.n[1] + cy) * g.n[0] + cx : 0;
-    const int beg = inside_domain ? a.cell_start[cell] : 0;
-    const int end = inside_domain ? a.cell_start[cell + 1] : 0;
+    // (wave-uniform, so the particle arrays are addressed from scalar bases)
+    const int beg = __builtin_amdgcn_readfirstlane(inside_domain ? a.cell_start[cell] : 0);
+    const int end = __builtin_amdgcn_readfirstlane(inside_domain ? a.cell_start[cell + 1] : 0);
     // cell i of the neighbourhood box, box index (i, j, k) <-> the patch base
     // minus one in each 3-wide dimension, plus (i, j, k), periodic
     // (one conditional wrap covers a single cell's box: offsets are -1..+1 and a
@@ -294,29 +311,44 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
     };
     constexpr int GSH = s == 0 ? 0 : (NB * QG * (int)sizeof(T) + 7) / 8; // doubles of the staged box
     constexpr int per_wave = stage_lds_doubles_per_wave<NF, T, AUX>();
-    static_assert(per_wave == 64 * 9 + 64 * NSRC * (int)sizeof(T) / 8 + (aux ? 64 * NSAUX : 0),
+    static_assert(per_wave == 64 * 9 + 64 * NSR * (int)sizeof(T) / 8,
                   "LDS layout and its size function disagree");
+    constexpr int CT = stage_lds_ctab_doubles<WX, WY, WZ, WPB, PM>();
+    // [NB] the box's cell indices: the block's, or this wave's
+    int* ctab = (int*)(PM ? lds : lds + wave * CT);
+    double* lw = lds + (PM ? CT : WPB * CT);
     // [NB][QG] neighbourhood fields (gather stages): the block's, or this wave's
-    T* Gs = (T*)(PM ? lds : lds + wave * (GSH + per_wave));
-    double* Wf = PM ? lds + GSH + wave * per_wave : lds + wave * (GSH + per_wave) + GSH; // [64][9] shape factors
+    T* Gs = (T*)(PM ? lw : lw + wave * (GSH + per_wave));
+    double* Wf = PM ? lw + GSH + wave * per_wave : lw + wave * (GSH + per_wave) + GSH; // [64][9] shape factors
     // q_c, vhat (inside_c during the gather): values of the working precision T,
     // so FP32-bulk keeps them as float (lossless) and the LDS per wave shrinks
-    T* Src = (T*)(Wf + 64 * 9);             // [64][NSRC]
-    double* Saux = (double*)(Src + 64 * NSRC); // [64][NSAUX]
+    T* Src = (T*)(Wf + 64 * 9);             // [64][NSR]
+    // the auxiliary sources sit in T in the same rows: in FP32-bulk the products
+    // N f are FP32 values already (their sum over particles is accumulated in FP64)
+    constexpr int STRIDE = PM ? 64 * WPB : 64;
+    const int tid = PM ? (int)threadIdx.x : lane;
+    // the box's cell indices, once: the staging loads and the deposit's flush
+    // read them instead of redoing the periodic index arithmetic per element
+    for (int e = tid; e < NB; e += STRIDE) ctab[e] = (int)box_cell(e);
+    if constexpr (PM) {
+        __syncthreads();
+    } else {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
     if constexpr (s != 0) {
         // stage the box's fields, all waves together: NB rows of QG values
         const T* gsrc;
         if constexpr (sizeof(T) == 8) gsrc = (const T*)a.gin;
         else gsrc = (const T*)a.gin32;
         // a compile-time trip count, unrolled, so that every load is in flight at once
-        constexpr int STRIDE = PM ? 64 * WPB : 64;
-        const int tid = PM ? (int)threadIdx.x : lane;
 #pragma unroll
         for (int e0 = 0; e0 < NB * QG; e0 += STRIDE) {
             const int e = e0 + tid;
             if (e < NB * QG) {
                 const int r = e / QG, q = e - r * QG;
-                Gs[e] = gsrc[box_cell(r) * QG + q];
+                Gs[e] = gsrc[(int64_t)ctab[r] * QG + q];
             }
         }
         if constexpr (PM) {
@@ -333,39 +365,36 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
     const double cnext[4] = {0.5, 0.5, 1.0, 0.0};
 
     // lane constants of the deposit's MFMA operands.  A (row t = 16m + l%16):
-    // the three shape factors of target t, clamped to a valid target beyond
-    // NT and masked.  B (column q = 16n + l%16): the product of two entries of
-    // the particle's source row, the constant 1 standing in where a column
-    // has one factor; the auxiliary columns read the auxiliary row first.
+    // the three shape factors of target t.  B (column q = 16n + l%16): the
+    // product of two entries of the particle's source row, the constant 1
+    // standing in where a column has one factor.  Row t of the product depends
+    // only on row t of A and column q only on column q of B, so the padding
+    // rows (t >= NT) and columns (q >= QGA), which are never flushed, read any
+    // in-range entries instead of being masked.
     int tdx[MT][3];
-    bool tok[MT];
 #pragma unroll
     for (int m = 0; m < MT; m++) {
         const int t = 16 * m + (lane & 15);
-        tok[m] = t < NT;
         const int tc = t < NT ? t : NT - 1;
         const int ii = tc % WX, jj = (tc / WX) % WY, kk = tc / (WX * WY);
         tdx[m][0] = WX == 1 ? 1 : ii;
         tdx[m][1] = 3 + (WY == 1 ? 1 : jj);
         tdx[m][2] = 6 + (WZ == 1 ? 1 : kk);
     }
-    int bi1[NTL], bi2[NTL], bkind[NTL]; // kind: 0 main, 1 auxiliary, 2 zero
+    int bi1[NTL], bi2[NTL];
 #pragma unroll
     for (int n = 0; n < NTL; n++) {
         const int q = 16 * n + (lane & 15);
         if (q < QG) {
             const int f = q / nh, c = q % nh;
-            bkind[n] = 0;
             bi1[n] = c;
             bi2[n] = f == 0 ? nh + 3 : nh + f - 1;
         } else if (aux && q < QGA) {
             const int x = q - QG;
-            bkind[n] = 1;
-            bi1[n] = x < NF ? x : NF + (x - NF) / 3;
+            bi1[n] = NSRC + (x < NF ? x : NF + (x - NF) / 3);
             bi2[n] = x < NF ? nh + 3 : nh + (x - NF) % 3;
         } else {
-            bkind[n] = 2;
-            bi1[n] = 0;
+            bi1[n] = nh + 3;
             bi2[n] = nh + 3;
         }
     }
@@ -385,7 +414,10 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
         for (int d = 0; d < 3; d++) pos[d] = a.cur.pos[d][ipl];
         const T pupt = a.cur.pup[3][ipl];
         const T pupv[3] = {a.cur.pup[0][ipl], a.cur.pup[1][ipl], a.cur.pup[2][ipl]};
-        const T vhat[3] = {pupv[0] / pupt, pupv[1] / pupt, pupv[2] / pupt};
+        // the direction and 1/pupt for the fields (within an ulp of the quotients;
+        // the position update keeps the exact quotient)
+        const T inv_pupt = T(1) / pupt;
+        const T vhat[3] = {pupv[0] * inv_pupt, pupv[1] * inv_pupt, pupv[2] * inv_pupt};
         const T N = a.cur.Nn[0][ipl], Nb = a.cur.Nn[1][ipl];
         int idx[3];
         double wd[3][3];
@@ -398,8 +430,8 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
 #pragma unroll
             for (int e = 0; e < 3; e++) Wf[lane * 9 + 3 * d + e] = act ? wd[d][e] : 0.0;
 #pragma unroll
-        for (int d = 0; d < 3; d++) Src[lane * NSRC + nh + d] = act ? vhat[d] : T(0);
-        Src[lane * NSRC + nh + 3] = T(1);
+        for (int d = 0; d < 3; d++) Src[lane * NSR + nh + d] = act ? vhat[d] : T(0);
+        Src[lane * NSR + nh + 3] = T(1);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -417,6 +449,9 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
             {
                 using MF = Mfma<T>;
                 const int g4 = lane >> 4, c16 = lane & 15;
+                // B columns past NH only feed result columns that are dropped, so
+                // every lane reads an in-range entry, unmasked
+                const int cq = g4 * nh + (c16 < nh ? c16 : nh - 1);
                 const int ntile = (min(64, end - base) + 15) >> 4;
 #pragma unroll 1
                 for (int m = 0; m < ntile; m++) {
@@ -424,28 +459,36 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
                     double w3[9];
 #pragma unroll
                     for (int e = 0; e < 9; e++) w3[e] = Wf[p * 9 + e];
-                    const T* vp = Src + p * NSRC + nh;
+                    const T* vp = Src + p * NSR + nh;
                     const T v0 = vp[0], v1 = vp[1], v2 = vp[2];
                     const T sg = g4 == 0 ? T(1) : (g4 == 1 ? -v0 : (g4 == 2 ? -v1 : -v2));
+                    // A = (w_x s_g) (w_y w_z): 3 + WY WZ products, then one per target
+                    T ax[WX], ayz[WY * WZ];
+#pragma unroll
+                    for (int ii = 0; ii < WX; ii++) ax[ii] = T(w3[WX == 1 ? 1 : ii]) * sg;
+#pragma unroll
+                    for (int kk = 0; kk < WZ; kk++)
+#pragma unroll
+                        for (int jj = 0; jj < WY; jj++)
+                            ayz[kk * WY + jj] = T(w3[3 + (WY == 1 ? 1 : jj)] * w3[6 + (WZ == 1 ? 1 : kk)]);
                     typename MF::acc_t accg = {T(0), T(0), T(0), T(0)};
 #pragma unroll
                     for (int t = 0; t < NT; t++) {
                         const int ii = t % WX, jj = (t / WX) % WY, kk = t / (WX * WY);
-                        const double w = w3[WX == 1 ? 1 : ii] * (w3[3 + (WY == 1 ? 1 : jj)] * w3[6 + (WZ == 1 ? 1 : kk)]);
-                        const T A = T(w) * sg;
-                        const T B = c16 < nh ? Gs[box_of(t) * QG + g4 * nh + c16] : T(0);
+                        const T A = ax[ii] * ayz[kk * WY + jj];
+                        const T B = Gs[box_of(t) * QG + cq];
                         accg = MF::step(A, B, accg);
                     }
                     if (c16 < nh) {
 #pragma unroll
-                        for (int r = 0; r < 4; r++) Src[(16 * m + MF::row(r, lane)) * NSRC + c16] = accg[r];
+                        for (int r = 0; r < 4; r++) Src[(16 * m + MF::row(r, lane)) * NSR + c16] = accg[r];
                     }
                 }
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
-                for (int c = 0; c < nh; c++) inside[c] = Src[lane * NSRC + c];
+                for (int c = 0; c < nh; c++) inside[c] = Src[lane * NSR + c];
             }
             // ---- the RK registers this stage reads, as one batch of loads whose
             // latency the Hamiltonian and commutator arithmetic covers
@@ -453,7 +496,8 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
             // The accumulator after stage 1, y_n + dt/6 k1, is not stored: with
             // Y2 = y_n + dt/2 k1 it equals y_n + (Y2 - y_n)/3, which stage 2
             // rebuilds from the two arrays it reads anyway (one write and one
-            // read of the flavour state saved per step; rounding-level change).
+            // read of the flavour state saved per step; rounding-level change,
+            // with a multiply by the rounded 1/3 in place of the division).
             if constexpr (s == 1) {
 #pragma unroll
                 for (int c = 0; c < 2 * nh; c++) yn[c] = ao[c] = Y[c];
@@ -461,7 +505,7 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
 #pragma unroll
                 for (int c = 0; c < 2 * nh; c++) yn[c] = a.cur.f[(int64_t)c * a.np + ipl];
 #pragma unroll
-                for (int c = 0; c < 2 * nh; c++) ao[c] = yn[c] + (Y[c] - yn[c]) / T(3);
+                for (int c = 0; c < 2 * nh; c++) ao[c] = yn[c] + (Y[c] - yn[c]) * T(1.0 / 3.0);
             } else {
 #pragma unroll
                 for (int c = 0; c < 2 * nh; c++) yn[c] = s < 4 ? a.cur.f[(int64_t)c * a.np + ipl] : T(0);
@@ -470,7 +514,6 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
             }
             // ---- Hamiltonians / hbar: H = H_vac + coef inside, Hbar = conj(H_vac - coef inside)
             T H[nh], Hb[nh];
-            const T inv_pupt = T(1) / pupt;
 #pragma unroll
             for (int c = 0; c < nh; c++) {
                 const T hv = T(a.vac[c]) * inv_pupt;
@@ -544,18 +587,17 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
 #pragma unroll
         for (int c = 0; c < nh; c++) {
             const T qc = is_im_comp<NF>(c) ? N * Dep[c] + Nb * Dep[nh + c] : N * Dep[c] - Nb * Dep[nh + c];
-            Src[lane * NSRC + c] = act ? qc : T(0);
+            Src[lane * NSR + c] = act ? qc : T(0);
         }
         if (aux) {
 #pragma unroll
             for (int x = 0; x < NF; x++)
-                Saux[lane * NSAUX + x] =
-                    act ? double(N * Dep[herm_re<NF>(x, x)]) + double(Nb * Dep[nh + herm_re<NF>(x, x)]) : 0.0;
+                Src[lane * NSR + NSRC + x] = act ? N * Dep[herm_re<NF>(x, x)] + Nb * Dep[nh + herm_re<NF>(x, x)] : T(0);
             int o = NF;
 #pragma unroll
             for (int x = 0; x < NF; x++)
 #pragma unroll
-                for (int y2 = x + 1; y2 < NF; y2++) Saux[lane * NSAUX + (o++)] = act ? double(Nb * Dep[nh + herm_im<NF>(x, y2)]) : 0.0;
+                for (int y2 = x + 1; y2 < NF; y2++) Src[lane * NSR + NSRC + (o++)] = act ? Nb * Dep[nh + herm_im<NF>(x, y2)] : T(0);
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -565,33 +607,57 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
         // overlap the matrix work of the other; rows past the chunk's particles
         // hold zero sources (inactive lanes), so rounding the count up is exact
         // (the deposit-only launch keeps one k-step per iteration: there the
-        // registers of the second would cost a wave of occupancy)
+        // registers of the second would cost a wave of occupancy).  Each operand
+        // has one per-lane LDS pointer that advances by a whole iteration; the
+        // second k-step is an immediate offset.  B is the product in T: in
+        // FP32-bulk the flux sources q v are FP32 values like q itself, and the
+        // sum over particles stays FP64.
         constexpr int KU = s == 0 ? 1 : 2;
         const int nact = min(64, end - base);
         const int nkg = (nact + 4 * KU - 1) / (4 * KU);
+        // (32-bit LDS pointers made opaque once, so that the wave's base is not
+        // split off again and re-added to every address in the loop)
+        typedef __attribute__((address_space(3))) const double* lds_d;
+        typedef __attribute__((address_space(3))) const T* lds_t;
+        lds_d wa[MT][3];
+        lds_t sb[NTL][2];
+#pragma unroll
+        for (int m = 0; m < MT; m++)
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                wa[m][d] = (lds_d)(Wf + (lane >> 4) * 9 + tdx[m][d]);
+                asm volatile("" : "+v"(wa[m][d]));
+            }
+#pragma unroll
+        for (int n = 0; n < NTL; n++) {
+            sb[n][0] = (lds_t)(Src + (lane >> 4) * NSR + bi1[n]);
+            sb[n][1] = (lds_t)(Src + (lane >> 4) * NSR + bi2[n]);
+            asm volatile("" : "+v"(sb[n][0]));
+            asm volatile("" : "+v"(sb[n][1]));
+        }
+#pragma unroll 1
         for (int kp = 0; kp < nkg; kp++) {
 #pragma unroll
             for (int h = 0; h < KU; h++) {
-                const int p = 4 * KU * kp + 4 * h + (lane >> 4);
-                const double* wf = Wf + p * 9;
-                const T* sr = Src + p * NSRC;
-                const double* sa = Saux + p * NSAUX;
                 double A[MT];
 #pragma unroll
-                for (int m = 0; m < MT; m++) {
-                    const double w = wf[tdx[m][0]] * wf[tdx[m][1]] * wf[tdx[m][2]];
-                    A[m] = tok[m] ? w : 0.0;
-                }
+                for (int m = 0; m < MT; m++) A[m] = wa[m][0][36 * h] * wa[m][1][36 * h] * wa[m][2][36 * h];
 #pragma unroll
                 for (int n = 0; n < NTL; n++) {
-                    const double f2 = double(sr[bi2[n]]);
-                    double B = double(sr[bi1[n]]) * f2;
-                    if (aux) B = bkind[n] == 1 ? sa[bi1[n]] * f2 : B;
-                    B = bkind[n] == 2 ? 0.0 : B;
+                    const double B = double(sb[n][0][4 * NSR * h] * sb[n][1][4 * NSR * h]);
 #pragma unroll
                     for (int m = 0; m < MT; m++)
                         acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[m], B, acc[m][n], 0, 0, 0);
                 }
+            }
+#pragma unroll
+            for (int m = 0; m < MT; m++)
+#pragma unroll
+                for (int d = 0; d < 3; d++) wa[m][d] += 36 * KU;
+#pragma unroll
+            for (int n = 0; n < NTL; n++) {
+                sb[n][0] += 4 * NSR * KU;
+                sb[n][1] += 4 * NSR * KU;
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -602,7 +668,7 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
     // For the f64 16x16x4 product, element r of lane l holds
     // D[16m + 4r + l/16][16n + l%16] (rows interleaved across the four 16-lane
     // groups, unlike the f32 form's 4(l/16) + r).
-    if constexpr (!PM) {
+    if constexpr (!MERGE) {
         // ---- flush this wave's deposit straight to the grid
         if (end <= beg) return;
 #pragma unroll
@@ -611,7 +677,7 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
             for (int r = 0; r < 4; r++) {
                 const int t = 16 * m + 4 * r + (lane >> 4);
                 if (t >= NT) continue;
-                const int64_t tc = box_cell(box_of(t));
+                const int64_t tc = ctab[box_of(t)];
 #pragma unroll
                 for (int n = 0; n < NTL; n++) {
                     const int q = 16 * n + (lane & 15);
@@ -621,8 +687,8 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
             }
         return;
     }
-    // ---- merge the four waves' deposits over the box in LDS, then flush
-    double* Mb = lds; // [NB][QGA], over the staged box and the waves' regions
+    // ---- merge the waves' deposits over the box in LDS, then flush
+    double* Mb = lw; // [NB][QGA], over the staged box and the waves' regions (not the cell table)
     __syncthreads();
     for (int e = threadIdx.x; e < NB * QGA; e += 64 * WPB) Mb[e] = 0.0;
     __syncthreads();
@@ -646,7 +712,7 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
         const double v = Mb[e];
         if (v == 0.0) continue;
         const int bi = e / QGA, q = e - bi * QGA;
-        const int64_t tc = box_cell(bi);
+        const int64_t tc = ctab[bi];
         if (q < QG) unsafeAtomicAdd(&a.gout[tc * QG + q], v);
         else unsafeAtomicAdd(&a.aux[tc * NAUX + (q - QG)], v);
     }

@@ -527,16 +527,17 @@ int scan_counts(emu_state* st, int* out)
 template <int NF, typename T, int WX, int WY, int WZ, int S, bool AUX>
 void launch_stage_k(emu_state* st, StageArgs<T>& a)
 {
-    constexpr bool PM = stage_patch_mode<T, S, AUX>();
-    constexpr int WPB = PM ? 4 : 2; // waves per workgroup
+    constexpr int MODE = stage_mode<T, S, AUX>();
+    constexpr bool PM = MODE >= 1;
+    constexpr int WPB = stage_waves_per_block<MODE>(); // waves per workgroup
     using PT = Patch<WX, WY, WZ, WPB, PM>;
     const Geom& g = st->ctx->g;
     const int64_t npatch = (int64_t)((g.n[0] + PT::PX - 1) / PT::PX) * ((g.n[1] + PT::PY - 1) / PT::PY) *
                            ((g.n[2] + PT::PZ - 1) / PT::PZ);
     const int64_t nblk = PM ? npatch : (npatch + WPB - 1) / WPB;
     a.blocks_per_xcd = (int)((nblk + 7) / 8);
-    const size_t lds = (size_t)stage_lds_doubles_per_block<NF, T, WX, WY, WZ, WPB, S, AUX, PM>() * sizeof(double);
-    hipLaunchKernelGGL((k_stage<NF, T, WX, WY, WZ, WPB, S, AUX, PM>), dim3((unsigned)(8 * a.blocks_per_xcd)),
+    const size_t lds = (size_t)stage_lds_doubles_per_block<NF, T, WX, WY, WZ, WPB, S, AUX, MODE>() * sizeof(double);
+    hipLaunchKernelGGL((k_stage<NF, T, WX, WY, WZ, WPB, S, AUX, MODE>), dim3((unsigned)(8 * a.blocks_per_xcd)),
                        dim3(64 * WPB), lds, st->ctx->stream, a);
 }
 
