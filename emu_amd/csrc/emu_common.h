@@ -105,44 +105,45 @@ EMU_HD int pmod(int i, int n)
 template <int NF, typename T>
 EMU_HD void commutator_rhs(const T* H, const T* f, T* out, T scale)
 {
-    T hr[NF][NF], hi[NF][NF], fr[NF][NF], fi[NF][NF];
+    // C = A - A^dagger with A = H f needs Im A_ii and A_ij, A_ji for i < j only.
+    // Each is one FMA chain; the products with a zero factor (the imaginary
+    // parts of diagonal entries) are left out at compile time, and the chains
+    // start from -0, which leaves the first product exact.
+    auto re = [&](const T* X, int i, int j) -> T { return X[herm_re<NF>(i < j ? i : j, i < j ? j : i)]; };
+    auto im = [&](const T* X, int i, int j) -> T { // i != j
+        return i < j ? X[herm_im<NF>(i, j)] : -X[herm_im<NF>(j, i)];
+    };
+    auto a_re = [&](int i, int j) -> T {
+        T s = T(-0.0);
 #pragma unroll
-    for (int i = 0; i < NF; i++)
-#pragma unroll
-        for (int j = 0; j < NF; j++) {
-            hr[i][j] = H[herm_re<NF>(i, j)];
-            fr[i][j] = f[herm_re<NF>(i, j)];
-            if (i == j) { hi[i][j] = T(0); fi[i][j] = T(0); }
-            else {
-                const T s = i < j ? T(1) : T(-1);
-                hi[i][j] = s * H[herm_im<NF>(i, j)];
-                fi[i][j] = s * f[herm_im<NF>(i, j)];
-            }
+        for (int k = 0; k < NF; k++) {
+            s = fma(re(H, i, k), re(f, k, j), s);
+            if (i != k && k != j) s = fma(-im(H, i, k), im(f, k, j), s);
         }
-    T ar[NF][NF], ai[NF][NF]; // A = H f
+        return s;
+    };
+    auto a_im = [&](int i, int j) -> T {
+        T s = T(-0.0);
 #pragma unroll
-    for (int i = 0; i < NF; i++)
-#pragma unroll
-        for (int j = 0; j < NF; j++) {
-            T sr = T(0), si = T(0);
-#pragma unroll
-            for (int k = 0; k < NF; k++) {
-                sr += hr[i][k] * fr[k][j] - hi[i][k] * fi[k][j];
-                si += hr[i][k] * fi[k][j] + hi[i][k] * fr[k][j];
-            }
-            ar[i][j] = sr;
-            ai[i][j] = si;
+        for (int k = 0; k < NF; k++) {
+            if (k != j) s = fma(re(H, i, k), im(f, k, j), s);
+            if (i != k) s = fma(im(H, i, k), re(f, k, j), s);
         }
+        return s;
+    };
 #pragma unroll
-    for (int i = 0; i < NF; i++)
+    for (int i = 0; i < NF; i++) {
+        // C_ii = 2i Im A_ii; (-i) C_ii = 2 Im A_ii
+        out[herm_re<NF>(i, i)] = T(2) * a_im(i, i) * scale;
 #pragma unroll
-        for (int j = i; j < NF; j++) {
+        for (int j = i + 1; j < NF; j++) {
             // C_ij = A_ij - conj(A_ji); (-i)(Cr + i Ci) = Ci - i Cr
-            const T cr = ar[i][j] - ar[j][i];
-            const T ci = ai[i][j] + ai[j][i];
+            const T cr = a_re(i, j) - a_re(j, i);
+            const T ci = a_im(i, j) + a_im(j, i);
             out[herm_re<NF>(i, j)] = ci * scale;
-            if (j > i) out[herm_im<NF>(i, j)] = -cr * scale;
+            out[herm_im<NF>(i, j)] = -cr * scale;
         }
+    }
 }
 
 // |X|_SU^2, HermitianUtils.py:102-120.
