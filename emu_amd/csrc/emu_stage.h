@@ -381,10 +381,18 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
         tdx[m][1] = 3 + (WY == 1 ? 1 : jj);
         tdx[m][2] = 6 + (WZ == 1 ? 1 : kk);
     }
+    // A last column tile of at most 4 columns (NF = 3 without the auxiliaries: 36 =
+    // 2 x 16 + 4) runs on the 4-block 4x4x4 form instead: two 16-cycle steps in
+    // place of two 64-cycle 16x16x4 steps.  With blocks as the row quarters
+    // t = 16m + 4b + i, its A operand is lane for lane the 16x16x4 one; its B
+    // column is 16 NTF + l%4, and its result D[t][q] sits in lane 16i + 4b + j.
+    constexpr int QREM = QGA % 16;
+    constexpr bool SMALL = QREM != 0 && QREM <= 4;
+    constexpr int NTF = SMALL ? NTL - 1 : NTL; // 16-column tiles
     int bi1[NTL], bi2[NTL];
 #pragma unroll
     for (int n = 0; n < NTL; n++) {
-        const int q = 16 * n + (lane & 15);
+        const int q = SMALL && n == NTF ? 16 * n + (lane & 3) : 16 * n + (lane & 15);
         if (q < QG) {
             const int f = q / nh, c = q % nh;
             bi1[n] = c;
@@ -399,10 +407,13 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
         }
     }
     mf_d4 acc[MT][NTL];
+    double accs[MT]; // the 4x4x4 tile
 #pragma unroll
-    for (int m = 0; m < MT; m++)
+    for (int m = 0; m < MT; m++) {
+        accs[m] = 0.0;
 #pragma unroll
         for (int n = 0; n < NTL; n++) acc[m][n] = mf_d4{0.0, 0.0, 0.0, 0.0};
+    }
     const bool deposits = s <= 3;
 
     for (int base = beg; base < end; base += 64) {
@@ -452,6 +463,13 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
                 // B columns past NH only feed result columns that are dropped, so
                 // every lane reads an in-range entry, unmasked
                 const int cq = g4 * nh + (c16 < nh ? c16 : nh - 1);
+                constexpr int NCB = (nh + 3) / 4; // 4-column blocks (FP64 form)
+                int cqb[NCB];
+#pragma unroll
+                for (int cb = 0; cb < NCB; cb++) {
+                    const int c = 4 * cb + (lane & 3);
+                    cqb[cb] = g4 * nh + (c < nh ? c : nh - 1);
+                }
                 const int ntile = (min(64, end - base) + 15) >> 4;
 #pragma unroll 1
                 for (int m = 0; m < ntile; m++) {
@@ -471,17 +489,43 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
 #pragma unroll
                         for (int jj = 0; jj < WY; jj++)
                             ayz[kk * WY + jj] = T(w3[3 + (WY == 1 ? 1 : jj)] * w3[6 + (WZ == 1 ? 1 : kk)]);
-                    typename MF::acc_t accg = {T(0), T(0), T(0), T(0)};
+                    if constexpr (sizeof(T) == 8) {
+                        // FP64: the NH result columns in blocks of 4 on the 4-block
+                        // 4x4x4 form (16 cycles each): 3 blocks instead of one 16-column
+                        // 16x16x4 (64 cycles) at NF = 3.  Block b is particles
+                        // 16m + 4b .. +3; A[b][i][k] sits in lane 16k + 4b + i, which is
+                        // the same particle and group as the 16x16x4 operand above;
+                        // B[b][k][j] in lane 16k + 4b + j; D[b][i][j] in lane 16i + 4b + j.
+                        double accb[NCB];
 #pragma unroll
-                    for (int t = 0; t < NT; t++) {
-                        const int ii = t % WX, jj = (t / WX) % WY, kk = t / (WX * WY);
-                        const T A = ax[ii] * ayz[kk * WY + jj];
-                        const T B = Gs[box_of(t) * QG + cq];
-                        accg = MF::step(A, B, accg);
-                    }
-                    if (c16 < nh) {
+                        for (int cb = 0; cb < NCB; cb++) accb[cb] = 0.0;
 #pragma unroll
-                        for (int r = 0; r < 4; r++) Src[(16 * m + MF::row(r, lane)) * NSR + c16] = accg[r];
+                        for (int t = 0; t < NT; t++) {
+                            const int ii = t % WX, jj = (t / WX) % WY, kk = t / (WX * WY);
+                            const T A = ax[ii] * ayz[kk * WY + jj];
+#pragma unroll
+                            for (int cb = 0; cb < NCB; cb++)
+                                accb[cb] = __builtin_amdgcn_mfma_f64_4x4x4f64(A, Gs[box_of(t) * QG + cqb[cb]], accb[cb], 0, 0, 0);
+                        }
+                        const int pr = 16 * m + 4 * ((lane >> 2) & 3) + (lane >> 4);
+#pragma unroll
+                        for (int cb = 0; cb < NCB; cb++) {
+                            const int c = 4 * cb + (lane & 3);
+                            if (c < nh) Src[pr * NSR + c] = T(accb[cb]);
+                        }
+                    } else {
+                        typename MF::acc_t accg = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+                        for (int t = 0; t < NT; t++) {
+                            const int ii = t % WX, jj = (t / WX) % WY, kk = t / (WX * WY);
+                            const T A = ax[ii] * ayz[kk * WY + jj];
+                            const T B = Gs[box_of(t) * QG + cq];
+                            accg = MF::step(A, B, accg);
+                        }
+                        if (c16 < nh) {
+#pragma unroll
+                            for (int r = 0; r < 4; r++) Src[(16 * m + MF::row(r, lane)) * NSR + c16] = accg[r];
+                        }
                     }
                 }
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -645,9 +689,14 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
 #pragma unroll
                 for (int n = 0; n < NTL; n++) {
                     const double B = double(sb[n][0][4 * NSR * h] * sb[n][1][4 * NSR * h]);
+                    if (SMALL && n == NTF) {
 #pragma unroll
-                    for (int m = 0; m < MT; m++)
-                        acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[m], B, acc[m][n], 0, 0, 0);
+                        for (int m = 0; m < MT; m++) accs[m] = __builtin_amdgcn_mfma_f64_4x4x4f64(A[m], B, accs[m], 0, 0, 0);
+                    } else {
+#pragma unroll
+                        for (int m = 0; m < MT; m++)
+                            acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[m], B, acc[m][n], 0, 0, 0);
+                    }
                 }
             }
 #pragma unroll
@@ -679,12 +728,22 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
                 if (t >= NT) continue;
                 const int64_t tc = ctab[box_of(t)];
 #pragma unroll
-                for (int n = 0; n < NTL; n++) {
+                for (int n = 0; n < NTF; n++) {
                     const int q = 16 * n + (lane & 15);
                     if (q < QG) unsafeAtomicAdd(&a.gout[tc * QG + q], acc[m][n][r]);
                     else if (aux && q < QGA) unsafeAtomicAdd(&a.aux[tc * NAUX + (q - QG)], acc[m][n][r]);
                 }
             }
+        if constexpr (SMALL) {
+#pragma unroll
+            for (int m = 0; m < MT; m++) {
+                const int t = 16 * m + 4 * ((lane >> 2) & 3) + (lane >> 4), q = 16 * NTF + (lane & 3);
+                if (t >= NT || q >= QGA) continue;
+                const int64_t tc = ctab[box_of(t)];
+                if (q < QG) unsafeAtomicAdd(&a.gout[tc * QG + q], accs[m]);
+                else unsafeAtomicAdd(&a.aux[tc * NAUX + (q - QG)], accs[m]);
+            }
+        }
         return;
     }
     // ---- merge the waves' deposits over the box in LDS, then flush
@@ -701,11 +760,18 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
                 if (t >= NT) continue;
                 const int bi = box_of(t);
 #pragma unroll
-                for (int n = 0; n < NTL; n++) {
+                for (int n = 0; n < NTF; n++) {
                     const int q = 16 * n + (lane & 15);
                     if (q < QGA) unsafeAtomicAdd(&Mb[bi * QGA + q], acc[m][n][r]);
                 }
             }
+        if constexpr (SMALL) {
+#pragma unroll
+            for (int m = 0; m < MT; m++) {
+                const int t = 16 * m + 4 * ((lane >> 2) & 3) + (lane >> 4), q = 16 * NTF + (lane & 3);
+                if (t < NT && q < QGA) unsafeAtomicAdd(&Mb[box_of(t) * QGA + q], accs[m]);
+            }
+        }
     }
     __syncthreads();
     for (int e = threadIdx.x; e < NB * QGA; e += 64 * WPB) {
