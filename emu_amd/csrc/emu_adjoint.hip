@@ -1,9 +1,11 @@
 // emu_adjoint.hip -- the flavour right-hand side and its vector-Jacobian
 // product (VJP), for reverse-mode gradients through the RK4 evolution.
 //
-// Particles stay in a fixed order (gradients are per particle), so every
-// kernel handles one particle per thread with its own stencil and the
-// deposits use FP64 atomics.  All arrays are SoA: [component][np].
+// Particles stay in a fixed order (gradients are per particle).  The gathers
+// take one particle per thread with its own stencil; the deposits sum over
+// the particles that share a stencil on the matrix cores first
+// (k_seg_deposit), then add to the grid with FP64 atomics.  All arrays are
+// SoA: [component][np].
 //
 // Forward RHS for stage state Y (positions frozen within the step):
 //   q_p  = N_p f_p - conj(Nbar_p fbar_p)                    (per particle)
@@ -73,37 +75,6 @@ __global__ void k_fill_base(double* grid, int64_t ncell, double m00)
     const int64_t n = ncell * 4 * NH<NF>;
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
         grid[t] = (t % (4 * NH<NF>)) == 0 ? m00 : 0.0;
-}
-
-// deposit of q (forward) into [ncell][4 NH]
-template <int NF>
-__global__ void k_deposit_q(AdjIn in, Geom g, double* grid)
-{
-    constexpr int nh = NH<NF>, QG = 4 * nh;
-    const int64_t np = in.np;
-    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
-        const double pos[3] = {in.pos[p], in.pos[np + p], in.pos[2 * np + p]};
-        const double pt = in.pup[3 * np + p];
-        const double v[3] = {in.pup[p] / pt, in.pup[np + p] / pt, in.pup[2 * np + p] / pt};
-        const double N = in.Nn[p], Nb = in.Nn[np + p];
-        double q[nh];
-        for (int c = 0; c < nh; c++) {
-            const double f = in.Y[(int64_t)c * np + p], fb = in.Y[(int64_t)(nh + c) * np + p];
-            q[c] = is_im_comp<NF>(c) ? N * f + Nb * fb : N * f - Nb * fb;
-        }
-        Stencil S;
-        stencil_of(g, pos, S);
-        for (int t = 0; t < S.n; t++) {
-            double* G = grid + S.cell[t] * QG;
-            for (int c = 0; c < nh; c++) {
-                const double wq = S.w[t] * q[c];
-                unsafeAtomicAdd(&G[c], wq);
-                unsafeAtomicAdd(&G[nh + c], wq * v[0]);
-                unsafeAtomicAdd(&G[2 * nh + c], wq * v[1]);
-                unsafeAtomicAdd(&G[3 * nh + c], wq * v[2]);
-            }
-        }
-    }
 }
 
 struct HamParams {
@@ -208,28 +179,134 @@ __global__ void k_vjp_comm(AdjIn in, Geom g, const double* grid, HamParams hp, c
     }
 }
 
-// VJP part 2: scatter mu (adjoint of the gather) into the grid adjoint.
-template <int NF>
-__global__ void k_vjp_scatter(AdjIn in, Geom g, const double* mu, double* gadj)
+// The deposit of both directions with the sum over particles on the matrix
+// cores.  KIND 0: the forward q_c = N f_c - conj(Nbar fbar_c) with (1, v);
+// KIND 1: the gather's adjoint mu_c with (1, -v).  Per-particle atomics put
+// every particle of a cell on the same 27 x 4 NH grid entries, so the sum
+// over a cell's particles is done first, as in the fused stage kernel.
+//
+// One wave takes 64 consecutive particles of the fixed order.  Particles
+// whose stencils share their centre cell (the key) share their 27 targets;
+// for each distinct key in the wave, D[t][q] = sum_p W[p][t] R[p][q] runs on
+// f64 16x16x4 MFMAs over the k-steps that span the key's lanes, with the
+// other lanes' sources zeroed, and is then added to the grid once.  The
+// order is arbitrary for correctness; the initial order is cell-major, so a
+// wave usually holds one or two keys.
+template <int NF, int WX, int WY, int WZ, int KIND>
+__global__ __launch_bounds__(128) void k_seg_deposit(AdjIn in, Geom g, const double* mu, double* grid)
 {
     constexpr int nh = NH<NF>, QG = 4 * nh;
+    constexpr int NT = WX * WY * WZ, MT = (NT + 15) / 16, NTL = (QG + 15) / 16;
+    constexpr int NSR = nh + 4;            // q (or mu), v (or -v), 1
+    constexpr int REG = 64 * 9 + 64 * NSR + 32; // doubles per wave: weights, sources, keys
+    __shared__ double lds[2 * REG];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double* Wf = lds + wave * REG;
+    double* Sr = Wf + 64 * 9;
+    int* Kc = (int*)(Sr + 64 * NSR);
     const int64_t np = in.np;
-    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
-        const double pos[3] = {in.pos[p], in.pos[np + p], in.pos[2 * np + p]};
-        const double pt = in.pup[3 * np + p];
-        const double v[3] = {in.pup[p] / pt, in.pup[np + p] / pt, in.pup[2 * np + p] / pt};
-        Stencil S;
-        stencil_of(g, pos, S);
-        for (int t = 0; t < S.n; t++) {
-            double* G = gadj + S.cell[t] * QG;
-            for (int c = 0; c < nh; c++) {
-                const double wm = S.w[t] * mu[(int64_t)c * np + p];
-                unsafeAtomicAdd(&G[c], wm);
-                unsafeAtomicAdd(&G[nh + c], -wm * v[0]);
-                unsafeAtomicAdd(&G[2 * nh + c], -wm * v[1]);
-                unsafeAtomicAdd(&G[3 * nh + c], -wm * v[2]);
+    const int64_t p0 = ((int64_t)blockIdx.x * 2 + wave) * 64;
+    if (p0 >= np) return; // uniform over the wave
+    const int64_t p = p0 + lane;
+    const bool act = p < np;
+    const int64_t pl = act ? p : p0;
+    const double pos[3] = {in.pos[pl], in.pos[np + pl], in.pos[2 * np + pl]};
+    const double pt = in.pup[3 * np + pl];
+    const double v[3] = {in.pup[pl] / pt, in.pup[np + pl] / pt, in.pup[2 * np + pl] / pt};
+    int idx[3];
+    double wd[3][3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) shape3((pos[d] - g.plo[d]) * g.dxi[d], g.order[d], idx[d], wd[d]);
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+#pragma unroll
+        for (int e = 0; e < 3; e++) Wf[lane * 9 + 3 * d + e] = wd[d][e];
+    if constexpr (KIND == 0) {
+        const double N = in.Nn[pl], Nb = in.Nn[np + pl];
+#pragma unroll
+        for (int c = 0; c < nh; c++) {
+            const double f = in.Y[(int64_t)c * np + pl], fb = in.Y[(int64_t)(nh + c) * np + pl];
+            Sr[lane * NSR + c] = act ? (is_im_comp<NF>(c) ? N * f + Nb * fb : N * f - Nb * fb) : 0.0;
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < nh; c++) Sr[lane * NSR + c] = act ? mu[(int64_t)c * np + pl] : 0.0;
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++) Sr[lane * NSR + nh + d] = KIND == 0 ? v[d] : -v[d];
+    Sr[lane * NSR + nh + 3] = 1.0;
+    const int cx = pmod(idx[0], g.n[0]), cy = pmod(idx[1], g.n[1]), cz = pmod(idx[2], g.n[2]);
+    const int key = (cz * g.n[1] + cy) * g.n[0] + cx;
+    Kc[lane] = key;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // operand lane constants, as in the stage kernel: row t = 16m + l%16 reads the
+    // shape factors of target t; column q = f nh + c is source c times (1, v)_f
+    int tdx[MT][3];
+#pragma unroll
+    for (int m = 0; m < MT; m++) {
+        const int t = 16 * m + (lane & 15);
+        const int tc = t < NT ? t : NT - 1;
+        const int ii = tc % WX, jj = (tc / WX) % WY, kk = tc / (WX * WY);
+        tdx[m][0] = WX == 1 ? 1 : ii;
+        tdx[m][1] = 3 + (WY == 1 ? 1 : jj);
+        tdx[m][2] = 6 + (WZ == 1 ? 1 : kk);
+    }
+    int bi1[NTL], bi2[NTL];
+#pragma unroll
+    for (int n = 0; n < NTL; n++) {
+        const int q = 16 * n + (lane & 15);
+        const int f = q / nh, c = q % nh;
+        bi1[n] = q < QG ? c : nh + 3;
+        bi2[n] = q < QG ? (f == 0 ? nh + 3 : nh + f - 1) : nh + 3;
+    }
+    unsigned long long rem = __ballot(act);
+    while (rem) { // uniform: one pass per distinct key in the wave
+        const int lead = __builtin_ctzll(rem);
+        const int key0 = __builtin_amdgcn_readlane(key, lead);
+        const unsigned long long seg = __ballot(act && key == key0);
+        rem &= ~seg;
+        const int lo = __builtin_ctzll(seg), hi = 63 - __builtin_clzll(seg);
+        mf_d4 acc[MT][NTL];
+#pragma unroll
+        for (int m = 0; m < MT; m++)
+#pragma unroll
+            for (int n = 0; n < NTL; n++) acc[m][n] = mf_d4{0.0, 0.0, 0.0, 0.0};
+        for (int ks = lo >> 2; ks <= (hi >> 2); ks++) {
+            const int pp = 4 * ks + (lane >> 4);
+            const double in_seg = ((seg >> pp) & 1ull) ? 1.0 : 0.0;
+            const double* wf = Wf + pp * 9;
+            const double* sr = Sr + pp * NSR;
+            double A[MT];
+#pragma unroll
+            for (int m = 0; m < MT; m++) A[m] = wf[tdx[m][0]] * wf[tdx[m][1]] * wf[tdx[m][2]];
+#pragma unroll
+            for (int n = 0; n < NTL; n++) {
+                const double B = in_seg * (sr[bi1[n]] * sr[bi2[n]]);
+#pragma unroll
+                for (int m = 0; m < MT; m++) acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[m], B, acc[m][n], 0, 0, 0);
             }
         }
+        // element r of lane l holds D[16m + 4r + l/16][16n + l%16]
+        const int kx = key0 % g.n[0], ky = (key0 / g.n[0]) % g.n[1], kz = key0 / (g.n[0] * g.n[1]);
+#pragma unroll
+        for (int m = 0; m < MT; m++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int t = 16 * m + 4 * r + (lane >> 4);
+                if (t >= NT) continue;
+                const int ii = t % WX, jj = (t / WX) % WY, kk = t / (WX * WY);
+                const int tx = WX == 1 ? kx : pmod(kx + ii - 1, g.n[0]);
+                const int ty = WY == 1 ? ky : pmod(ky + jj - 1, g.n[1]);
+                const int tz = WZ == 1 ? kz : pmod(kz + kk - 1, g.n[2]);
+                double* G = grid + (((int64_t)tz * g.n[1] + ty) * g.n[0] + tx) * QG;
+#pragma unroll
+                for (int n = 0; n < NTL; n++) {
+                    const int q = 16 * n + (lane & 15);
+                    if (q < QG) unsafeAtomicAdd(&G[q], acc[m][n][r]);
+                }
+            }
     }
 }
 
@@ -394,13 +471,41 @@ static double m00_of(const emu_ctx* ctx)
     return ctx->p.rho * ctx->p.Ye / phys::Mp / ctx->g.inv_cell_volume;
 }
 
+// k_seg_deposit for the stencil widths of the grid (3 where the shape order is
+// above 0, else 1), 2 waves per workgroup of 128 threads
+template <int NF, int KIND>
+static void seg_deposit_nf(emu_ctx* ctx, const AdjIn& in, const double* mu, double* grid)
+{
+    const int wx = ctx->g.order[0] > 0 ? 3 : 1, wy = ctx->g.order[1] > 0 ? 3 : 1, wz = ctx->g.order[2] > 0 ? 3 : 1;
+    const unsigned nb = (unsigned)((in.np + 127) / 128);
+    const int code = (wx == 3) * 4 + (wy == 3) * 2 + (wz == 3);
+#define SEG_LAUNCH(X, Y, Z) hipLaunchKernelGGL((k_seg_deposit<NF, X, Y, Z, KIND>), dim3(nb), dim3(128), 0, ctx->stream, in, ctx->g, mu, grid)
+    switch (code) {
+    case 0: SEG_LAUNCH(1, 1, 1); break;
+    case 1: SEG_LAUNCH(1, 1, 3); break;
+    case 2: SEG_LAUNCH(1, 3, 1); break;
+    case 3: SEG_LAUNCH(1, 3, 3); break;
+    case 4: SEG_LAUNCH(3, 1, 1); break;
+    case 5: SEG_LAUNCH(3, 1, 3); break;
+    case 6: SEG_LAUNCH(3, 3, 1); break;
+    default: SEG_LAUNCH(3, 3, 3); break;
+    }
+#undef SEG_LAUNCH
+}
+
+template <int KIND> static void seg_deposit(emu_ctx* ctx, const AdjIn& in, const double* mu, double* grid)
+{
+    if (in.np == 0) return;
+    if (ctx->p.nf == 2) seg_deposit_nf<2, KIND>(ctx, in, mu, grid);
+    else seg_deposit_nf<3, KIND>(ctx, in, mu, grid);
+}
+
 static void deposit(emu_ctx* ctx, const AdjIn& in, double* grid)
 {
-    const int64_t np = in.np;
     const int64_t ng = ncell_of(ctx) * 4 * ctx->p.nf * ctx->p.nf;
     if (ctx->p.nf == 2) hipLaunchKernelGGL(k_fill_base<2>, dim3(nblk(ng)), dim3(kB), 0, ctx->stream, grid, ncell_of(ctx), m00_of(ctx));
     else hipLaunchKernelGGL(k_fill_base<3>, dim3(nblk(ng)), dim3(kB), 0, ctx->stream, grid, ncell_of(ctx), m00_of(ctx));
-    ADJ_DISPATCH(k_deposit_q, in, ctx->g, grid);
+    seg_deposit<0>(ctx, in, nullptr, grid);
 }
 
 extern "C" int emu_adj_rhs(emu_ctx* ctx, int64_t np, const double* pos, const double* pup, const double* Nn,
@@ -430,7 +535,7 @@ extern "C" int emu_adj_rhs_vjp(emu_ctx* ctx, int64_t np, const double* pos, cons
     deposit(ctx, in, grid);
     ADJ_DISPATCH(k_vjp_comm, in, ctx->g, grid, ham_params(ctx), lam_k, lam_Y, mu);
     hipLaunchKernelGGL(k_zero, dim3(nblk(ng)), dim3(kB), 0, ctx->stream, gadj, ng);
-    ADJ_DISPATCH(k_vjp_scatter, in, ctx->g, mu, gadj);
+    seg_deposit<1>(ctx, in, mu, gadj);
     ADJ_DISPATCH(k_vjp_gather, in, ctx->g, gadj, lam_Y, lam_Nbar);
     EMU_LAUNCH_CHECK(ctx, "adj_rhs_vjp");
     return EMU_OK;
