@@ -139,6 +139,60 @@ def cpu_baseline(parms_dict, budget_s=15.0):
                       f"{threads} threads of {ncores} ({cpu_model})"}
 
 
+def run_adjoint(args):
+    """One gradient of J = sum over cells |N_emu|^2 after nsteps RK4 steps with
+    respect to the initial flavour state and the antineutrino weight ratio:
+    the forward run keeping checkpoints, then the reverse sweep (recompute
+    per segment and the VJP of every stage).  Particle-steps/s counts np x
+    nsteps for the whole gradient.  Algorithmic bytes: the forward formula
+    once for the forward run, once for the recompute and twice for the reverse
+    (one scatter and one gather per stage), all FP64."""
+    import torch
+    from emu_amd import adjoint, configs, core
+    workload = "adjoint32" if args.workload == "ff64" else args.workload
+    nsteps = args.steps if args.steps != 50 else 200
+    parms = configs.WORKLOADS[workload]()
+    emu = core.Emu(parms)
+    parts = core.FlavoredNeutrinoContainer.init_particles(emu)
+    st = core.FusedState(emu, parts.np, "fp64")
+    st.load(parts)
+    dt = st.info()["dt"]  # compute_dt of the initial state, held fixed over the run
+    st.close()
+    cp = max(1, min(20, nsteps))
+    prob = adjoint.AdjointProblem(emu, parts, dt=dt, nsteps=nsteps, checkpoint_every=cp)
+    f0 = prob.f0.clone()
+    warm = adjoint.AdjointProblem(emu, parts, dt=dt, nsteps=min(2, nsteps), checkpoint_every=1)
+    warm.gradients(warm.f0.clone(), 1.0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    J, lam, cps, Nn = prob.run(f0, 1.0)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    g = prob.adjoint_sweep(lam, cps, Nn)
+    dratio = float((prob.lam_nbar * prob.Nn_base[1]).sum().item())
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    ncell = parms.ncell[0] * parms.ncell[1] * parms.ncell[2]
+    ppc = parts.np / ncell
+    bps = 4 * algorithmic_bytes(parms.nf, ppc, "fp64")
+    total = t2 - t0
+    achieved = parts.np * nsteps * bps / total / 1e9
+    out = {
+        "metric": "adjoint gradient (forward with checkpoints + reverse sweep), particle-steps/s",
+        "value": parts.np * nsteps / total, "unit": "particle-steps/s", "n_gpus": 1, "steps": nsteps,
+        "ms_per_step": total / nsteps * 1e3, "higher_is_better": True, "dtype": "f64",
+        "data": "synthetic (seeded initial conditions of the workload, generated on the GPU)",
+        "config": {"workload": f"{workload}: {parms.nf} flavours, {parms.ncell[0]}x{parms.ncell[1]}x{parms.ncell[2]} "
+                               f"periodic cells, {round(ppc)} directions/cell, {parts.np} particles",
+                   "particles": parts.np, "checkpoint_every": cp, "dt_s": dt},
+        "forward_ms_per_step": (t1 - t0) / nsteps * 1e3, "reverse_ms_per_step": (t2 - t1) / nsteps * 1e3,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes_per_particle_step": bps},
+        "J": J, "dJ_dratio": dratio, "grad_f0_norm": float(g.norm().item()),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -147,6 +201,10 @@ def main():
     ap.add_argument("--workload", default="ff64")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--precision", default="both", choices=["both", "fp64", "fp32"])
+    ap.add_argument("--adjoint", action="store_true",
+                    help="time one gradient through the hand-written adjoint instead (forward with checkpoints and "
+                         "the reverse sweep; the 2-flavour 32^3 x 92 workload, 200 steps, checkpoints every 20, "
+                         "unless --workload / --steps say otherwise)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device (5 times): the known-size kernel that scales the "
                          "FETCH_SIZE / WRITE_SIZE counters")
@@ -169,6 +227,9 @@ def main():
     from emu_amd import capi, configs, core
     capi.lib()  # fails loudly without the HIP library
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.adjoint:
+        run_adjoint(args)
+        return
     parms = configs.WORKLOADS[args.workload]()
     emu = core.Emu(parms)
     parts = core.FlavoredNeutrinoContainer.init_particles(emu)

@@ -25,6 +25,8 @@
 #include "emu_stage.h"
 #include "../../include/emu_adjoint.h"
 
+#include <hipcub/hipcub.hpp>
+
 #include <algorithm>
 #include <cmath>
 
@@ -179,21 +181,51 @@ __global__ void k_vjp_comm(AdjIn in, Geom g, const double* grid, HamParams hp, c
     }
 }
 
+// The centre cell of each particle's stencil, and the particle count per cell.
+__global__ void k_cell_keys(AdjIn in, Geom g, int* key, int* counts)
+{
+    const int64_t np = in.np;
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
+        int c[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            int idx;
+            double w[3];
+            shape3((in.pos[d * np + p] - g.plo[d]) * g.dxi[d], g.order[d], idx, w);
+            c[d] = pmod(idx, g.n[d]);
+        }
+        const int k = (c[2] * g.n[1] + c[1]) * g.n[0] + c[0];
+        key[p] = k;
+        atomicAdd(&counts[k], 1);
+    }
+}
+
+// perm[start[key] + i] = p: the particles listed cell by cell (the order within
+// a cell follows the atomics and does not matter to the deposit's sums beyond
+// rounding).
+__global__ void k_cell_perm(int64_t np, const int* key, const int* start, int* cursor, int* perm)
+{
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
+        const int k = key[p];
+        perm[start[k] + atomicAdd(&cursor[k], 1)] = (int)p;
+    }
+}
+
 // The deposit of both directions with the sum over particles on the matrix
 // cores.  KIND 0: the forward q_c = N f_c - conj(Nbar fbar_c) with (1, v);
 // KIND 1: the gather's adjoint mu_c with (1, -v).  Per-particle atomics put
 // every particle of a cell on the same 27 x 4 NH grid entries, so the sum
 // over a cell's particles is done first, as in the fused stage kernel.
 //
-// One wave takes 64 consecutive particles of the fixed order.  Particles
-// whose stencils share their centre cell (the key) share their 27 targets;
-// for each distinct key in the wave, D[t][q] = sum_p W[p][t] R[p][q] runs on
-// f64 16x16x4 MFMAs over the k-steps that span the key's lanes, with the
-// other lanes' sources zeroed, and is then added to the grid once.  The
-// order is arbitrary for correctness; the initial order is cell-major, so a
-// wave usually holds one or two keys.
+// The particles are taken through `perm`, which lists them by the centre cell
+// of their stencil (the key; k_cell_keys and k_cell_perm), 64 per wave.
+// Particles with one key share their 27 targets; for each distinct key in the
+// wave, D[t][q] = sum_p W[p][t] R[p][q] runs on f64 16x16x4 MFMAs over the
+// k-steps that span the key's lanes, with the other lanes' sources zeroed,
+// and is then added to the grid once.  With the cell order a wave holds one
+// or two keys; any order gives the same sums.
 template <int NF, int WX, int WY, int WZ, int KIND>
-__global__ __launch_bounds__(128) void k_seg_deposit(AdjIn in, Geom g, const double* mu, double* grid)
+__global__ __launch_bounds__(128) void k_seg_deposit(AdjIn in, Geom g, const int* perm, const double* mu, double* grid)
 {
     constexpr int nh = NH<NF>, QG = 4 * nh;
     constexpr int NT = WX * WY * WZ, MT = (NT + 15) / 16, NTL = (QG + 15) / 16;
@@ -209,7 +241,7 @@ __global__ __launch_bounds__(128) void k_seg_deposit(AdjIn in, Geom g, const dou
     if (p0 >= np) return; // uniform over the wave
     const int64_t p = p0 + lane;
     const bool act = p < np;
-    const int64_t pl = act ? p : p0;
+    const int64_t pl = perm[act ? p : p0]; // the particles in cell order
     const double pos[3] = {in.pos[pl], in.pos[np + pl], in.pos[2 * np + pl]};
     const double pt = in.pup[3 * np + pl];
     const double v[3] = {in.pup[pl] / pt, in.pup[np + pl] / pt, in.pup[2 * np + pl] / pt};
@@ -453,11 +485,52 @@ int64_t ncell_of(const emu_ctx* ctx) { return (int64_t)ctx->g.n[0] * ctx->g.n[1]
 
 using namespace emu;
 
-extern "C" int64_t emu_adj_work_doubles(const emu_ctx* ctx, int64_t np)
+// work: grid and grid adjoint [2][ncell][4 NH], mu [NH][np], the objective's
+// [2 ncell + 8], then the cell sort: key, perm [np] ints, counts/start
+// [ncell + 1], cursor [ncell] ints, and the scan's temporary storage.
+static int64_t sort_offset_doubles(const emu_ctx* ctx, int64_t np)
 {
-    if (!ctx) return -1;
     const int nh = ctx->p.nf * ctx->p.nf;
     return 2 * ncell_of(ctx) * 4 * nh + (int64_t)nh * np + 2 * ncell_of(ctx) + 8;
+}
+
+static size_t scan_bytes(const emu_ctx* ctx)
+{
+    size_t b = 0;
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, b, (const int*)nullptr, (int*)nullptr, (int)(ncell_of(ctx) + 1),
+                                           ctx->stream);
+    return b;
+}
+
+struct CellSort {
+    int *key, *perm, *start, *cursor;
+    void* tmp;
+    size_t tmp_bytes;
+};
+
+static CellSort cell_sort_space(const emu_ctx* ctx, int64_t np, double* work)
+{
+    const int64_t nc = ncell_of(ctx);
+    CellSort cs;
+    int* base = (int*)(work + sort_offset_doubles(ctx, np));
+    cs.key = base;
+    cs.perm = cs.key + np;
+    cs.start = cs.perm + np;
+    cs.cursor = cs.start + (nc + 1);
+    const int64_t ints = 2 * np + 2 * nc + 1;
+    // the scan's storage on a 256-byte boundary (the allocation reserves the slack)
+    const uintptr_t t = (uintptr_t)(work + sort_offset_doubles(ctx, np) + (ints + 1) / 2);
+    cs.tmp = (void*)((t + 255) & ~(uintptr_t)255);
+    cs.tmp_bytes = scan_bytes(ctx);
+    return cs;
+}
+
+extern "C" int64_t emu_adj_work_doubles(const emu_ctx* ctx, int64_t np)
+{
+    if (!ctx || np < 0) return -1;
+    const int64_t nc = ncell_of(ctx);
+    const int64_t ints = 2 * np + 2 * nc + 1;
+    return sort_offset_doubles(ctx, np) + (ints + 1) / 2 + 32 + (int64_t)(scan_bytes(ctx) + 7) / 8;
 }
 
 #define ADJ_DISPATCH(kernel, ...)                                                                          \
@@ -474,12 +547,12 @@ static double m00_of(const emu_ctx* ctx)
 // k_seg_deposit for the stencil widths of the grid (3 where the shape order is
 // above 0, else 1), 2 waves per workgroup of 128 threads
 template <int NF, int KIND>
-static void seg_deposit_nf(emu_ctx* ctx, const AdjIn& in, const double* mu, double* grid)
+static void seg_deposit_nf(emu_ctx* ctx, const AdjIn& in, const int* perm, const double* mu, double* grid)
 {
     const int wx = ctx->g.order[0] > 0 ? 3 : 1, wy = ctx->g.order[1] > 0 ? 3 : 1, wz = ctx->g.order[2] > 0 ? 3 : 1;
     const unsigned nb = (unsigned)((in.np + 127) / 128);
     const int code = (wx == 3) * 4 + (wy == 3) * 2 + (wz == 3);
-#define SEG_LAUNCH(X, Y, Z) hipLaunchKernelGGL((k_seg_deposit<NF, X, Y, Z, KIND>), dim3(nb), dim3(128), 0, ctx->stream, in, ctx->g, mu, grid)
+#define SEG_LAUNCH(X, Y, Z) hipLaunchKernelGGL((k_seg_deposit<NF, X, Y, Z, KIND>), dim3(nb), dim3(128), 0, ctx->stream, in, ctx->g, perm, mu, grid)
     switch (code) {
     case 0: SEG_LAUNCH(1, 1, 1); break;
     case 1: SEG_LAUNCH(1, 1, 3); break;
@@ -493,19 +566,34 @@ static void seg_deposit_nf(emu_ctx* ctx, const AdjIn& in, const double* mu, doub
 #undef SEG_LAUNCH
 }
 
-template <int KIND> static void seg_deposit(emu_ctx* ctx, const AdjIn& in, const double* mu, double* grid)
+template <int KIND>
+static void seg_deposit(emu_ctx* ctx, const AdjIn& in, const int* perm, const double* mu, double* grid)
 {
     if (in.np == 0) return;
-    if (ctx->p.nf == 2) seg_deposit_nf<2, KIND>(ctx, in, mu, grid);
-    else seg_deposit_nf<3, KIND>(ctx, in, mu, grid);
+    if (ctx->p.nf == 2) seg_deposit_nf<2, KIND>(ctx, in, perm, mu, grid);
+    else seg_deposit_nf<3, KIND>(ctx, in, perm, mu, grid);
 }
 
-static void deposit(emu_ctx* ctx, const AdjIn& in, double* grid)
+// the particles listed by the centre cell of their stencil (positions are
+// frozen within a step, so every deposit of a call shares one list)
+static int cell_sort(emu_ctx* ctx, const AdjIn& in, const CellSort& cs)
+{
+    const int64_t np = in.np, nc = ncell_of(ctx);
+    const int64_t zero_doubles = (2 * nc + 1 + 1) / 2; // start and cursor are adjacent ints
+    hipLaunchKernelGGL(k_zero, dim3(nblk(zero_doubles)), dim3(kB), 0, ctx->stream, (double*)cs.start, zero_doubles);
+    hipLaunchKernelGGL(k_cell_keys, dim3(nblk(np)), dim3(kB), 0, ctx->stream, in, ctx->g, cs.key, cs.start);
+    size_t bytes = cs.tmp_bytes;
+    EMU_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(cs.tmp, bytes, cs.start, cs.start, (int)(nc + 1), ctx->stream));
+    hipLaunchKernelGGL(k_cell_perm, dim3(nblk(np)), dim3(kB), 0, ctx->stream, np, cs.key, cs.start, cs.cursor, cs.perm);
+    return EMU_OK;
+}
+
+static void deposit(emu_ctx* ctx, const AdjIn& in, const CellSort& cs, double* grid)
 {
     const int64_t ng = ncell_of(ctx) * 4 * ctx->p.nf * ctx->p.nf;
     if (ctx->p.nf == 2) hipLaunchKernelGGL(k_fill_base<2>, dim3(nblk(ng)), dim3(kB), 0, ctx->stream, grid, ncell_of(ctx), m00_of(ctx));
     else hipLaunchKernelGGL(k_fill_base<3>, dim3(nblk(ng)), dim3(kB), 0, ctx->stream, grid, ncell_of(ctx), m00_of(ctx));
-    seg_deposit<0>(ctx, in, nullptr, grid);
+    seg_deposit<0>(ctx, in, cs.perm, nullptr, grid);
 }
 
 extern "C" int emu_adj_rhs(emu_ctx* ctx, int64_t np, const double* pos, const double* pup, const double* Nn,
@@ -515,7 +603,9 @@ extern "C" int emu_adj_rhs(emu_ctx* ctx, int64_t np, const double* pos, const do
     if (np == 0) return EMU_OK;
     const AdjIn in{np, pos, pup, Nn, Y};
     double* grid = work;
-    deposit(ctx, in, grid);
+    const CellSort cs = cell_sort_space(ctx, np, work);
+    if (int e = cell_sort(ctx, in, cs)) return e;
+    deposit(ctx, in, cs, grid);
     ADJ_DISPATCH(k_rhs, in, ctx->g, grid, ham_params(ctx), k);
     EMU_LAUNCH_CHECK(ctx, "adj_rhs");
     return EMU_OK;
@@ -532,10 +622,12 @@ extern "C" int emu_adj_rhs_vjp(emu_ctx* ctx, int64_t np, const double* pos, cons
     double* grid = work;
     double* gadj = work + ng;
     double* mu = gadj + ng;
-    deposit(ctx, in, grid);
+    const CellSort cs = cell_sort_space(ctx, np, work);
+    if (int e = cell_sort(ctx, in, cs)) return e;
+    deposit(ctx, in, cs, grid);
     ADJ_DISPATCH(k_vjp_comm, in, ctx->g, grid, ham_params(ctx), lam_k, lam_Y, mu);
     hipLaunchKernelGGL(k_zero, dim3(nblk(ng)), dim3(kB), 0, ctx->stream, gadj, ng);
-    seg_deposit<1>(ctx, in, mu, gadj);
+    seg_deposit<1>(ctx, in, cs.perm, mu, gadj);
     ADJ_DISPATCH(k_vjp_gather, in, ctx->g, gadj, lam_Y, lam_Nbar);
     EMU_LAUNCH_CHECK(ctx, "adj_rhs_vjp");
     return EMU_OK;
