@@ -37,30 +37,34 @@ constexpr int kB = 256;
 
 unsigned nblk(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + kB - 1) / kB, 1 << 20)); }
 
-struct Stencil {
-    int64_t cell[27];
-    double w[27];
-    int n;
-};
-
-__device__ __forceinline__ void stencil_of(const Geom& g, const double pos[3], Stencil& S)
+// f(cell, weight) over a particle's stencil in the reference's order (k outer,
+// then j, then i; Evolve.cpp:118-124): 3 points per dimension where the shape
+// order is above 0, else the particle's own cell.  Unrolled, with the wrapped
+// indices of each dimension computed once, so nothing is indexed at run time.
+template <typename F> __device__ __forceinline__ void for_stencil(const Geom& g, const double pos[3], F&& f)
 {
-    int idx[3];
+    int idx[3], cw[3][3];
     double wd[3][3];
-    for (int d = 0; d < 3; d++) shape3((pos[d] - g.plo[d]) * g.dxi[d], g.order[d], idx[d], wd[d]);
-    const int wx = g.order[0] > 0 ? 3 : 1, wy = g.order[1] > 0 ? 3 : 1, wz = g.order[2] > 0 ? 3 : 1;
-    int n = 0;
-    for (int kk = 0; kk < wz; kk++)
-        for (int jj = 0; jj < wy; jj++)
-            for (int ii = 0; ii < wx; ii++) {
-                const int ox = wx == 1 ? 1 : ii, oy = wy == 1 ? 1 : jj, oz = wz == 1 ? 1 : kk;
-                const int ci = pmod(idx[0] + ox - 1, g.n[0]), cj = pmod(idx[1] + oy - 1, g.n[1]),
-                          ck = pmod(idx[2] + oz - 1, g.n[2]);
-                S.cell[n] = ((int64_t)ck * g.n[1] + cj) * g.n[0] + ci;
-                S.w[n] = wd[0][ox] * wd[1][oy] * wd[2][oz];
-                n++;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        shape3((pos[d] - g.plo[d]) * g.dxi[d], g.order[d], idx[d], wd[d]);
+#pragma unroll
+        for (int o = 0; o < 3; o++) cw[d][o] = pmod(idx[d] + o - 1, g.n[d]);
+    }
+    const bool w3[3] = {g.order[0] > 0, g.order[1] > 0, g.order[2] > 0};
+#pragma unroll
+    for (int kk = 0; kk < 3; kk++) {
+        if (!w3[2] && kk != 1) continue;
+#pragma unroll
+        for (int jj = 0; jj < 3; jj++) {
+            if (!w3[1] && jj != 1) continue;
+#pragma unroll
+            for (int ii = 0; ii < 3; ii++) {
+                if (!w3[0] && ii != 1) continue;
+                f(((int64_t)cw[2][kk] * g.n[1] + cw[1][jj]) * g.n[0] + cw[0][ii], wd[0][ii] * wd[1][jj] * wd[2][kk]);
             }
-    S.n = n;
+        }
+    }
 }
 
 struct AdjIn {
@@ -87,7 +91,7 @@ struct HamParams {
 // H/hbar and Hbar/hbar of a particle from the deposited grid
 template <int NF>
 __device__ __forceinline__ void hamiltonians(const AdjIn& in, int64_t p, const Geom& g, const double* grid,
-                                             const HamParams& hp, Stencil& S, double v[3], double* H, double* Hb)
+                                             const HamParams& hp, double v[3], double* H, double* Hb)
 {
     constexpr int nh = NH<NF>, QG = 4 * nh;
     const int64_t np = in.np;
@@ -96,15 +100,15 @@ __device__ __forceinline__ void hamiltonians(const AdjIn& in, int64_t p, const G
     v[0] = in.pup[p] / pt;
     v[1] = in.pup[np + p] / pt;
     v[2] = in.pup[2 * np + p] / pt;
-    stencil_of(g, pos, S);
     double inside[nh];
+#pragma unroll
     for (int c = 0; c < nh; c++) inside[c] = 0.0;
-    for (int t = 0; t < S.n; t++) {
-        const double* G = grid + S.cell[t] * QG;
-        const double w = S.w[t];
+    for_stencil(g, pos, [&](int64_t cell, double w) {
+        const double* G = grid + cell * QG;
+#pragma unroll
         for (int c = 0; c < nh; c++)
             inside[c] += w * (G[c] - v[0] * G[nh + c] - v[1] * G[2 * nh + c] - v[2] * G[3 * nh + c]);
-    }
+    });
     for (int c = 0; c < nh; c++) {
         const double hv = hp.vac[c] / pt, hs = hp.coef * inside[c];
         H[c] = hv + hs;
@@ -112,15 +116,17 @@ __device__ __forceinline__ void hamiltonians(const AdjIn& in, int64_t p, const G
     }
 }
 
+// (perm, when given, takes the particles in cell order: the lanes of a wave then
+// read the same grid cells)
 template <int NF>
-__global__ void k_rhs(AdjIn in, Geom g, const double* grid, HamParams hp, double* k)
+__global__ __launch_bounds__(256) void k_rhs(AdjIn in, Geom g, const int* perm, const double* grid, HamParams hp, double* k)
 {
     constexpr int nh = NH<NF>;
     const int64_t np = in.np;
-    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
-        Stencil S;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < np; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t p = perm ? perm[i] : i;
         double v[3], H[nh], Hb[nh], f[nh], out[nh];
-        hamiltonians<NF>(in, p, g, grid, hp, S, v, H, Hb);
+        hamiltonians<NF>(in, p, g, grid, hp, v, H, Hb);
         for (int c = 0; c < nh; c++) f[c] = in.Y[(int64_t)c * np + p];
         commutator_rhs<NF, double>(H, f, out, 1.0);
         for (int c = 0; c < nh; c++) k[(int64_t)c * np + p] = out[c];
@@ -153,15 +159,15 @@ template <int NF> __device__ __forceinline__ void double_offdiag(double* x)
 // VJP part 1: commutator adjoints.  lamY <- dL/dY through [H, .];
 // mu <- the adjoint of the gathered field (per component).
 template <int NF>
-__global__ void k_vjp_comm(AdjIn in, Geom g, const double* grid, HamParams hp, const double* lamk, double* lamY,
-                           double* mu)
+__global__ __launch_bounds__(256) void k_vjp_comm(AdjIn in, Geom g, const int* perm, const double* grid, HamParams hp, const double* lamk,
+                           double* lamY, double* mu)
 {
     constexpr int nh = NH<NF>;
     const int64_t np = in.np;
-    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
-        Stencil S;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < np; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t p = perm ? perm[i] : i;
         double v[3], H[nh], Hb[nh];
-        hamiltonians<NF>(in, p, g, grid, hp, S, v, H, Hb);
+        hamiltonians<NF>(in, p, g, grid, hp, v, H, Hb);
         double gH[2][nh];
         for (int bar = 0; bar < 2; bar++) {
             double lam[nh], L[nh], f[nh], gy[nh];
@@ -344,23 +350,24 @@ __global__ __launch_bounds__(128) void k_seg_deposit(AdjIn in, Geom g, const int
 
 // VJP part 3: gather of the grid adjoint (adjoint of the deposit).
 template <int NF>
-__global__ void k_vjp_gather(AdjIn in, Geom g, const double* gadj, double* lamY, double* lamNbar)
+__global__ __launch_bounds__(256) void k_vjp_gather(AdjIn in, Geom g, const int* perm, const double* gadj, double* lamY, double* lamNbar)
 {
     constexpr int nh = NH<NF>, QG = 4 * nh;
     const int64_t np = in.np;
-    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < np; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t p = perm ? perm[i] : i;
         const double pos[3] = {in.pos[p], in.pos[np + p], in.pos[2 * np + p]};
         const double pt = in.pup[3 * np + p];
         const double v[3] = {in.pup[p] / pt, in.pup[np + p] / pt, in.pup[2 * np + p] / pt};
-        Stencil S;
-        stencil_of(g, pos, S);
         double lq[nh];
+#pragma unroll
         for (int c = 0; c < nh; c++) lq[c] = 0.0;
-        for (int t = 0; t < S.n; t++) {
-            const double* G = gadj + S.cell[t] * QG;
+        for_stencil(g, pos, [&](int64_t cell, double w) {
+            const double* G = gadj + cell * QG;
+#pragma unroll
             for (int c = 0; c < nh; c++)
-                lq[c] += S.w[t] * (G[c] + v[0] * G[nh + c] + v[1] * G[2 * nh + c] + v[2] * G[3 * nh + c]);
-        }
+                lq[c] += w * (G[c] + v[0] * G[nh + c] + v[1] * G[2 * nh + c] + v[2] * G[3 * nh + c]);
+        });
         const double N = in.Nn[p], Nb = in.Nn[np + p];
         double ln = 0.0;
         for (int c = 0; c < nh; c++) {
@@ -380,14 +387,12 @@ __global__ void k_obj_deposit(AdjIn in, Geom g, double* g2)
     const int64_t np = in.np;
     for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
         const double pos[3] = {in.pos[p], in.pos[np + p], in.pos[2 * np + p]};
-        Stencil S;
-        stencil_of(g, pos, S);
         const double N = in.Nn[p];
         const double re = N * in.Y[(int64_t)herm_re<NF>(0, 1) * np + p], im = N * in.Y[(int64_t)herm_im<NF>(0, 1) * np + p];
-        for (int t = 0; t < S.n; t++) {
-            unsafeAtomicAdd(&g2[2 * S.cell[t]], S.w[t] * re);
-            unsafeAtomicAdd(&g2[2 * S.cell[t] + 1], S.w[t] * im);
-        }
+        for_stencil(g, pos, [&](int64_t cell, double w) {
+            unsafeAtomicAdd(&g2[2 * cell], w * re);
+            unsafeAtomicAdd(&g2[2 * cell + 1], w * im);
+        });
     }
 }
 
@@ -406,13 +411,11 @@ __global__ void k_obj_grad(AdjIn in, Geom g, const double* g2, double* dJdY)
     const int64_t np = in.np;
     for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
         const double pos[3] = {in.pos[p], in.pos[np + p], in.pos[2 * np + p]};
-        Stencil S;
-        stencil_of(g, pos, S);
         double re = 0.0, im = 0.0;
-        for (int t = 0; t < S.n; t++) {
-            re += S.w[t] * g2[2 * S.cell[t]];
-            im += S.w[t] * g2[2 * S.cell[t] + 1];
-        }
+        for_stencil(g, pos, [&](int64_t cell, double w) {
+            re += w * g2[2 * cell];
+            im += w * g2[2 * cell + 1];
+        });
         const double N = in.Nn[p];
         for (int c = 0; c < 2 * NH<NF>; c++) dJdY[(int64_t)c * np + p] = 0.0;
         dJdY[(int64_t)herm_re<NF>(0, 1) * np + p] = 2.0 * N * re;
@@ -606,7 +609,7 @@ extern "C" int emu_adj_rhs(emu_ctx* ctx, int64_t np, const double* pos, const do
     const CellSort cs = cell_sort_space(ctx, np, work);
     if (int e = cell_sort(ctx, in, cs)) return e;
     deposit(ctx, in, cs, grid);
-    ADJ_DISPATCH(k_rhs, in, ctx->g, grid, ham_params(ctx), k);
+    ADJ_DISPATCH(k_rhs, in, ctx->g, cs.perm, grid, ham_params(ctx), k);
     EMU_LAUNCH_CHECK(ctx, "adj_rhs");
     return EMU_OK;
 }
@@ -625,10 +628,10 @@ extern "C" int emu_adj_rhs_vjp(emu_ctx* ctx, int64_t np, const double* pos, cons
     const CellSort cs = cell_sort_space(ctx, np, work);
     if (int e = cell_sort(ctx, in, cs)) return e;
     deposit(ctx, in, cs, grid);
-    ADJ_DISPATCH(k_vjp_comm, in, ctx->g, grid, ham_params(ctx), lam_k, lam_Y, mu);
+    ADJ_DISPATCH(k_vjp_comm, in, ctx->g, cs.perm, grid, ham_params(ctx), lam_k, lam_Y, mu);
     hipLaunchKernelGGL(k_zero, dim3(nblk(ng)), dim3(kB), 0, ctx->stream, gadj, ng);
     seg_deposit<1>(ctx, in, cs.perm, mu, gadj);
-    ADJ_DISPATCH(k_vjp_gather, in, ctx->g, gadj, lam_Y, lam_Nbar);
+    ADJ_DISPATCH(k_vjp_gather, in, ctx->g, cs.perm, gadj, lam_Y, lam_Nbar);
     EMU_LAUNCH_CHECK(ctx, "adj_rhs_vjp");
     return EMU_OK;
 }
