@@ -27,10 +27,13 @@ def setup(mods, oracle, nf, ncell=(4, 4, 4), nphi=4, amp=1e-4, **kw):
     return emu, c, deck, parts
 
 
+@pytest.mark.parametrize("order", [0, 1, 2])
 @pytest.mark.parametrize("nf", [2, 3])
-def test_adj_rhs_matches_oracle(mods, oracle, nf):
+def test_adj_rhs_matches_oracle(mods, oracle, nf, order):
+    """The adjoint's forward RHS (cell-listed MFMA deposit, unrolled gather) at
+    each shape order against the oracle's deposit + interpolation."""
     torch, core, adjoint = mods
-    emu, c, deck, parts = setup(mods, oracle, nf, rho_g_ccm=1e9, Ye=0.5)
+    emu, c, deck, parts = setup(mods, oracle, nf, rho_g_ccm=1e9, Ye=0.5, shape_order=order)
     prob = adjoint.AdjointProblem(emu, c, dt=1e-13, nsteps=1)
     k = prob.rhs(prob.pos0, prob.Nn_base, prob.f0).cpu().numpy()
     prm = deck.params()

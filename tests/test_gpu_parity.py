@@ -255,6 +255,49 @@ def test_fused_fp64_parity(core, oracle, nf, ncell):
     assert info["time"] == pytest.approx(want["rdata"][0, 0], rel=1e-15)
 
 
+@pytest.mark.parametrize("order", [0, 1])
+@pytest.mark.parametrize("nf", [2, 3])
+def test_lower_shape_orders_parity(core, oracle, nf, order):
+    """SHAPE_FACTOR_ORDER 0 (nearest cell: 1-point stencils) and 1 (cloud in
+    cell: the 3-point stencils with one zero weight) on the per-function
+    deposit and interpolation and on 6 fused FP64 steps with a dt that moves
+    particles across cells, against the oracle at the same order."""
+    d = cases.fast_flavor_dict(nf=nf, ncell=(4, 3, 5), L=(2.0, 1.5, 2.5), nphi=8, shape_order=order,
+                               st4_amplitude=1e-3)
+    deck, prm, emu = make(core, oracle, d)
+    parts = oracle.init_particles(deck)
+    c = core.FlavoredNeutrinoContainer.from_aos(emu, parts)
+    g_or = oracle.Grid(nf, prm)
+    oracle.deposit_to_mesh(nf, prm, parts, g_or)
+    oracle.fill_boundary(nf, prm, g_or)
+    state = core.MultiFab(emu)
+    core.deposit_to_mesh(c, state)
+    state.FillBoundary()
+    fab = state.data.cpu().numpy().reshape(emu.ncomp, -1)
+    want = g_or.fab().reshape(emu.ncomp, -1)
+    # the isotropic direction set's net fluxes cancel to rounding (~1e-16 of N), so
+    # N and F are measured against the largest of them (|F| <= N)
+    np.testing.assert_array_equal(fab[:3], want[:3])
+    assert np.abs(fab[3:] - want[3:]).max() / np.abs(want[3:]).max() < 1e-12
+    state.data.copy_(__import__("torch").from_numpy(g_or.fab().copy()))
+    rhs_or = parts.copy()
+    oracle.interpolate_rhs_from_mesh(nf, prm, rhs_or, g_or)
+    rhs = core.FlavoredNeutrinoContainer.from_aos(emu, parts)
+    core.interpolate_rhs_from_mesh(rhs, state)
+    assert cases.max_rel_err(rhs.to_aos()["rdata"], rhs_or["rdata"]) < 1e-12
+    fixed = 0.3 * 0.5 / oracle.clight
+    st = core.FusedState(emu, c.np, "fp64")
+    st.load(c)
+    st.evolve(6, fixed_dt=fixed)
+    st.store(c)
+    st.close()
+    oracle_steps(oracle, nf, prm, parts, 6, fixed_dt=fixed)
+    got = cases.by_id(c.to_aos())
+    want = cases.by_id(parts)
+    np.testing.assert_array_equal(got["pos"], want["pos"])
+    assert cases.max_rel_err(got["rdata"], want["rdata"]) < 1e-12
+
+
 @pytest.mark.parametrize("nf", [2, 3])
 def test_fused_fp64_computed_dt(core, oracle, nf):
     d = cases.fast_flavor_dict(nf=nf, ncell=(4, 4, 4), nphi=4)

@@ -206,20 +206,30 @@ def random_particles(oracle, nf, ncell, L, n, rng, isotropic_pairs=False):
 
 
 def bspline_weights(delta, order, n):
-    """Independent periodic deposit weights on cell centres."""
+    """Independent periodic deposit weights on cell centres: the quadratic
+    B-spline (order 2), the cloud-in-cell hat (order 1) or the nearest cell
+    (order 0), centred on the particle."""
     w = np.zeros(n)
     for T in range(-3, n + 3):
         x = delta - (T + 0.5)
-        v = bspline2(x) if order == 2 else (1.0 if (order == 0 and T == math.floor(delta)) else 0.0)
+        if order == 2:
+            v = bspline2(x)
+        elif order == 1:
+            v = max(0.0, 1.0 - abs(x))
+        else:
+            v = 1.0 if T == math.floor(delta) else 0.0
         w[T % n] += v
     return w
 
 
+@pytest.mark.parametrize("order", [0, 1, 2])
 @pytest.mark.parametrize("nf", [2, 3])
-def test_deposit_brute_force_and_conservation(oracle, nf):
-    rng = np.random.default_rng(11)
+def test_deposit_brute_force_and_conservation(oracle, nf, order):
+    rng = np.random.default_rng(11 + order)
     ncell, L = (4, 3, 5), (1.0, 0.75, 1.25)
     d, p, parts, K = random_particles(oracle, nf, ncell, L, 60, rng)
+    d.shape_order = order
+    p = d.params()
     g = oracle.Grid(nf, p)
     oracle.deposit_to_mesh(nf, p, parts, g)
     valid = g.valid()  # [comp, k, j, i]
@@ -227,9 +237,9 @@ def test_deposit_brute_force_and_conservation(oracle, nf):
     want = np.zeros_like(valid)
     dx = np.array(L) / np.array(ncell)
     for q in parts:
-        wx = bspline_weights(q["pos"][0] / dx[0], 2, ncell[0])
-        wy = bspline_weights(q["pos"][1] / dx[1], 2, ncell[1])
-        wz = bspline_weights(q["pos"][2] / dx[2], 2, ncell[2])
+        wx = bspline_weights(q["pos"][0] / dx[0], order, ncell[0])
+        wy = bspline_weights(q["pos"][1] / dx[1], order, ncell[1])
+        wz = bspline_weights(q["pos"][2] / dx[2], order, ncell[2])
         W = wz[:, None, None] * wy[None, :, None] * wx[None, None, :]
         r = q["rdata"]
         vhat = r[K["pupx"]:K["pupx"] + 3] / r[K["pupt"]]
