@@ -1,0 +1,134 @@
+"""Edge cases of the flavor step on the GPU, against the oracle: no particles
+at all, ragged cell occupancy around the fused kernel's 64-particle chunks
+(empty cells, single particles, 63/64/65 and 127/128/129 particles), and one
+crowded cell that spans seven chunks."""
+import numpy as np
+import pytest
+
+import cases
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def core():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    from emu_amd import core as c
+    return c
+
+
+def make(core, oracle, d):
+    deck = cases.oracle_deck(oracle, d)
+    return deck, deck.params(), core.Emu(cases.product_params(d))
+
+
+def placed(oracle, deck, ncell, L, occupancy, seed):
+    """Valid particles of the deck's initial conditions (normalised flavour
+    states, their lengths, weights and directions), moved into the cells of
+    `occupancy` (linear cell index -> count) at uniform positions inside each."""
+    src = oracle.init_particles(deck)
+    n = sum(occupancy.values())
+    assert n <= len(src)
+    parts = src[:n].copy()
+    rng = np.random.default_rng(seed)
+    dx = np.array(L) / np.array(ncell)
+    i = 0
+    for cell, cnt in sorted(occupancy.items()):
+        cx, cy, cz = cell % ncell[0], (cell // ncell[0]) % ncell[1], cell // (ncell[0] * ncell[1])
+        for _ in range(cnt):
+            pos = (np.array([cx, cy, cz]) + rng.uniform(0.05, 0.95, 3)) * dx
+            parts["pos"][i] = pos
+            parts["rdata"][i, 1:4] = pos
+            i += 1
+    parts["id"] = np.arange(1, n + 1)
+    return parts
+
+
+def run_fused_against_oracle(core, oracle, nf, prm, emu, parts, prec, nsteps, fixed):
+    c = core.FlavoredNeutrinoContainer.from_aos(emu, parts)
+    st = core.FusedState(emu, c.np, prec)
+    st.load(c)
+    st.evolve(nsteps, fixed_dt=fixed)
+    st.store(c)
+    info = st.info()
+    st.close()
+    g = oracle.Grid(nf, prm)
+    vv = oracle.vvac_max(nf, prm, parts)
+    oracle.deposit_to_mesh(nf, prm, parts, g)
+    for _ in range(nsteps):
+        err, _, bad = oracle.rk4_step(nf, prm, parts, g, fixed, vv)
+        assert err == 0
+    got = cases.by_id(c.to_aos())
+    want = cases.by_id(parts)
+    np.testing.assert_array_equal(got["id"], want["id"])
+    if prec == "fp64":
+        np.testing.assert_array_equal(got["pos"], want["pos"])
+        assert cases.max_rel_err(got["rdata"], want["rdata"]) < 1e-12
+    else:
+        # FP32 bulk keeps the momenta in FP32: the FP64 position update moves
+        # along a direction rounded to ~2^-23, so the position error is bounded
+        # by that fraction of the distance travelled (absolute, not relative:
+        # coordinates near 0 have no relative bound)
+        travel = nsteps * fixed * oracle.clight
+        assert np.abs(got["pos"] - want["pos"]).max() <= 2.0 ** -22 * travel
+        assert cases.max_rel_err(got["rdata"], want["rdata"]) < 2e-5
+    assert info["steps"] == nsteps
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("nf", [2, 3])
+def test_ragged_occupancy(core, oracle, nf, prec):
+    """Cells holding 0, 1, 2, 63, 64, 65, 127, 128, 129, 200, 17, 5 or 40
+    particles, 4 steps with a dt that moves particles across cells."""
+    ncell, L = (4, 4, 4), (2.0, 2.0, 2.0)
+    d = cases.fast_flavor_dict(nf=nf, ncell=ncell, L=L, nphi=16, st4_amplitude=1e-3)
+    deck, prm, emu = make(core, oracle, d)
+    counts = [0, 1, 2, 63, 64, 65, 127, 128, 129, 200, 0, 0, 17, 5, 0, 40]
+    occ = {cell: counts[cell % len(counts)] for cell in range(64)}
+    parts = placed(oracle, deck, ncell, L, occ, seed=5)
+    run_fused_against_oracle(core, oracle, nf, prm, emu, parts, prec, 4, 0.3 * 0.5 / oracle.clight)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_crowded_cell(core, oracle, prec):
+    """400 particles in one cell (six full chunks and a partial one), its
+    neighbours empty, and a few particles elsewhere."""
+    nf, ncell, L = 3, (3, 3, 3), (1.5, 1.5, 1.5)
+    d = cases.fast_flavor_dict(nf=nf, ncell=ncell, L=L, nphi=16, st4_amplitude=1e-3)
+    deck, prm, emu = make(core, oracle, d)
+    parts = placed(oracle, deck, ncell, L, {13: 400, 0: 3, 26: 1}, seed=9)
+    run_fused_against_oracle(core, oracle, nf, prm, emu, parts, prec, 3, 0.3 * 0.5 / oracle.clight)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_no_particles(core, oracle, prec):
+    """An empty container: the deposit leaves only the background fields, the
+    flavour bound of compute_dt has nothing to bound (the step is the
+    translation bound, as in the reference's compute_dt), and the fused state
+    steps through time with nothing to move."""
+    nf, ncell, L = 2, (3, 3, 3), (1.5, 1.5, 1.5)
+    d = cases.fast_flavor_dict(nf=nf, ncell=ncell, L=L, nphi=4)
+    deck, prm, emu = make(core, oracle, d)
+    empty = oracle.init_particles(deck)[:0]
+    c = core.FlavoredNeutrinoContainer.from_aos(emu, empty)
+    assert c.np == 0
+    state = core.MultiFab(emu)
+    core.deposit_to_mesh(c, state)
+    state.FillBoundary()
+    g = oracle.Grid(nf, prm)
+    oracle.deposit_to_mesh(nf, prm, empty, g)
+    oracle.fill_boundary(nf, prm, g)
+    np.testing.assert_array_equal(state.data.cpu().numpy().reshape(-1), g.fab().reshape(-1))
+    err, dt_or = oracle.compute_dt(nf, prm, g, 0.0)
+    assert err == 0 and dt_or == pytest.approx(0.4 * 0.5 / oracle.clight, rel=1e-14)
+    assert core.compute_dt(state, 0.0) == pytest.approx(dt_or, rel=1e-14)
+    c.Renormalize()
+    st = core.FusedState(emu, 0, prec)
+    st.load(c)
+    st.evolve(2)
+    info = st.info()
+    st.store(c)
+    st.close()
+    assert info["steps"] == 2
+    assert info["time"] == pytest.approx(2 * dt_or, rel=1e-14)
