@@ -16,6 +16,7 @@ EMU_OK = 0
 ERRORS = {1: "trace", 2: "negdiag", 3: "length", 4: "dt", 10: "arg", 11: "hip", 12: "unsupported",
           13: "nomem", 14: "move"}
 PLAIN_FP64, FP32_BULK = 0, 1
+SLOTS_PACKED, SLOTS_ALIGNED = 0x100, 0x200  # or-ed into emu_state_create's precision
 DIRS_SPHERE, DIRS_CUBE26, DIRS_FIBONACCI = 0, 1, 2
 
 

@@ -277,9 +277,11 @@ class Integrator:
 
 
 class FusedState:
-    """emu_state: the fused, cell-sorted RK4 step (plain FP64 or FP32-bulk)."""
+    """emu_state: the fused, cell-sorted RK4 step (plain FP64 or FP32-bulk).
+    slots: "auto", "packed" (cells' particles back to back) or "aligned" (each
+    cell's run starts on a 128-byte boundary); results are the same."""
 
-    def __init__(self, emu: Emu, np_: int, precision: str = "fp64"):
+    def __init__(self, emu: Emu, np_: int, precision: str = "fp64", slots: str = "auto"):
         from .params import TABLEAUS
         if emu.parms.tableau() != TABLEAUS[4]:
             raise ValueError("the fused step is classical RK4 (integration.type = 1, integration.rk.type = 4); "
@@ -287,8 +289,10 @@ class FusedState:
         self.emu = emu
         self.np = int(np_)
         self.precision = {"fp64": capi.PLAIN_FP64, "fp32": capi.FP32_BULK}[precision]
+        layout = {"auto": 0, "packed": capi.SLOTS_PACKED, "aligned": capi.SLOTS_ALIGNED}[slots]
         st = ctypes.c_void_p()
-        check(lib().emu_state_create(emu.ctx, self.np, self.precision, ctypes.byref(st)), "emu_state_create", emu.ctx)
+        check(lib().emu_state_create(emu.ctx, self.np, self.precision | layout, ctypes.byref(st)), "emu_state_create",
+              emu.ctx)
         self.st = st
 
     def close(self):

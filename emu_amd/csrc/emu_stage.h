@@ -68,14 +68,15 @@ template <typename T> struct StageArgs {
     const int* mbase;
     const int* rank;
     const unsigned char* mv;
-    const int* cell_start;
+    const int* cell_start; // first slot of each cell (a multiple of the slot alignment)
+    const int* cell_cnt;   // particles of each cell
     const double* gin;  // grid read by the gather (FP64 path)
     const float* gin32; // its FP32 copy (FP32-bulk path)
     double* gout;       // grid deposited by this launch
     double* aux;        // compute_dt auxiliaries
     const double* dt;
     unsigned long long* err;
-    int64_t np;
+    int64_t np; // slots per array: the stride of the component-major arrays
     int64_t ncell;
     int stage; // 0 = deposit only, 1..4
     int want_aux;
@@ -291,7 +292,7 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
     const int cell = inside_domain ? (cz * g.n[1] + cy) * g.n[0] + cx : 0;
     // (wave-uniform, so the particle arrays are addressed from scalar bases)
     const int beg = __builtin_amdgcn_readfirstlane(inside_domain ? a.cell_start[cell] : 0);
-    const int end = __builtin_amdgcn_readfirstlane(inside_domain ? a.cell_start[cell + 1] : 0);
+    const int end = __builtin_amdgcn_readfirstlane(inside_domain ? beg + a.cell_cnt[cell] : 0);
     // cell i of the neighbourhood box, box index (i, j, k) <-> the patch base
     // minus one in each 3-wide dimension, plus (i, j, k), periodic
     // (one conditional wrap covers a single cell's box: offsets are -1..+1 and a

@@ -74,14 +74,14 @@ __device__ __forceinline__ int move1(int from, int to, int n, int* bad)
 }
 
 // One wave per old cell: move index and rank within (cell, move); m[c][e].
-__global__ void k_move_ranks(const int* __restrict__ newcell, const int* __restrict__ cell_start, Geom g,
-                             int* __restrict__ rank, unsigned char* __restrict__ mv, int* __restrict__ m,
-                             unsigned long long* err)
+__global__ void k_move_ranks(const int* __restrict__ newcell, const int* __restrict__ cell_start,
+                             const int* __restrict__ cell_cnt, Geom g, int* __restrict__ rank,
+                             unsigned char* __restrict__ mv, int* __restrict__ m, unsigned long long* err)
 {
     const int c = blockIdx.x;
     const int lane = threadIdx.x;
     const int cx = c % g.n[0], cy = (c / g.n[0]) % g.n[1], cz = c / (g.n[0] * g.n[1]);
-    const int beg = cell_start[c], end = cell_start[c + 1];
+    const int beg = cell_start[c], end = beg + cell_cnt[c];
     int running = 0; // lane e (< 27) holds the running count of move e
     for (int base = beg; base < end; base += 64) {
         const int ip = base + lane;
@@ -131,15 +131,19 @@ __device__ __forceinline__ int64_t source_of(int t, int e, const Geom& g)
     return ((int64_t)sz * g.n[1] + sy) * g.n[0] + sx;
 }
 
-// counts[t] = number of particles arriving in t.
-__global__ void k_new_counts(const int* __restrict__ m, Geom g, int64_t ncell, int* __restrict__ counts)
+// cnt[t] = number of particles arriving in t, counts[t] = the slots they take
+// (cnt rounded up to a multiple of align); both 0 at t = ncell, for the scans.
+__global__ void k_new_counts(const int* __restrict__ m, Geom g, int64_t ncell, int align, int* __restrict__ counts,
+                             int* __restrict__ cnt)
 {
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (t >= ncell) return;
+    if (t > ncell) return;
     int n = 0;
-    for (int e = 0; e < 27; e++)
-        if (valid_move(e, g)) n += m[source_of((int)t, e, g) * 27 + e];
-    counts[t] = n;
+    if (t < ncell)
+        for (int e = 0; e < 27; e++)
+            if (valid_move(e, g)) n += m[source_of((int)t, e, g) * 27 + e];
+    cnt[t] = n;
+    counts[t] = (n + align - 1) / align * align;
 }
 
 // base[s][e] for every source s = t - e, in canonical move order.
@@ -286,51 +290,76 @@ __global__ void k_iota(int* p, int64_t n)
         p[t] = (int)t;
 }
 
-// Load: sorted slot j takes input particle order[j] (stable radix sort by cell).
+// Load: cnt[c] = particles binned in c; counts[c] becomes the slots they take
+// (rounded up to a multiple of align).  counts[ncell] and cnt[ncell] stay 0.
+__global__ void k_pad_counts(int* __restrict__ counts, int* __restrict__ cnt, int64_t ncell, int align)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t <= ncell; t += (int64_t)gridDim.x * blockDim.x) {
+        const int n = t < ncell ? counts[t] : 0;
+        cnt[t] = n;
+        counts[t] = (n + align - 1) / align * align;
+    }
+}
+
+// Load: the j-th particle in (cell, input index) order is input particle
+// order[j] of cell key[j]; it takes slot start[key] + (j - ustart[key]), where
+// ustart are the cells' first positions back to back and start their slots.
 template <int NF, typename T>
-__global__ void k_load(PView P, const int64_t* ids, const int* __restrict__ order, Arrays<T> dst, int64_t np,
-                       double nscale)
+__global__ void k_load(PView P, const int64_t* ids, const int* __restrict__ order, const int* __restrict__ key,
+                       const int* __restrict__ start, const int* __restrict__ ustart, Arrays<T> dst, int64_t np,
+                       int64_t stride, double nscale)
 {
     using I = PIdx<NF>;
     constexpr int nh = NH<NF>;
     for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < np; j += (int64_t)gridDim.x * blockDim.x) {
         const int64_t ip = order[j];
-        for (int d = 0; d < 3; d++) dst.pos[d][j] = P.pos[d][ip];
-        for (int d = 0; d < 4; d++) dst.pup[d][j] = T(P.r[I::pupx + d][ip]);
-        dst.Nn[0][j] = T(P.r[I::N][ip] * nscale);
-        dst.Nn[1][j] = T(P.r[I::Nbar][ip] * nscale);
-        dst.Ll[0][j] = T(P.r[I::L][ip]);
-        dst.Ll[1][j] = T(P.r[I::Lbar][ip]);
-        dst.id[j] = ids ? ids[ip] : ip;
-        for (int c = 0; c < nh; c++) {
-            dst.f[(int64_t)c * np + j] = T(P.r[I::f + c][ip]);
-            dst.f[(int64_t)(nh + c) * np + j] = T(P.r[I::fbar + c][ip]);
+        const int c = key[j];
+        const int64_t k = start[c] + (j - ustart[c]);
+        for (int d = 0; d < 3; d++) dst.pos[d][k] = P.pos[d][ip];
+        for (int d = 0; d < 4; d++) dst.pup[d][k] = T(P.r[I::pupx + d][ip]);
+        dst.Nn[0][k] = T(P.r[I::N][ip] * nscale);
+        dst.Nn[1][k] = T(P.r[I::Nbar][ip] * nscale);
+        dst.Ll[0][k] = T(P.r[I::L][ip]);
+        dst.Ll[1][k] = T(P.r[I::Lbar][ip]);
+        dst.id[k] = ids ? ids[ip] : ip;
+        for (int c2 = 0; c2 < nh; c2++) {
+            dst.f[(int64_t)c2 * stride + k] = T(P.r[I::f + c2][ip]);
+            dst.f[(int64_t)(nh + c2) * stride + k] = T(P.r[I::fbar + c2][ip]);
         }
     }
 }
 
+// Store: one wave per cell copies its cnt[c] particles from slots start[c]..
+// to the caller's positions ustart[c].. (cell order, no gaps).
 template <int NF, typename T>
-__global__ void k_store(Arrays<T> src, PView P, int64_t* ids, const double* time, int64_t np, double nscale)
+__global__ void k_store(Arrays<T> src, PView P, int64_t* ids, const double* time, const int* __restrict__ start,
+                        const int* __restrict__ cnt, const int* __restrict__ ustart, int64_t ncell, int64_t stride,
+                        double nscale)
 {
     using I = PIdx<NF>;
     constexpr int nh = NH<NF>;
     const double t = *time;
-    for (int64_t ip = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; ip < np; ip += (int64_t)gridDim.x * blockDim.x) {
-        for (int d = 0; d < 3; d++) {
-            P.pos[d][ip] = src.pos[d][ip];
-            P.r[I::x + d][ip] = src.pos[d][ip];
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x / 64);
+    for (int64_t c = blockIdx.x * (int64_t)(blockDim.x / 64) + threadIdx.x / 64; c < ncell; c += waves) {
+        const int n = cnt[c];
+        for (int i = threadIdx.x % 64; i < n; i += 64) {
+            const int64_t k = (int64_t)start[c] + i, ip = (int64_t)ustart[c] + i;
+            for (int d = 0; d < 3; d++) {
+                P.pos[d][ip] = src.pos[d][k];
+                P.r[I::x + d][ip] = src.pos[d][k];
+            }
+            P.r[I::time][ip] = t;
+            for (int d = 0; d < 4; d++) P.r[I::pupx + d][ip] = double(src.pup[d][k]);
+            P.r[I::N][ip] = double(src.Nn[0][k]) / nscale;
+            P.r[I::Nbar][ip] = double(src.Nn[1][k]) / nscale;
+            P.r[I::L][ip] = double(src.Ll[0][k]);
+            P.r[I::Lbar][ip] = double(src.Ll[1][k]);
+            for (int c2 = 0; c2 < nh; c2++) {
+                P.r[I::f + c2][ip] = double(src.f[(int64_t)c2 * stride + k]);
+                P.r[I::fbar + c2][ip] = double(src.f[(int64_t)(nh + c2) * stride + k]);
+            }
+            if (ids) ids[ip] = src.id[k];
         }
-        P.r[I::time][ip] = t;
-        for (int d = 0; d < 4; d++) P.r[I::pupx + d][ip] = double(src.pup[d][ip]);
-        P.r[I::N][ip] = double(src.Nn[0][ip]) / nscale;
-        P.r[I::Nbar][ip] = double(src.Nn[1][ip]) / nscale;
-        P.r[I::L][ip] = double(src.Ll[0][ip]);
-        P.r[I::Lbar][ip] = double(src.Ll[1][ip]);
-        for (int c = 0; c < nh; c++) {
-            P.r[I::f + c][ip] = double(src.f[(int64_t)c * np + ip]);
-            P.r[I::fbar + c][ip] = double(src.f[(int64_t)(nh + c) * np + ip]);
-        }
-        if (ids) ids[ip] = src.id[ip];
     }
 }
 
@@ -367,12 +396,12 @@ __device__ __forceinline__ double wave_sum(double v)
 }
 
 template <int NF, typename T>
-__global__ __launch_bounds__(64) void k_diag_deposit(Arrays<T> A, const int* __restrict__ cell_start, Geom g,
-                                                     int64_t np, double* out4)
+__global__ __launch_bounds__(64) void k_diag_deposit(Arrays<T> A, const int* __restrict__ cell_start,
+                                                     const int* __restrict__ cell_cnt, Geom g, int64_t np, double* out4)
 {
     const int cell = blockIdx.x;
     const int lane = threadIdx.x;
-    const int beg = cell_start[cell], end = cell_start[cell + 1];
+    const int beg = cell_start[cell], end = beg + cell_cnt[cell];
     for (int base = beg; base < end; base += 64) {
         const int ip = base + lane;
         const bool act = ip < end;
@@ -435,6 +464,14 @@ struct emu_state {
     int nf, nh, QG, naux;
     int64_t ncell;
     size_t tsize;
+    // Slots: cell c's particles sit in slots cell_start[c] .. cell_start[c] +
+    // cell_cnt[c] - 1, and every cell's run starts at a multiple of `align`
+    // (1: back to back; 128 bytes of T otherwise, so that a wave's loads and
+    // stores of a cell's particles cover whole cache lines that no other cell
+    // shares).  cap slots (the stride of the per-particle arrays) hold any
+    // layout: np + ncell (align - 1).
+    int align = 1;
+    int64_t cap = 1;
     // persistent arrays, two copies for the sort
     void* block[2] = {nullptr, nullptr};
     // scratch
@@ -442,9 +479,11 @@ struct emu_state {
     void* acc = nullptr;
     int* newcell = nullptr;
     int* rank = nullptr;
-    int* counts = nullptr;
+    int* counts = nullptr;         // ncell + 1: the scan's input (padded counts)
     int* cell_start = nullptr;     // ncell + 1
     int* new_start = nullptr;      // ncell + 1
+    int* cell_cnt = nullptr;       // ncell + 1: particles per cell (current order)
+    int* new_cnt = nullptr;        // ncell + 1: the same after this step's moves
     int* mcount = nullptr;         // [ncell][27] particles per (old cell, move)
     int* mbase = nullptr;          // [ncell][27] destination of the first of them
     unsigned char* mv = nullptr;   // move index per particle
@@ -479,7 +518,7 @@ template <typename T> Arrays<T> arrays_of(emu_state* st, int copy)
 {
     Arrays<T> A;
     char* b = (char*)st->block[copy];
-    const int64_t np = st->np;
+    const int64_t np = st->cap;
     for (int d = 0; d < 3; d++) { A.pos[d] = (double*)b; b += np * sizeof(double); }
     A.id = (int64_t*)b;
     b += np * sizeof(int64_t);
@@ -512,14 +551,14 @@ int ensure_cub(emu_state* st, size_t need)
     return EMU_OK;
 }
 
-// out = exclusive scan of counts (ncell + 1 entries, counts[ncell] = 0)
-int scan_counts(emu_state* st, int* out)
+// out = exclusive scan of in (ncell + 1 entries, in[ncell] = 0)
+int scan_counts(emu_state* st, const int* in, int* out)
 {
     size_t need = 0;
-    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, need, st->counts, out, (int)(st->ncell + 1), st->ctx->stream);
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, need, in, out, (int)(st->ncell + 1), st->ctx->stream);
     if (ensure_cub(st, need)) return set_error(st->ctx, EMU_ERR_NOMEM, "scan scratch");
-    EMU_HIP(st->ctx, hipcub::DeviceScan::ExclusiveSum(st->cub_tmp, st->cub_bytes, st->counts, out,
-                                                     (int)(st->ncell + 1), st->ctx->stream));
+    EMU_HIP(st->ctx, hipcub::DeviceScan::ExclusiveSum(st->cub_tmp, st->cub_bytes, in, out, (int)(st->ncell + 1),
+                                                     st->ctx->stream));
     return EMU_OK;
 }
 
@@ -586,13 +625,14 @@ StageArgs<T> stage_args(emu_state* st, int stage, const double* gin, double* gou
     a.rank = st->rank;
     a.mv = st->mv;
     a.cell_start = st->cell_start;
+    a.cell_cnt = st->cell_cnt;
     a.gin = gin;
     a.gin32 = st->grid32;
     a.gout = gout;
     a.aux = st->aux;
     a.dt = st->scal + 1;
     a.err = st->red + 2;
-    a.np = st->np;
+    a.np = st->cap;
     a.ncell = st->ncell;
     a.stage = stage;
     a.want_aux = want_aux ? 1 : 0;
@@ -672,7 +712,11 @@ extern "C" int emu_state_destroy(emu_state* st);
 
 extern "C" int emu_state_create(emu_ctx* ctx, int64_t np, int precision, emu_state** out)
 {
-    if (!ctx || !out || np < 0 || (precision != EMU_PLAIN_FP64 && precision != EMU_FP32_BULK)) return EMU_ERR_ARG;
+    const int layout = precision & ~0xff;
+    precision &= 0xff;
+    if (!ctx || !out || np < 0 || (precision != EMU_PLAIN_FP64 && precision != EMU_FP32_BULK) ||
+        (layout != 0 && layout != EMU_STATE_SLOTS_PACKED && layout != EMU_STATE_SLOTS_ALIGNED))
+        return EMU_ERR_ARG;
     if (np > (int64_t)INT32_MAX) return set_error(ctx, EMU_ERR_UNSUPPORTED, "more than 2^31-1 particles per state");
     *out = nullptr;
     auto* st = new emu_state();
@@ -686,7 +730,14 @@ extern "C" int emu_state_create(emu_ctx* ctx, int64_t np, int precision, emu_sta
     st->ncell = (int64_t)ctx->g.n[0] * ctx->g.n[1] * ctx->g.n[2];
     st->tsize = precision == EMU_FP32_BULK ? sizeof(float) : sizeof(double);
     st->m00 = ctx->p.rho * ctx->p.Ye / phys::Mp / ctx->g.inv_cell_volume;
-    const int64_t npa = std::max<int64_t>(np, 1);
+    // slot alignment: 128 bytes of T; by default only where the cells hold on
+    // average two alignments' worth of particles (padding <= np / 2)
+    const int a128 = (int)(128 / st->tsize);
+    bool aligned = layout == EMU_STATE_SLOTS_ALIGNED || (layout == 0 && np >= 2 * a128 * st->ncell);
+    if (np == 0 || np + st->ncell * (a128 - 1) > (int64_t)INT32_MAX) aligned = false;
+    st->align = aligned ? a128 : 1;
+    st->cap = std::max<int64_t>(np + st->ncell * (st->align - 1), 1);
+    const int64_t npa = st->cap;
     const size_t pb = persistent_bytes(npa, st->nh, st->tsize);
     bool ok = hipMalloc(&st->block[0], pb) == hipSuccess && hipMalloc(&st->block[1], pb) == hipSuccess &&
               hipMalloc(&st->Y, (size_t)npa * 2 * st->nh * st->tsize) == hipSuccess &&
@@ -696,6 +747,8 @@ extern "C" int emu_state_create(emu_ctx* ctx, int64_t np, int precision, emu_sta
               hipMalloc(&st->counts, (size_t)(st->ncell + 1) * sizeof(int)) == hipSuccess &&
               hipMalloc(&st->cell_start, (size_t)(st->ncell + 1) * sizeof(int)) == hipSuccess &&
               hipMalloc(&st->new_start, (size_t)(st->ncell + 1) * sizeof(int)) == hipSuccess &&
+              hipMalloc(&st->cell_cnt, (size_t)(st->ncell + 1) * sizeof(int)) == hipSuccess &&
+              hipMalloc(&st->new_cnt, (size_t)(st->ncell + 1) * sizeof(int)) == hipSuccess &&
               hipMalloc(&st->mcount, (size_t)st->ncell * 27 * sizeof(int)) == hipSuccess &&
               hipMalloc(&st->mbase, (size_t)st->ncell * 27 * sizeof(int)) == hipSuccess &&
               hipMalloc(&st->mv, (size_t)npa) == hipSuccess &&
@@ -718,9 +771,10 @@ extern "C" int emu_state_create(emu_ctx* ctx, int64_t np, int precision, emu_sta
 extern "C" int emu_state_destroy(emu_state* st)
 {
     if (!st) return EMU_OK;
-    void* ptrs[] = {st->block[0], st->block[1], st->Y, st->acc, st->newcell, st->rank, st->counts, st->cell_start,
-                    st->new_start, st->mcount, st->mbase, st->mv, st->grid[0], st->grid[1], st->grid32, st->aux, st->scal,
-                    st->steps, st->red, st->cub_tmp};
+    void* ptrs[] = {st->block[0], st->block[1], st->Y,        st->acc,     st->newcell, st->rank,
+                    st->counts,   st->cell_start, st->new_start, st->cell_cnt, st->new_cnt, st->mcount,
+                    st->mbase,    st->mv,         st->grid[0], st->grid[1], st->grid32,  st->aux,
+                    st->scal,     st->steps,      st->red,     st->cub_tmp};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& m : st->marks) {
@@ -739,7 +793,13 @@ template <typename T> static int load_t(emu_state* st, const PView& P, const int
     hipLaunchKernelGGL(k_fill_int, dim3(grid_blocks(st->ncell + 1)), dim3(256), 0, s, st->counts, st->ncell + 1, 0);
     if (np > 0)
         hipLaunchKernelGGL(k_bin_input, dim3(grid_blocks(np)), dim3(256), 0, s, P, ctx->g, st->newcell, st->counts);
-    int e = scan_counts(st, st->cell_start);
+    // the cells' first positions back to back (new_start, scratch here), then
+    // their padded slot runs (cell_start)
+    int e = scan_counts(st, st->counts, st->new_start);
+    if (e) return e;
+    hipLaunchKernelGGL(k_pad_counts, dim3(grid_blocks(st->ncell + 1)), dim3(256), 0, s, st->counts, st->cell_cnt,
+                       st->ncell, st->align);
+    e = scan_counts(st, st->counts, st->cell_start);
     if (e) return e;
     if (np == 0) return EMU_OK;
     st->nscale = 1.0;
@@ -774,11 +834,11 @@ template <typename T> static int load_t(emu_state* st, const PView& P, const int
     if (!e) {
         Arrays<T> A = arrays_of<T>(st, st->cur);
         if (st->nf == 2)
-            hipLaunchKernelGGL((k_load<2, T>), dim3(grid_blocks(np)), dim3(256), 0, s, P, ids, order, A, np,
-                               st->nscale);
+            hipLaunchKernelGGL((k_load<2, T>), dim3(grid_blocks(np)), dim3(256), 0, s, P, ids, order, keys_out,
+                               st->cell_start, st->new_start, A, np, st->cap, st->nscale);
         else
-            hipLaunchKernelGGL((k_load<3, T>), dim3(grid_blocks(np)), dim3(256), 0, s, P, ids, order, A, np,
-                               st->nscale);
+            hipLaunchKernelGGL((k_load<3, T>), dim3(grid_blocks(np)), dim3(256), 0, s, P, ids, order, keys_out,
+                               st->cell_start, st->new_start, A, np, st->cap, st->nscale);
         if (hipGetLastError() != hipSuccess) e = set_error(ctx, EMU_ERR_HIP, "state_load");
     }
     (void)hipStreamSynchronize(s);
@@ -845,14 +905,30 @@ extern "C" int emu_state_store(emu_state* st, emu_particles* parts, int64_t* ids
     for (int d = 0; d < 3; d++) P.pos[d] = parts->pos[d];
     for (int k = 0; k < EMU_MAX_ATTRIBS; k++) P.r[k] = k < na ? parts->rdata[k] : nullptr;
     if (st->np > 0) {
+        // the caller's positions of each cell's first particle (new_start is
+        // scratch between steps)
+        int e = scan_counts(st, st->cell_cnt, st->new_start);
+        if (e) return e;
+        const dim3 grid(grid_blocks(st->ncell, 4)), block(256); // one wave per cell
+        const int* cs = st->cell_start;
+        const int* cn = st->cell_cnt;
+        const int* us = st->new_start;
         if (st->precision == EMU_FP32_BULK) {
             Arrays<float> A = arrays_of<float>(st, st->cur);
-            if (st->nf == 2) hipLaunchKernelGGL((k_store<2, float>), dim3(grid_blocks(st->np)), dim3(256), 0, ctx->stream, A, P, ids, st->scal, st->np, st->nscale);
-            else hipLaunchKernelGGL((k_store<3, float>), dim3(grid_blocks(st->np)), dim3(256), 0, ctx->stream, A, P, ids, st->scal, st->np, st->nscale);
+            if (st->nf == 2)
+                hipLaunchKernelGGL((k_store<2, float>), grid, block, 0, ctx->stream, A, P, ids, st->scal, cs, cn, us,
+                                   st->ncell, st->cap, st->nscale);
+            else
+                hipLaunchKernelGGL((k_store<3, float>), grid, block, 0, ctx->stream, A, P, ids, st->scal, cs, cn, us,
+                                   st->ncell, st->cap, st->nscale);
         } else {
             Arrays<double> A = arrays_of<double>(st, st->cur);
-            if (st->nf == 2) hipLaunchKernelGGL((k_store<2, double>), dim3(grid_blocks(st->np)), dim3(256), 0, ctx->stream, A, P, ids, st->scal, st->np, st->nscale);
-            else hipLaunchKernelGGL((k_store<3, double>), dim3(grid_blocks(st->np)), dim3(256), 0, ctx->stream, A, P, ids, st->scal, st->np, st->nscale);
+            if (st->nf == 2)
+                hipLaunchKernelGGL((k_store<2, double>), grid, block, 0, ctx->stream, A, P, ids, st->scal, cs, cn, us,
+                                   st->ncell, st->cap, st->nscale);
+            else
+                hipLaunchKernelGGL((k_store<3, double>), grid, block, 0, ctx->stream, A, P, ids, st->scal, cs, cn, us,
+                                   st->ncell, st->cap, st->nscale);
         }
     }
     EMU_LAUNCH_CHECK(ctx, "state_store");
@@ -885,10 +961,10 @@ extern "C" int emu_state_evolve(emu_state* st, int nsteps, double fixed_dt)
         // stable counting-sort plan by the new cell
         ev_record(st, 2, true);
         hipLaunchKernelGGL(k_move_ranks, dim3((unsigned)st->ncell), dim3(64), 0, s, st->newcell, st->cell_start,
-                           ctx->g, st->rank, st->mv, st->mcount, st->red + 2);
-        hipLaunchKernelGGL(k_new_counts, dim3(grid_blocks(st->ncell)), dim3(256), 0, s, st->mcount, ctx->g,
-                           st->ncell, st->counts);
-        int e = scan_counts(st, st->new_start);
+                           st->cell_cnt, ctx->g, st->rank, st->mv, st->mcount, st->red + 2);
+        hipLaunchKernelGGL(k_new_counts, dim3(grid_blocks(st->ncell + 1)), dim3(256), 0, s, st->mcount, ctx->g,
+                           st->ncell, st->align, st->counts, st->new_cnt);
+        int e = scan_counts(st, st->counts, st->new_start);
         if (e) return e;
         hipLaunchKernelGGL(k_bases, dim3(grid_blocks(st->ncell)), dim3(256), 0, s, st->mcount, st->new_start, ctx->g,
                            st->ncell, st->mbase);
@@ -914,6 +990,7 @@ extern "C" int emu_state_evolve(emu_state* st, int nsteps, double fixed_dt)
         ev_record(st, 0, false);
         st->cur = 1 - st->cur;
         std::swap(st->cell_start, st->new_start);
+        std::swap(st->cell_cnt, st->new_cnt);
         // deposit of y_{n+1} for the next step's stage 1
         ev_record(st, 1, true);
         fill_grid(st, A, false);
@@ -989,10 +1066,10 @@ template <typename T> static void diag_t(emu_state* st, double* g4)
     Arrays<T> A = arrays_of<T>(st, st->cur);
     if (st->nf == 2)
         hipLaunchKernelGGL((k_diag_deposit<2, T>), dim3((unsigned)st->ncell), dim3(64), 0, st->ctx->stream, A,
-                           st->cell_start, st->ctx->g, st->np, g4);
+                           st->cell_start, st->cell_cnt, st->ctx->g, st->cap, g4);
     else
         hipLaunchKernelGGL((k_diag_deposit<3, T>), dim3((unsigned)st->ncell), dim3(64), 0, st->ctx->stream, A,
-                           st->cell_start, st->ctx->g, st->np, g4);
+                           st->cell_start, st->cell_cnt, st->ctx->g, st->cap, g4);
 }
 
 extern "C" int emu_state_flavor_amplitude(emu_state* st, double* sums)
@@ -1005,13 +1082,10 @@ extern "C" int emu_state_flavor_amplitude(emu_state* st, double* sums)
                        st->ncell * st->naux, 1, 0.0, nullptr, 0);
     if (st->precision == EMU_FP32_BULK) diag_t<float>(st, st->aux);
     else diag_t<double>(st, st->aux);
-    double* dsum = st->scal + 3; // one free slot; a second comes from red[4] reinterpreted
-    double* dsum2 = reinterpret_cast<double*>(st->red + 4);
-    (void)dsum2;
+    // the two sums in red[4..5] (free between steps), as doubles
     hipLaunchKernelGGL(k_fill_u64v, dim3(1), dim3(64), 0, s, st->red + 4, 2, 0ull);
     hipLaunchKernelGGL(k_diag_reduce, dim3(grid_blocks(st->ncell)), dim3(256), 0, s, st->aux, st->ncell,
                        reinterpret_cast<double*>(st->red + 4));
-    (void)dsum;
     EMU_LAUNCH_CHECK(ctx, "flavor_amplitude");
     double h[2];
     EMU_HIP(ctx, hipMemcpyAsync(h, st->red + 4, sizeof h, hipMemcpyDeviceToHost, s));

@@ -44,6 +44,13 @@ typedef enum emu_status {
 } emu_status;
 
 enum { EMU_PLAIN_FP64 = 0, EMU_FP32_BULK = 1 };
+/* Slot layout of a fused state, or-ed into emu_state_create's precision.  The
+ * state keeps its particles sorted by cell; ALIGNED starts every cell's run of
+ * slots on a 128-byte boundary of the per-particle arrays (32 slots in
+ * FP32-bulk, 16 in FP64), PACKED keeps the runs back to back.  Neither (0)
+ * aligns when the cells hold on average at least two boundaries' worth of
+ * particles, which bounds the padding at half the particle count. */
+enum { EMU_STATE_SLOTS_PACKED = 0x100, EMU_STATE_SLOTS_ALIGNED = 0x200 };
 
 /* Run parameters in the reference's internal units (Source/Parameters.H:11-131
  * after its unit conversions): masses in grams, angles in radians. */
@@ -139,7 +146,8 @@ int emu_copy_particles(emu_ctx* ctx, emu_particles* dst, const emu_particles* sr
 
 /* ---- fused state ----------------------------------------------------------
  * The integrator loop of main.cpp:129-213 (AMReX TimeIntegrator RK4 + the
- * post-step hook) on a library-owned SoA state sorted by cell. */
+ * post-step hook) on a library-owned SoA state sorted by cell.  precision is
+ * EMU_PLAIN_FP64 or EMU_FP32_BULK, optionally or-ed with one slot layout flag. */
 int emu_state_create(emu_ctx* ctx, int64_t np, int precision, emu_state** out);
 int emu_state_destroy(emu_state* st);
 /* Load particles (device SoA view, any order); computes Vvac_max, deposits and

@@ -1,7 +1,9 @@
 """Edge cases of the flavor step on the GPU, against the oracle: no particles
 at all, ragged cell occupancy around the fused kernel's 64-particle chunks
 (empty cells, single particles, 63/64/65 and 127/128/129 particles), and one
-crowded cell that spans seven chunks."""
+crowded cell that spans seven chunks; each in both slot layouts of the fused
+state (cells back to back, and each cell's run on a 128-byte boundary), and
+the two layouts against each other."""
 import numpy as np
 import pytest
 
@@ -45,9 +47,9 @@ def placed(oracle, deck, ncell, L, occupancy, seed):
     return parts
 
 
-def run_fused_against_oracle(core, oracle, nf, prm, emu, parts, prec, nsteps, fixed):
+def run_fused_against_oracle(core, oracle, nf, prm, emu, parts, prec, nsteps, fixed, slots):
     c = core.FlavoredNeutrinoContainer.from_aos(emu, parts)
-    st = core.FusedState(emu, c.np, prec)
+    st = core.FusedState(emu, c.np, prec, slots)
     st.load(c)
     st.evolve(nsteps, fixed_dt=fixed)
     st.store(c)
@@ -76,9 +78,10 @@ def run_fused_against_oracle(core, oracle, nf, prm, emu, parts, prec, nsteps, fi
     assert info["steps"] == nsteps
 
 
+@pytest.mark.parametrize("slots", ["packed", "aligned"])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 @pytest.mark.parametrize("nf", [2, 3])
-def test_ragged_occupancy(core, oracle, nf, prec):
+def test_ragged_occupancy(core, oracle, nf, prec, slots):
     """Cells holding 0, 1, 2, 63, 64, 65, 127, 128, 129, 200, 17, 5 or 40
     particles, 4 steps with a dt that moves particles across cells."""
     ncell, L = (4, 4, 4), (2.0, 2.0, 2.0)
@@ -87,22 +90,24 @@ def test_ragged_occupancy(core, oracle, nf, prec):
     counts = [0, 1, 2, 63, 64, 65, 127, 128, 129, 200, 0, 0, 17, 5, 0, 40]
     occ = {cell: counts[cell % len(counts)] for cell in range(64)}
     parts = placed(oracle, deck, ncell, L, occ, seed=5)
-    run_fused_against_oracle(core, oracle, nf, prm, emu, parts, prec, 4, 0.3 * 0.5 / oracle.clight)
+    run_fused_against_oracle(core, oracle, nf, prm, emu, parts, prec, 4, 0.3 * 0.5 / oracle.clight, slots)
 
 
+@pytest.mark.parametrize("slots", ["packed", "aligned"])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-def test_crowded_cell(core, oracle, prec):
+def test_crowded_cell(core, oracle, prec, slots):
     """400 particles in one cell (six full chunks and a partial one), its
     neighbours empty, and a few particles elsewhere."""
     nf, ncell, L = 3, (3, 3, 3), (1.5, 1.5, 1.5)
     d = cases.fast_flavor_dict(nf=nf, ncell=ncell, L=L, nphi=16, st4_amplitude=1e-3)
     deck, prm, emu = make(core, oracle, d)
     parts = placed(oracle, deck, ncell, L, {13: 400, 0: 3, 26: 1}, seed=9)
-    run_fused_against_oracle(core, oracle, nf, prm, emu, parts, prec, 3, 0.3 * 0.5 / oracle.clight)
+    run_fused_against_oracle(core, oracle, nf, prm, emu, parts, prec, 3, 0.3 * 0.5 / oracle.clight, slots)
 
 
+@pytest.mark.parametrize("slots", ["packed", "aligned"])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-def test_no_particles(core, oracle, prec):
+def test_no_particles(core, oracle, prec, slots):
     """An empty container: the deposit leaves only the background fields, the
     flavour bound of compute_dt has nothing to bound (the step is the
     translation bound, as in the reference's compute_dt), and the fused state
@@ -124,7 +129,7 @@ def test_no_particles(core, oracle, prec):
     assert err == 0 and dt_or == pytest.approx(0.4 * 0.5 / oracle.clight, rel=1e-14)
     assert core.compute_dt(state, 0.0) == pytest.approx(dt_or, rel=1e-14)
     c.Renormalize()
-    st = core.FusedState(emu, 0, prec)
+    st = core.FusedState(emu, 0, prec, slots)
     st.load(c)
     st.evolve(2)
     info = st.info()
@@ -132,3 +137,38 @@ def test_no_particles(core, oracle, prec):
     st.close()
     assert info["steps"] == 2
     assert info["time"] == pytest.approx(2 * dt_or, rel=1e-14)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("nf", [2, 3])
+def test_slot_layouts_agree(core, oracle, nf, prec):
+    """The aligned layout only moves where each cell's run of particles starts;
+    the order within a cell is that of the packed layout.  What still differs
+    between any two runs is the order in which the deposit's FP64 atomics
+    from different cells land, at the grid's last bit.  So: 6 steps with a
+    dt that moves particles across cells give the same ids and positions bit
+    for bit and flavour states within that rounding; then 2 steps with
+    compute_dt's dt (the auxiliary deposit) give the same dt to rounding.  A
+    6 x 5 x 4 box of 92 directions: runs of 92, and their neighbours after
+    moves."""
+    d = cases.fast_flavor_dict(nf=nf, ncell=(6, 5, 4), nphi=16, st4_amplitude=1e-3)
+    deck, prm, emu = make(core, oracle, d)
+    parts = oracle.init_particles(deck)
+    out = {}
+    for slots in ("packed", "aligned"):
+        c = core.FlavoredNeutrinoContainer.from_aos(emu, parts)
+        st = core.FusedState(emu, c.np, prec, slots)
+        st.load(c)
+        st.evolve(6, fixed_dt=0.3 * 0.5 / oracle.clight)
+        st.store(c)
+        fixed = c.to_aos()
+        st.evolve(2)
+        out[slots] = (fixed, st.info())
+        st.close()
+    (a, ia), (b, ib) = out["packed"], out["aligned"]
+    np.testing.assert_array_equal(a["id"], b["id"])
+    np.testing.assert_array_equal(a["pos"], b["pos"])
+    assert cases.max_rel_err(b["rdata"], a["rdata"]) < (1e-13 if prec == "fp64" else 2e-5)
+    assert ib["steps"] == ia["steps"] == 8
+    assert ib["dt"] == pytest.approx(ia["dt"], rel=1e-12)
+    assert ib["time"] == pytest.approx(ia["time"], rel=1e-12)
