@@ -447,10 +447,12 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // the stage state: y_n (stages 0, 1), Y2 and Y4 in Y, Y3 in the
+        // accumulator's array (stage 2 writes it there: see below)
         T Y[2 * nh];
+        const T* ysrc = s <= 1 ? a.cur.f : (s == 3 ? a.acc : a.Y);
 #pragma unroll
-        for (int c = 0; c < 2 * nh; c++)
-            Y[c] = s <= 1 ? a.cur.f[(int64_t)c * a.np + ipl] : a.Y[(int64_t)c * a.np + ipl];
+        for (int c = 0; c < 2 * nh; c++) Y[c] = ysrc[(int64_t)c * a.np + ipl];
         T Dep[2 * nh]; // state deposited by this launch
         if constexpr (s == 0) {
 #pragma unroll
@@ -538,11 +540,18 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
             // ---- the RK registers this stage reads, as one batch of loads whose
             // latency the Hamiltonian and commutator arithmetic covers
             T yn[2 * nh], ao[2 * nh];
-            // The accumulator after stage 1, y_n + dt/6 k1, is not stored: with
-            // Y2 = y_n + dt/2 k1 it equals y_n + (Y2 - y_n)/3, which stage 2
-            // rebuilds from the two arrays it reads anyway (one write and one
-            // read of the flavour state saved per step; rounding-level change,
-            // with a multiply by the rounded 1/3 in place of the division).
+            // The accumulators after stages 1 and 2 are not stored.  With
+            // Y2 = y_n + dt/2 k1 and Y3 = y_n + dt/2 k2,
+            //   acc1 = y_n + dt/6 k1 = y_n + (Y2 - y_n)/3,
+            //   acc2 = acc1 + dt/3 k2 = Y3 + (Y2 - Y3)/3 (y_n cancels; in this
+            //   form the rounded 1/3 only scales the small difference, where
+            //   Y2/3 + 2 Y3/3 would scale the state by (1/3 + 2/3 rounded) - 1,
+            //   3e-8 in FP32, every step),
+            // which stages 2 and 3 rebuild from arrays they read: stage 2 keeps
+            // Y2 and writes Y3 into the accumulator's array (free until stage 3
+            // writes acc3 there), so stage 3 reads y_n, Y2 and Y3.  That saves
+            // two writes and a read of the flavour state per step; the change
+            // is at rounding level, with multiplies by the rounded 1/3 and 2/3.
             if constexpr (s == 1) {
 #pragma unroll
                 for (int c = 0; c < 2 * nh; c++) yn[c] = ao[c] = Y[c];
@@ -551,6 +560,13 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
                 for (int c = 0; c < 2 * nh; c++) yn[c] = a.cur.f[(int64_t)c * a.np + ipl];
 #pragma unroll
                 for (int c = 0; c < 2 * nh; c++) ao[c] = yn[c] + (Y[c] - yn[c]) * T(1.0 / 3.0);
+            } else if constexpr (s == 3) {
+#pragma unroll
+                for (int c = 0; c < 2 * nh; c++) yn[c] = a.cur.f[(int64_t)c * a.np + ipl];
+#pragma unroll
+                for (int c = 0; c < 2 * nh; c++) ao[c] = a.Y[(int64_t)c * a.np + ipl];
+#pragma unroll
+                for (int c = 0; c < 2 * nh; c++) ao[c] = (ao[c] - Y[c]) * T(1.0 / 3.0) + Y[c];
             } else {
 #pragma unroll
                 for (int c = 0; c < 2 * nh; c++) yn[c] = s < 4 ? a.cur.f[(int64_t)c * a.np + ipl] : T(0);
@@ -578,10 +594,13 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
                     ao[c] = a_acc * k[c] + ao[c];
                 }
                 if (act) {
+                    // stage 1: Y2 -> Y; stage 2: Y3 -> acc's array; stage 3: acc3 ->
+                    // acc, Y4 -> Y (each lane overwrites only the slot it read)
 #pragma unroll
                     for (int c = 0; c < 2 * nh; c++) {
-                        if (s != 1) a.acc[(int64_t)c * a.np + ip] = ao[c];
-                        a.Y[(int64_t)c * a.np + ip] = Dep[c];
+                        if (s == 3) a.acc[(int64_t)c * a.np + ip] = ao[c];
+                        if (s == 2) a.acc[(int64_t)c * a.np + ip] = Dep[c];
+                        else a.Y[(int64_t)c * a.np + ip] = Dep[c];
                     }
                 }
                 if (s == 1 && act) { // the end-of-step cell, for the sort before stage 4

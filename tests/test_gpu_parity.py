@@ -261,9 +261,14 @@ def test_lower_shape_orders_parity(core, oracle, nf, order):
     """SHAPE_FACTOR_ORDER 0 (nearest cell: 1-point stencils) and 1 (cloud in
     cell: the 3-point stencils with one zero weight) on the per-function
     deposit and interpolation and on 6 fused FP64 steps with a dt that moves
-    particles across cells, against the oracle at the same order."""
+    particles across cells, against the oracle at the same order.  The
+    densities are a tenth of the fast-flavor deck's, so that this dt stays
+    within compute_dt's flavour bound: at the full densities it is 5x that
+    bound, and the oracle's own step then grows a one-ulp change of the
+    flavour state to ~3e-11 in 6 steps at these orders (1e-15 here), which
+    no 1e-12 comparison could survive."""
     d = cases.fast_flavor_dict(nf=nf, ncell=(4, 3, 5), L=(2.0, 1.5, 2.5), nphi=8, shape_order=order,
-                               st4_amplitude=1e-3)
+                               st4_amplitude=1e-3, st4_ndens=4.89e31, st4_ndensbar=4.89e31)
     deck, prm, emu = make(core, oracle, d)
     parts = oracle.init_particles(deck)
     c = core.FlavoredNeutrinoContainer.from_aos(emu, parts)
@@ -326,7 +331,11 @@ def test_adaptive_dt_parity(core, oracle, nf, path, speedup):
     dt_adaptive).  2 cm cells keep the translation bound (cfl_factor 0.9, as
     the fused sort needs c dt < dx) above both, and the antineutrino flux makes
     |F - Fbar| the adaptive bound: a speedup of 4 is capped by dt_stupid, 1e9
-    by dt_adaptive.  Every computed dt of 3 steps matches."""
+    by dt_adaptive.  Every computed dt of 3 steps matches.  At 1e9 the step is
+    far past the conservative bound, and the oracle's own 3 steps grow a
+    one-ulp change of the flavour state to 1.2e-11 (NF=2) / 1.6e-11 (NF=3)
+    of each component's range (9e-16 at 4): the FP64 state bar there is 1e-9,
+    a hundred such ulps."""
     ncell = (4, 3, 5)
     d = cases.fast_flavor_dict(nf=nf, ncell=ncell, L=tuple(2.0 * n for n in ncell), nphi=4, st4_amplitude=1e-3,
                                cfl_factor=0.9, max_adaptive_speedup=speedup)
@@ -361,7 +370,7 @@ def test_adaptive_dt_parity(core, oracle, nf, path, speedup):
         got = cases.by_id(c.to_aos())
     want = cases.by_id(parts)
     if path != "fp32":
-        assert cases.max_rel_err(got["rdata"], want["rdata"]) < 1e-11
+        assert cases.max_rel_err(got["rdata"], want["rdata"]) < (1e-11 if speedup == 4.0 else 1e-9)
         return
     # At the adaptive step (31x the stupid one) the oracle itself moves 5-8% of the
     # largest |f11| (~1e-5) when its input is perturbed by 6e-8 relative, so
