@@ -186,13 +186,18 @@ __device__ __forceinline__ void end_position(const double pos[3], const double v
 //  2: as 1, and the deposits are also summed over the box in LDS before one
 //     set of global atomics (56% fewer for a 2 x 2 patch at the 3-D stencil),
 //     at the cost of two more barriers.
-// PM (a shared patch) is MODE >= 1.
-template <int WX, int WY, int WZ, int WPB, bool PM> struct Patch {
-    static constexpr int PX = PM && WX == 3 ? 2 : 1;
+// PM (a shared patch) is MODE >= 1.  In MODE 0 a wave may also take CPW = 2
+// cells, x-adjacent, as one patch: their particles form one list of 16-particle
+// tiles (the first cell's, then the second's), so that a cell of 92 particles no
+// longer leaves 36 of a chunk's 64 lanes idle, and the wave stages the two
+// cells' 4 x 3 x 3 box once.
+template <int WX, int WY, int WZ, int WPB, bool PM, int CPW = 1> struct Patch {
+    static_assert(CPW == 1 || (CPW == 2 && !PM && WX == 3), "two cells per wave: MODE 0, along a 3-wide x");
+    static constexpr int PX = (PM || CPW == 2) && WX == 3 ? 2 : 1;
     static constexpr int PY = PM && WY == 3 && WPB >= 2 * PX ? 2 : 1;
     static constexpr int PZ = PM ? WPB / (PX * PY) : 1;
-    static constexpr int CELLS = PM ? WPB : 1; // cells of one patch
-    static_assert(PX * PY * PZ == CELLS, "a patch holds one cell per wave");
+    static constexpr int CELLS = PM ? WPB : CPW; // cells of one patch
+    static_assert(PX * PY * PZ == CELLS, "a patch holds one cell per wave, or the wave's CPW cells");
     static constexpr int NBX = PX + WX - 1, NBY = PY + WY - 1, NBZ = PZ + WZ - 1;
     static constexpr int NB = NBX * NBY * NBZ;
 };
@@ -208,9 +213,17 @@ template <int NF, typename T, bool AUX> constexpr int stage_lds_doubles_per_wave
 
 // LDS of the table of the neighbourhood box's cell indices (NB ints), in
 // doubles: one per workgroup (PM) or per wave.
-template <int WX, int WY, int WZ, int WPB, bool PM> constexpr int stage_lds_ctab_doubles()
+template <int WX, int WY, int WZ, int WPB, bool PM, int CPW> constexpr int stage_lds_ctab_doubles()
 {
-    return (Patch<WX, WY, WZ, WPB, PM>::NB + 1) / 2;
+    return (Patch<WX, WY, WZ, WPB, PM, CPW>::NB + 1) / 2;
+}
+
+// cells per wave: two in FP32-bulk's independent-wave launches of the 3-D
+// stencil (its 4 x 3 x 3 box still fits three waves per SIMD in LDS; FP64's
+// would not fit two)
+template <typename T, int WX, int MODE> constexpr int stage_cells_per_wave()
+{
+    return MODE == 0 && sizeof(T) == 4 && WX == 3 ? 2 : 1;
 }
 
 // LDS of a workgroup, in doubles.  The cell tables come first.  PM: then the
@@ -223,9 +236,10 @@ template <int NF, typename T, int WX, int WY, int WZ, int WPB, int S, bool AUX, 
 constexpr int stage_lds_doubles_per_block()
 {
     constexpr bool PM = MODE >= 1;
+    constexpr int CPW = stage_cells_per_wave<T, WX, MODE>();
     constexpr int nh = NF * NF;
-    constexpr int NB = Patch<WX, WY, WZ, WPB, PM>::NB;
-    constexpr int ct = (PM ? 1 : WPB) * stage_lds_ctab_doubles<WX, WY, WZ, WPB, PM>();
+    constexpr int NB = Patch<WX, WY, WZ, WPB, PM, CPW>::NB;
+    constexpr int ct = (PM ? 1 : WPB) * stage_lds_ctab_doubles<WX, WY, WZ, WPB, PM, CPW>();
     constexpr int gsh = S == 0 ? 0 : (NB * 4 * nh * (int)sizeof(T) + 7) / 8;
     if (!PM) return ct + WPB * (gsh + stage_lds_doubles_per_wave<NF, T, AUX>());
     constexpr int work = gsh + WPB * stage_lds_doubles_per_wave<NF, T, AUX>();
@@ -270,7 +284,8 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
     const Geom& g = a.g;
     constexpr int s = S;
     constexpr bool aux = AUX;
-    using PT = Patch<WX, WY, WZ, WPB, PM>;
+    constexpr int CPW = stage_cells_per_wave<T, WX, MODE>();
+    using PT = Patch<WX, WY, WZ, WPB, PM, CPW>;
     constexpr int NB = PT::NB;
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int lane = threadIdx.x & 63;
@@ -279,7 +294,7 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
     const int npz = (g.n[2] + PT::PZ - 1) / PT::PZ;
     const int b = blockIdx.x;
     const int bgroup = (b & 7) * a.blocks_per_xcd + (b >> 3);
-    // PM: the block's patch; otherwise this wave's one-cell patch, WPB consecutive per block
+    // PM: the block's patch; otherwise this wave's patch of CPW cells, WPB consecutive per block
     const int patch = __builtin_amdgcn_readfirstlane(PM ? bgroup : bgroup * WPB + wave);
     if (patch >= npx * npy * npz) return; // uniform over the workgroup (PM) or the wave
     const int bx = (patch % npx) * PT::PX, by = ((patch / npx) % npy) * PT::PY, bz = (patch / (npx * npy)) * PT::PZ;
@@ -293,11 +308,18 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
     // (wave-uniform, so the particle arrays are addressed from scalar bases)
     const int beg = __builtin_amdgcn_readfirstlane(inside_domain ? a.cell_start[cell] : 0);
     const int end = __builtin_amdgcn_readfirstlane(inside_domain ? beg + a.cell_cnt[cell] : 0);
+    // CPW = 2: the wave's second cell, x + 1 (none past the domain's last column)
+    const bool inside1 = CPW == 2 && inside_domain && cx + 1 < g.n[0];
+    const int cell1 = inside1 ? cell + 1 : 0;
+    const int beg1 = __builtin_amdgcn_readfirstlane(inside1 ? a.cell_start[cell1] : 0);
+    const int end1 = __builtin_amdgcn_readfirstlane(inside1 ? beg1 + a.cell_cnt[cell1] : 0);
     // cell i of the neighbourhood box, box index (i, j, k) <-> the patch base
     // minus one in each 3-wide dimension, plus (i, j, k), periodic
     // (one conditional wrap covers a single cell's box: offsets are -1..+1 and a
     // 3-wide stencil implies at least 2 cells in that dimension)
-    auto wrapc = [](int x, int n) { return PM ? pmod(x, n) : (x < 0 ? x + n : (x >= n ? x - n : x)); };
+    auto wrapc = [](int x, int n) {
+        return PM || CPW == 2 ? pmod(x, n) : (x < 0 ? x + n : (x >= n ? x - n : x));
+    };
     auto box_cell = [&](int e) -> int64_t {
         const int i = e % PT::NBX, j = (e / PT::NBX) % PT::NBY, k = e / (PT::NBX * PT::NBY);
         const int ci = wrapc(bx - (WX == 3 ? 1 : 0) + i, g.n[0]);
@@ -305,7 +327,8 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
         const int ck = wrapc(bz - (WZ == 3 ? 1 : 0) + k, g.n[2]);
         return ((int64_t)ck * g.n[1] + cj) * g.n[0] + ci;
     };
-    // box index of this wave's stencil target t
+    // box index of this wave's stencil target t (of its first cell; the second
+    // cell's is one further along x)
     auto box_of = [&](int t) -> int {
         const int ii = t % WX, jj = (t / WX) % WY, kk = t / (WX * WY);
         return ((oz + kk) * PT::NBY + (oy + jj)) * PT::NBX + (ox + ii);
@@ -314,7 +337,7 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
     constexpr int per_wave = stage_lds_doubles_per_wave<NF, T, AUX>();
     static_assert(per_wave == 64 * 9 + 64 * NSR * (int)sizeof(T) / 8,
                   "LDS layout and its size function disagree");
-    constexpr int CT = stage_lds_ctab_doubles<WX, WY, WZ, WPB, PM>();
+    constexpr int CT = stage_lds_ctab_doubles<WX, WY, WZ, WPB, PM, CPW>();
     // [NB] the box's cell indices: the block's, or this wave's
     int* ctab = (int*)(PM ? lds : lds + wave * CT);
     double* lw = lds + (PM ? CT : WPB * CT);
@@ -416,11 +439,65 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
         for (int n = 0; n < NTL; n++) acc[m][n] = mf_d4{0.0, 0.0, 0.0, 0.0};
     }
     const bool deposits = s <= 3;
+    // The accumulators' deposit to the grid, for the cell whose box offset
+    // along x is w (0, or 1 for a wave's second cell).  For the f64 16x16x4
+    // product, element r of lane l holds D[16m + 4r + l/16][16n + l%16] (rows
+    // interleaved across the four 16-lane groups, unlike the f32 form's
+    // 4(l/16) + r).
+    auto flush_direct = [&](int w) {
+#pragma unroll
+        for (int m = 0; m < MT; m++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int t = 16 * m + 4 * r + (lane >> 4);
+                if (t >= NT) continue;
+                const int64_t tc = ctab[box_of(t) + w];
+#pragma unroll
+                for (int n = 0; n < NTF; n++) {
+                    const int q = 16 * n + (lane & 15);
+                    if (q < QG) unsafeAtomicAdd(&a.gout[tc * QG + q], acc[m][n][r]);
+                    else if (aux && q < QGA) unsafeAtomicAdd(&a.aux[tc * NAUX + (q - QG)], acc[m][n][r]);
+                }
+            }
+        if constexpr (SMALL) {
+#pragma unroll
+            for (int m = 0; m < MT; m++) {
+                const int t = 16 * m + 4 * ((lane >> 2) & 3) + (lane >> 4), q = 16 * NTF + (lane & 3);
+                if (t >= NT || q >= QGA) continue;
+                const int64_t tc = ctab[box_of(t) + w];
+                if (q < QG) unsafeAtomicAdd(&a.gout[tc * QG + q], accs[m]);
+                else unsafeAtomicAdd(&a.aux[tc * NAUX + (q - QG)], accs[m]);
+            }
+        }
+    };
 
-    for (int base = beg; base < end; base += 64) {
-        const int64_t ip = base + lane;
-        const bool act = ip < end;
-        const int64_t ipl = act ? ip : beg; // a valid index for inactive lanes' loads
+    // The wave's particles as a list of slots: the first cell's in 16-particle
+    // tiles (its last tile padded), then the second cell's (CPW = 2), so that a
+    // tile never mixes cells; 64 slots (four tiles) per chunk.  With one cell
+    // this is the cell's run, 64 particles at a time.
+    const int n0 = end - beg, n1 = end1 - beg1;
+    const int T0 = (n0 + 15) >> 4;        // tiles of the first cell
+    const int NSL = 16 * T0 + n1;         // slots in use
+    const int bany = n0 > 0 ? beg : beg1; // a particle of the wave, for inactive lanes' loads
+    // (one cell: the loop runs over the cell's run itself, base = beg + sbase)
+    const int lo = CPW == 1 ? beg : 0, hi = CPW == 1 ? end : NSL;
+    for (int base = lo; base < hi; base += 64) {
+        const int sbase = base - lo;
+        int64_t ip;
+        bool act;
+        int pcell;
+        if constexpr (CPW == 1) {
+            ip = base + lane;
+            act = ip < end;
+            pcell = cell;
+        } else {
+            const int sl = sbase + lane;
+            const bool first = sl < 16 * T0;
+            ip = first ? beg + sl : beg1 + (sl - 16 * T0);
+            act = first ? sl < n0 : sl < NSL;
+            pcell = first ? cell : cell1; // this lane's particle's cell
+        }
+        const int64_t ipl = act ? ip : (CPW == 1 ? beg : bany); // a valid index for inactive lanes' loads
         double pos[3];
 #pragma unroll
         for (int d = 0; d < 3; d++) pos[d] = a.cur.pos[d][ipl];
@@ -473,10 +550,13 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
                     const int c = 4 * cb + (lane & 3);
                     cqb[cb] = g4 * nh + (c < nh ? c : nh - 1);
                 }
-                const int ntile = (min(64, end - base) + 15) >> 4;
+                const int ntile = CPW == 1 ? (min(64, end - base) + 15) >> 4 : min(4, (NSL + 15 - sbase) >> 4);
 #pragma unroll 1
                 for (int m = 0; m < ntile; m++) {
                     const int p = 16 * m + c16; // the particle of this lane's A rows
+                    // the tile's cell's neighbourhood: the second cell's box is one
+                    // column further along x (box rows of QG values)
+                    const T* Gt = Gs + (CPW == 2 && (sbase >> 4) + m >= T0 ? QG : 0);
                     double w3[9];
 #pragma unroll
                     for (int e = 0; e < 9; e++) w3[e] = Wf[p * 9 + e];
@@ -508,7 +588,7 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
                             const T A = ax[ii] * ayz[kk * WY + jj];
 #pragma unroll
                             for (int cb = 0; cb < NCB; cb++)
-                                accb[cb] = __builtin_amdgcn_mfma_f64_4x4x4f64(A, Gs[box_of(t) * QG + cqb[cb]], accb[cb], 0, 0, 0);
+                                accb[cb] = __builtin_amdgcn_mfma_f64_4x4x4f64(A, Gt[box_of(t) * QG + cqb[cb]], accb[cb], 0, 0, 0);
                         }
                         const int pr = 16 * m + 4 * ((lane >> 2) & 3) + (lane >> 4);
 #pragma unroll
@@ -522,7 +602,7 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
                         for (int t = 0; t < NT; t++) {
                             const int ii = t % WX, jj = (t / WX) % WY, kk = t / (WX * WY);
                             const T A = ax[ii] * ayz[kk * WY + jj];
-                            const T B = Gs[box_of(t) * QG + cq];
+                            const T B = Gt[box_of(t) * QG + cq];
                             accg = MF::step(A, B, accg);
                         }
                         if (c16 < nh) {
@@ -627,7 +707,7 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
                 for (int d = 0; d < 3; d++) v[d] = double(pupv[d]) / double(pupt) * phys::c;
                 int nc;
                 end_position(pos, v, dt, g, npos, nc);
-                const int64_t j = (int64_t)a.mbase[(int64_t)cell * 27 + a.mv[ip]] + a.rank[ip];
+                const int64_t j = (int64_t)a.mbase[(int64_t)pcell * 27 + a.mv[ip]] + a.rank[ip];
                 if (j >= 0 && j < a.np) {
 #pragma unroll
                     for (int d = 0; d < 3; d++) a.dst.pos[d][j] = npos[d];
@@ -677,12 +757,16 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
         // FP32-bulk the flux sources q v are FP32 values like q itself, and the
         // sum over particles stays FP64.
         constexpr int KU = s == 0 ? 1 : 2;
-        const int nact = min(64, end - base);
-        const int nkg = (nact + 4 * KU - 1) / (4 * KU);
         // (32-bit LDS pointers made opaque once, so that the wave's base is not
         // split off again and re-added to every address in the loop)
         typedef __attribute__((address_space(3))) const double* lds_d;
         typedef __attribute__((address_space(3))) const T* lds_t;
+        // k-step iterations (4 KU slots each) up to the chunk's last particle.
+        // With two cells, the first cell is complete at the iteration where the
+        // second cell's tiles begin (kb, at a tile boundary): its deposit is
+        // flushed there and the accumulators start again from zero.
+        const int kend = ((CPW == 1 ? min(64, end - base) : min(NSL - sbase, 64)) + 4 * KU - 1) / (4 * KU);
+        const int kb = CPW == 2 && n0 > 0 && sbase < n0 && n0 <= sbase + 64 ? (16 * T0 - sbase) / (4 * KU) : -1;
         lds_d wa[MT][3];
         lds_t sb[NTL][2];
 #pragma unroll
@@ -700,7 +784,16 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
             asm volatile("" : "+v"(sb[n][1]));
         }
 #pragma unroll 1
-        for (int kp = 0; kp < nkg; kp++) {
+        for (int kp = 0; kp < kend; kp++) {
+            if (CPW == 2 && kp == kb) {
+                flush_direct(0);
+#pragma unroll
+                for (int m = 0; m < MT; m++) {
+                    accs[m] = 0.0;
+#pragma unroll
+                    for (int n = 0; n < NTL; n++) acc[m][n] = mf_d4{0.0, 0.0, 0.0, 0.0};
+                }
+            }
 #pragma unroll
             for (int h = 0; h < KU; h++) {
                 double A[MT];
@@ -729,41 +822,28 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
                 sb[n][1] += 4 * NSR * KU;
             }
         }
+        if (CPW == 2 && kb == kend) {
+            flush_direct(0);
+#pragma unroll
+            for (int m = 0; m < MT; m++) {
+                accs[m] = 0.0;
+#pragma unroll
+                for (int n = 0; n < NTL; n++) acc[m][n] = mf_d4{0.0, 0.0, 0.0, 0.0};
+            }
+        }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
     if (!deposits) return;
-    // For the f64 16x16x4 product, element r of lane l holds
-    // D[16m + 4r + l/16][16n + l%16] (rows interleaved across the four 16-lane
-    // groups, unlike the f32 form's 4(l/16) + r).
     if constexpr (!MERGE) {
-        // ---- flush this wave's deposit straight to the grid
-        if (end <= beg) return;
-#pragma unroll
-        for (int m = 0; m < MT; m++)
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int t = 16 * m + 4 * r + (lane >> 4);
-                if (t >= NT) continue;
-                const int64_t tc = ctab[box_of(t)];
-#pragma unroll
-                for (int n = 0; n < NTF; n++) {
-                    const int q = 16 * n + (lane & 15);
-                    if (q < QG) unsafeAtomicAdd(&a.gout[tc * QG + q], acc[m][n][r]);
-                    else if (aux && q < QGA) unsafeAtomicAdd(&a.aux[tc * NAUX + (q - QG)], acc[m][n][r]);
-                }
-            }
-        if constexpr (SMALL) {
-#pragma unroll
-            for (int m = 0; m < MT; m++) {
-                const int t = 16 * m + 4 * ((lane >> 2) & 3) + (lane >> 4), q = 16 * NTF + (lane & 3);
-                if (t >= NT || q >= QGA) continue;
-                const int64_t tc = ctab[box_of(t)];
-                if (q < QG) unsafeAtomicAdd(&a.gout[tc * QG + q], accs[m]);
-                else unsafeAtomicAdd(&a.aux[tc * NAUX + (q - QG)], accs[m]);
-            }
+        // ---- flush this wave's (last) cell's deposit straight to the grid
+        if constexpr (CPW == 2) {
+            if (n1 > 0) flush_direct(1);
+            return;
         }
+        if (end <= beg) return;
+        flush_direct(0);
         return;
     }
     // ---- merge the waves' deposits over the box in LDS, then flush

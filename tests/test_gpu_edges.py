@@ -172,3 +172,18 @@ def test_slot_layouts_agree(core, oracle, nf, prec):
     assert ib["steps"] == ia["steps"] == 8
     assert ib["dt"] == pytest.approx(ia["dt"], rel=1e-12)
     assert ib["time"] == pytest.approx(ia["time"], rel=1e-12)
+
+
+@pytest.mark.parametrize("nf", [2, 3])
+@pytest.mark.parametrize("ncell", [(4, 4, 4), (3, 5, 2), (2, 3, 7), (5, 2, 4)])
+def test_fp32_bulk_domain_shapes(core, oracle, nf, ncell):
+    """FP32-bulk takes two x-adjacent cells per wave: an odd x count leaves the
+    last column's wave one cell, and a 2-cell x dimension makes the pair's
+    4-wide box hold each periodic image twice.  6 steps with a dt that moves
+    particles across cells, against the oracle.  (The boxes are ones where the
+    oracle's own steps keep a one-ulp change at 1e-15: 5 x 2 x 3, for one,
+    grows it to 7e-11, which FP32's rounding would turn into 1e-2.)"""
+    d = cases.fast_flavor_dict(nf=nf, ncell=ncell, nphi=8, st4_amplitude=1e-3)
+    deck, prm, emu = make(core, oracle, d)
+    parts = oracle.init_particles(deck)
+    run_fused_against_oracle(core, oracle, nf, prm, emu, parts, "fp32", 6, 0.3 * 0.5 / oracle.clight, "aligned")

@@ -69,3 +69,17 @@ def test_adaptive_speedup_amplification(oracle, nf):
     assert amplification(oracle, nf, cases.fast_flavor_dict(**kw, max_adaptive_speedup=4.0), 3, None) < 1e-14
     big = amplification(oracle, nf, cases.fast_flavor_dict(**kw, max_adaptive_speedup=1e9), 3, None)
     assert 1e-12 < big < 1e-10
+
+
+@pytest.mark.parametrize("nf", [2, 3])
+@pytest.mark.parametrize("ncell", [(4, 4, 4), (3, 5, 2), (2, 3, 7), (5, 2, 4)])
+def test_fp32_shape_configs_well_conditioned(oracle, nf, ncell):
+    """The boxes of the FP32-bulk domain-shape test (amplitude 1e-3)."""
+    d = cases.fast_flavor_dict(nf=nf, ncell=ncell, nphi=8, st4_amplitude=1e-3)
+    assert amplification(oracle, nf, d, 6, FIXED) < 1e-14
+
+
+def test_ill_conditioned_box_exists(oracle):
+    """5 x 2 x 3 at the same settings is not usable for a parity test."""
+    d = cases.fast_flavor_dict(nf=2, ncell=(5, 2, 3), nphi=8, st4_amplitude=1e-3)
+    assert amplification(oracle, 2, d, 6, FIXED) > 1e-12
