@@ -2,8 +2,8 @@
 // RK update of every particle of a cell, and the deposit of the next stage
 // state as one small matrix product per cell on the FP64 matrix cores.
 //
-// One wave owns one cell at a time and takes the cell's particles 64 at a
-// time (lane = particle).
+// One wave owns one cell (FP32-bulk's RK stages: two x-adjacent cells) and
+// takes its particles 64 at a time (lane = particle).
 //
 // Gather (MFMA): with s(p) = (1, -vx, -vy, -vz) and G_g(t, c) the four field
 // groups (M, Phix, Phiy, Phiz) of neighbour t,
@@ -202,13 +202,36 @@ template <int WX, int WY, int WZ, int WPB, bool PM, int CPW = 1> struct Patch {
     static constexpr int NB = NBX * NBY * NBZ;
 };
 
-// LDS of one wave's own region, in doubles: 9 shape factors per particle
-// (double), the source row (NH + 4 values of T) and, with AUX, NF + NF(NF-1)/2
-// auxiliary sources (T).
-template <int NF, typename T, bool AUX> constexpr int stage_lds_doubles_per_wave()
+// compute_dt auxiliaries per cell.  AUX = 1: the diagonal sums N_ii + Nbar_ii
+// (i < NF), all that the conservative flavour bound needs; AUX = 2: also
+// Nbar Im(fbar_ab) v_d per pair a < b and direction d, for the adaptive bound
+// (max_adaptive_speedup > 1).  Deposited columns, and sources per particle:
+// N f_ii + Nbar fbar_ii, then Nbar Im(fbar_ab).
+template <int NF> constexpr int aux_columns(int AUX)
 {
-    constexpr int nh = NF * NF;
-    return 64 * 9 + 64 * (nh + 4) * (int)sizeof(T) / 8 + (AUX ? 64 * (NF + NF * (NF - 1) / 2) * (int)sizeof(T) / 8 : 0);
+    return AUX == 0 ? 0 : (AUX == 1 ? NF : NF + 3 * (NF * (NF - 1) / 2));
+}
+template <int NF> constexpr int aux_sources(int AUX)
+{
+    return AUX == 0 ? 0 : (AUX == 1 ? NF : NF + NF * (NF - 1) / 2);
+}
+
+// Values of T per particle in the LDS source rows: NH + 4 main sources, the
+// auxiliary ones, and one of padding where that count is even.  The deposit
+// reads one column of four consecutive rows at once; an odd row length keeps
+// those rows off the same LDS banks (16 doubles = 128 bytes would put every
+// row on bank 0).
+template <int NF> constexpr int stage_source_row(int AUX)
+{
+    const int n = NF * NF + 4 + aux_sources<NF>(AUX);
+    return n % 2 == 0 ? n + 1 : n;
+}
+
+// LDS of one wave's own region, in doubles: 9 shape factors per particle
+// (double) and the source row (T).
+template <int NF, typename T, int AUX> constexpr int stage_lds_doubles_per_wave()
+{
+    return 64 * 9 + 64 * stage_source_row<NF>(AUX) * (int)sizeof(T) / 8;
 }
 
 // LDS of the table of the neighbourhood box's cell indices (NB ints), in
@@ -232,7 +255,7 @@ template <typename T, int WX, int MODE> constexpr int stage_cells_per_wave()
 // deposited columns, double) reuses that space once every wave is done with
 // it.  Without PM: per wave, its own box and its region.  The host sizes the
 // launch with this function.
-template <int NF, typename T, int WX, int WY, int WZ, int WPB, int S, bool AUX, int MODE>
+template <int NF, typename T, int WX, int WY, int WZ, int WPB, int S, int AUX, int MODE>
 constexpr int stage_lds_doubles_per_block()
 {
     constexpr bool PM = MODE >= 1;
@@ -243,7 +266,7 @@ constexpr int stage_lds_doubles_per_block()
     constexpr int gsh = S == 0 ? 0 : (NB * 4 * nh * (int)sizeof(T) + 7) / 8;
     if (!PM) return ct + WPB * (gsh + stage_lds_doubles_per_wave<NF, T, AUX>());
     constexpr int work = gsh + WPB * stage_lds_doubles_per_wave<NF, T, AUX>();
-    constexpr int merge = MODE == 2 && S <= 3 ? NB * (4 * nh + (AUX ? NF + 3 * (NF * (NF - 1) / 2) : 0)) : 0;
+    constexpr int merge = MODE == 2 && S <= 3 ? NB * (4 * nh + aux_columns<NF>(AUX)) : 0;
     return ct + (work > merge ? work : merge);
 }
 
@@ -251,15 +274,16 @@ constexpr int stage_lds_doubles_per_block()
 // NF = 3): the deposit-only launch in both precisions and FP32-bulk stage 4
 // merge their deposits (MODE 2); the other RK stages run faster with
 // independent waves
-template <typename T, int S, bool AUX> constexpr int stage_mode()
+template <typename T, int S, int AUX> constexpr int stage_mode()
 {
     return S == 0 || (S == 4 && sizeof(T) == 4) ? 2 : 0;
 }
 template <int MODE> constexpr int stage_waves_per_block() { return MODE == 2 ? 4 : 2; }
 
 // S: 0 = deposit only (the step's result, or the initial state), 1..4 = RK
-// stage; AUX: also deposit the compute_dt auxiliaries (S = 0 or 3).
-template <int NF, typename T, int WX, int WY, int WZ, int WPB, int S, bool AUX, int MODE>
+// stage; AUX: also deposit the compute_dt auxiliaries (S = 0 or 3; 1 or 2 as
+// for aux_columns).
+template <int NF, typename T, int WX, int WY, int WZ, int WPB, int S, int AUX, int MODE>
 __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 2))) void k_stage(StageArgs<T> a)
 {
     constexpr bool PM = MODE >= 1;    // the workgroup shares one patch and its staged box
@@ -268,22 +292,24 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
     constexpr int QG = 4 * nh;
     constexpr int NT = WX * WY * WZ;        // target cells of a source cell
     constexpr int MT = (NT + 15) / 16;       // MFMA row tiles
-    constexpr int QGA = QG + (AUX ? NF + 3 * (NF * (NF - 1) / 2) : 0); // deposited columns
+    constexpr int QGA = QG + aux_columns<NF>(AUX); // deposited columns
     constexpr int NTL = (QGA + 15) / 16;     // MFMA column tiles
     constexpr int NPAIR = NF * (NF - 1) / 2;
-    // compute_dt auxiliaries per cell: N_ii + Nbar_ii (i < NF), then Nbar Im(fbar_ab) v_d
-    // per pair a < b and direction d.  N_ii and Nbar_ii separately follow from the sum
-    // and M_ii = N_ii - Nbar_ii (+ matter on 00).  They ride in the deposit's spare
-    // MFMA columns (QG .. QG + NAUX - 1): at NF = 3, 36 + 12 = 3 column tiles exactly.
+    // compute_dt auxiliaries per cell (aux_columns): N_ii + Nbar_ii, then (AUX = 2)
+    // Nbar Im(fbar_ab) v_d.  N_ii and Nbar_ii separately follow from the sum and
+    // M_ii = N_ii - Nbar_ii (+ matter on 00).  They are deposited as columns
+    // QG .. QGA - 1 of the same product, into rows of NAUX (the full set's
+    // count, whichever part is deposited).
     constexpr int NAUX = NF + 3 * NPAIR;
     constexpr int NSRC = nh + 4;             // q_c, vhat and the constant 1 per particle
-    constexpr int NSAUX = NF + NPAIR;        // N f_ii + Nbar fbar_ii, Nbar Im(fbar_ab)
-    constexpr int NSR = NSRC + (AUX ? NSAUX : 0); // a particle's source row: main, then auxiliary
+    constexpr int NSAUX = aux_sources<NF>(AUX);
+    constexpr int NSR = stage_source_row<NF>(AUX); // a particle's source row: main, auxiliary, padding
+    static_assert(NSR >= NSRC + NSAUX, "source row");
     extern __shared__ double lds[];
 
     const Geom& g = a.g;
     constexpr int s = S;
-    constexpr bool aux = AUX;
+    constexpr bool aux = AUX != 0;
     constexpr int CPW = stage_cells_per_wave<T, WX, MODE>();
     using PT = Patch<WX, WY, WZ, WPB, PM, CPW>;
     constexpr int NB = PT::NB;
@@ -405,18 +431,23 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
         tdx[m][1] = 3 + (WY == 1 ? 1 : jj);
         tdx[m][2] = 6 + (WZ == 1 ? 1 : kk);
     }
-    // A last column tile of at most 4 columns (NF = 3 without the auxiliaries: 36 =
-    // 2 x 16 + 4) runs on the 4-block 4x4x4 form instead: two 16-cycle steps in
-    // place of two 64-cycle 16x16x4 steps.  With blocks as the row quarters
-    // t = 16m + 4b + i, its A operand is lane for lane the 16x16x4 one; its B
-    // column is 16 NTF + l%4, and its result D[t][q] sits in lane 16i + 4b + j.
+    // The last columns, when at most 8 remain past the 16-column tiles (NF = 3:
+    // 36 = 2 x 16 + 4 without the auxiliaries, 39 = 2 x 16 + 7 with the
+    // diagonal ones), run as 4-column tiles on the 4-block 4x4x4 form instead:
+    // 16-cycle steps in place of a 64-cycle 16x16x4 step.  With blocks as the
+    // row quarters t = 16m + 4b + i, its A operand is lane for lane the 16x16x4
+    // one; the B column of small tile k is 16 NTF + 4k + l%4, and its result
+    // D[t][q] sits in lane 16i + 4b + j.
     constexpr int QREM = QGA % 16;
-    constexpr bool SMALL = QREM != 0 && QREM <= 4;
-    constexpr int NTF = SMALL ? NTL - 1 : NTL; // 16-column tiles
-    int bi1[NTL], bi2[NTL];
+    constexpr bool SMALL = QREM != 0 && QREM <= 8;
+    constexpr int NTF = QGA / 16 + (SMALL || QREM == 0 ? 0 : 1); // 16-column tiles
+    constexpr int NSM = SMALL ? (QREM + 3) / 4 : 0;               // 4-column tiles
+    constexpr int NCG = NTF + NSM;                                // column groups of the B operand
+    static_assert(NTL == NTF + (SMALL ? 1 : 0), "column tiles");
+    int bi1[NCG], bi2[NCG];
 #pragma unroll
-    for (int n = 0; n < NTL; n++) {
-        const int q = SMALL && n == NTF ? 16 * n + (lane & 3) : 16 * n + (lane & 15);
+    for (int n = 0; n < NCG; n++) {
+        const int q = n >= NTF ? 16 * NTF + 4 * (n - NTF) + (lane & 3) : 16 * n + (lane & 15);
         if (q < QG) {
             const int f = q / nh, c = q % nh;
             bi1[n] = c;
@@ -430,14 +461,18 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
             bi2[n] = nh + 3;
         }
     }
-    mf_d4 acc[MT][NTL];
-    double accs[MT]; // the 4x4x4 tile
+    mf_d4 acc[MT][NTF];
+    double accs[MT][NSM > 0 ? NSM : 1]; // the 4x4x4 tiles
+    auto zero_acc = [&]() {
 #pragma unroll
-    for (int m = 0; m < MT; m++) {
-        accs[m] = 0.0;
+        for (int m = 0; m < MT; m++) {
 #pragma unroll
-        for (int n = 0; n < NTL; n++) acc[m][n] = mf_d4{0.0, 0.0, 0.0, 0.0};
-    }
+            for (int k = 0; k < NSM; k++) accs[m][k] = 0.0;
+#pragma unroll
+            for (int n = 0; n < NTF; n++) acc[m][n] = mf_d4{0.0, 0.0, 0.0, 0.0};
+        }
+    };
+    zero_acc();
     const bool deposits = s <= 3;
     // The accumulators' deposit to the grid, for the cell whose box offset
     // along x is w (0, or 1 for a wave's second cell).  For the f64 16x16x4
@@ -459,16 +494,16 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
                     else if (aux && q < QGA) unsafeAtomicAdd(&a.aux[tc * NAUX + (q - QG)], acc[m][n][r]);
                 }
             }
-        if constexpr (SMALL) {
 #pragma unroll
-            for (int m = 0; m < MT; m++) {
-                const int t = 16 * m + 4 * ((lane >> 2) & 3) + (lane >> 4), q = 16 * NTF + (lane & 3);
+        for (int m = 0; m < MT; m++)
+#pragma unroll
+            for (int k = 0; k < NSM; k++) {
+                const int t = 16 * m + 4 * ((lane >> 2) & 3) + (lane >> 4), q = 16 * NTF + 4 * k + (lane & 3);
                 if (t >= NT || q >= QGA) continue;
                 const int64_t tc = ctab[box_of(t) + w];
-                if (q < QG) unsafeAtomicAdd(&a.gout[tc * QG + q], accs[m]);
-                else unsafeAtomicAdd(&a.aux[tc * NAUX + (q - QG)], accs[m]);
+                if (q < QG) unsafeAtomicAdd(&a.gout[tc * QG + q], accs[m][k]);
+                else unsafeAtomicAdd(&a.aux[tc * NAUX + (q - QG)], accs[m][k]);
             }
-        }
     };
 
     // The wave's particles as a list of slots: the first cell's in 16-particle
@@ -737,11 +772,14 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
 #pragma unroll
             for (int x = 0; x < NF; x++)
                 Src[lane * NSR + NSRC + x] = act ? N * Dep[herm_re<NF>(x, x)] + Nb * Dep[nh + herm_re<NF>(x, x)] : T(0);
-            int o = NF;
+            if constexpr (AUX == 2) {
+                int o = NF;
 #pragma unroll
-            for (int x = 0; x < NF; x++)
+                for (int x = 0; x < NF; x++)
 #pragma unroll
-                for (int y2 = x + 1; y2 < NF; y2++) Src[lane * NSR + NSRC + (o++)] = act ? Nb * Dep[nh + herm_im<NF>(x, y2)] : T(0);
+                    for (int y2 = x + 1; y2 < NF; y2++)
+                        Src[lane * NSR + NSRC + (o++)] = act ? Nb * Dep[nh + herm_im<NF>(x, y2)] : T(0);
+            }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -768,7 +806,7 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
         const int kend = ((CPW == 1 ? min(64, end - base) : min(NSL - sbase, 64)) + 4 * KU - 1) / (4 * KU);
         const int kb = CPW == 2 && n0 > 0 && sbase < n0 && n0 <= sbase + 64 ? (16 * T0 - sbase) / (4 * KU) : -1;
         lds_d wa[MT][3];
-        lds_t sb[NTL][2];
+        lds_t sb[NCG][2];
 #pragma unroll
         for (int m = 0; m < MT; m++)
 #pragma unroll
@@ -777,7 +815,7 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
                 asm volatile("" : "+v"(wa[m][d]));
             }
 #pragma unroll
-        for (int n = 0; n < NTL; n++) {
+        for (int n = 0; n < NCG; n++) {
             sb[n][0] = (lds_t)(Src + (lane >> 4) * NSR + bi1[n]);
             sb[n][1] = (lds_t)(Src + (lane >> 4) * NSR + bi2[n]);
             asm volatile("" : "+v"(sb[n][0]));
@@ -787,12 +825,7 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
         for (int kp = 0; kp < kend; kp++) {
             if (CPW == 2 && kp == kb) {
                 flush_direct(0);
-#pragma unroll
-                for (int m = 0; m < MT; m++) {
-                    accs[m] = 0.0;
-#pragma unroll
-                    for (int n = 0; n < NTL; n++) acc[m][n] = mf_d4{0.0, 0.0, 0.0, 0.0};
-                }
+                zero_acc();
             }
 #pragma unroll
             for (int h = 0; h < KU; h++) {
@@ -800,11 +833,12 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
 #pragma unroll
                 for (int m = 0; m < MT; m++) A[m] = wa[m][0][36 * h] * wa[m][1][36 * h] * wa[m][2][36 * h];
 #pragma unroll
-                for (int n = 0; n < NTL; n++) {
+                for (int n = 0; n < NCG; n++) {
                     const double B = double(sb[n][0][4 * NSR * h] * sb[n][1][4 * NSR * h]);
-                    if (SMALL && n == NTF) {
+                    if (n >= NTF) {
 #pragma unroll
-                        for (int m = 0; m < MT; m++) accs[m] = __builtin_amdgcn_mfma_f64_4x4x4f64(A[m], B, accs[m], 0, 0, 0);
+                        for (int m = 0; m < MT; m++)
+                            accs[m][n - NTF] = __builtin_amdgcn_mfma_f64_4x4x4f64(A[m], B, accs[m][n - NTF], 0, 0, 0);
                     } else {
 #pragma unroll
                         for (int m = 0; m < MT; m++)
@@ -817,19 +851,14 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
 #pragma unroll
                 for (int d = 0; d < 3; d++) wa[m][d] += 36 * KU;
 #pragma unroll
-            for (int n = 0; n < NTL; n++) {
+            for (int n = 0; n < NCG; n++) {
                 sb[n][0] += 4 * NSR * KU;
                 sb[n][1] += 4 * NSR * KU;
             }
         }
         if (CPW == 2 && kb == kend) {
             flush_direct(0);
-#pragma unroll
-            for (int m = 0; m < MT; m++) {
-                accs[m] = 0.0;
-#pragma unroll
-                for (int n = 0; n < NTL; n++) acc[m][n] = mf_d4{0.0, 0.0, 0.0, 0.0};
-            }
+            zero_acc();
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -865,13 +894,13 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
                     if (q < QGA) unsafeAtomicAdd(&Mb[bi * QGA + q], acc[m][n][r]);
                 }
             }
-        if constexpr (SMALL) {
 #pragma unroll
-            for (int m = 0; m < MT; m++) {
-                const int t = 16 * m + 4 * ((lane >> 2) & 3) + (lane >> 4), q = 16 * NTF + (lane & 3);
-                if (t < NT && q < QGA) unsafeAtomicAdd(&Mb[box_of(t) * QGA + q], accs[m]);
+        for (int m = 0; m < MT; m++)
+#pragma unroll
+            for (int k = 0; k < NSM; k++) {
+                const int t = 16 * m + 4 * ((lane >> 2) & 3) + (lane >> 4), q = 16 * NTF + 4 * k + (lane & 3);
+                if (t < NT && q < QGA) unsafeAtomicAdd(&Mb[box_of(t) * QGA + q], accs[m][k]);
             }
-        }
     }
     __syncthreads();
     for (int e = threadIdx.x; e < NB * QGA; e += 64 * WPB) {

@@ -170,7 +170,7 @@ template <typename T> using Arrays = PArrays<T>;
 // and Nbar_ii = (S_ii - M_ii + m00 d_i0) / 2, then Im(Fbar_ab) per direction.
 template <int NF>
 __global__ void k_dt_reduce(const double* __restrict__ grid, const double* __restrict__ aux, int64_t ncell,
-                            double cell_volume, double m00, unsigned long long* red)
+                            double cell_volume, double m00, int adaptive, unsigned long long* red)
 {
     constexpr int nh = NH<NF>;
     constexpr int QG = 4 * nh;
@@ -180,25 +180,29 @@ __global__ void k_dt_reduce(const double* __restrict__ grid, const double* __res
     if (t < ncell) {
         const double* G = grid + t * QG;
         const double* A = aux + t * NAUX;
-        double h[nh];
+        // the adaptive bound (only where max_adaptive_speedup > 1 uses it: the
+        // stage kernel then deposits Im(Fbar) as well)
+        if (adaptive) {
+            double h[nh];
 #pragma unroll
-        for (int c = 0; c < nh; c++) h[c] = G[c];
-        double V2 = su_mag2<NF, double>(h);
+            for (int c = 0; c < nh; c++) h[c] = G[c];
+            double V2 = su_mag2<NF, double>(h);
 #pragma unroll
-        for (int d = 0; d < 3; d++) {
+            for (int d = 0; d < 3; d++) {
 #pragma unroll
-            for (int c = 0; c < nh; c++) h[c] = G[(d + 1) * nh + c];
-            int o = NF;
+                for (int c = 0; c < nh; c++) h[c] = G[(d + 1) * nh + c];
+                int o = NF;
 #pragma unroll
-            for (int x = 0; x < NF; x++)
+                for (int x = 0; x < NF; x++)
 #pragma unroll
-                for (int y = x + 1; y < NF; y++) {
-                    h[herm_im<NF>(x, y)] -= 2.0 * A[o + d];
-                    o += 3;
-                }
-            V2 += su_mag2<NF, double>(h);
+                    for (int y = x + 1; y < NF; y++) {
+                        h[herm_im<NF>(x, y)] -= 2.0 * A[o + d];
+                        o += 3;
+                    }
+                V2 += su_mag2<NF, double>(h);
+            }
+            Vad = sqrt(V2) * M_SQRT2 * phys::GF / cell_volume;
         }
-        Vad = sqrt(V2) * M_SQRT2 * phys::GF / cell_volume;
 #pragma unroll
         for (int x = 0; x < NF; x++) {
             const double Mxx = G[herm_re<NF>(x, x)] - (x == 0 ? m00 : 0.0);
@@ -563,7 +567,7 @@ int scan_counts(emu_state* st, const int* in, int* out)
 }
 
 
-template <int NF, typename T, int WX, int WY, int WZ, int S, bool AUX>
+template <int NF, typename T, int WX, int WY, int WZ, int S, int AUX>
 void launch_stage_k(emu_state* st, StageArgs<T>& a)
 {
     constexpr int MODE = stage_mode<T, S, AUX>();
@@ -583,19 +587,22 @@ void launch_stage_k(emu_state* st, StageArgs<T>& a)
 template <int NF, typename T, int WX, int WY, int WZ>
 void launch_stage_t(emu_state* st, StageArgs<T>& a)
 {
-    const int aux = a.want_aux && (a.stage == 3 || a.stage == 0);
+    // the compute_dt auxiliaries ride on the deposit that compute_dt reads
+    const int aux = a.stage == 3 || a.stage == 0 ? a.want_aux : 0;
     switch (a.stage) {
     case 0:
-        if (aux) launch_stage_k<NF, T, WX, WY, WZ, 0, true>(st, a);
-        else launch_stage_k<NF, T, WX, WY, WZ, 0, false>(st, a);
+        if (aux == 2) launch_stage_k<NF, T, WX, WY, WZ, 0, 2>(st, a);
+        else if (aux == 1) launch_stage_k<NF, T, WX, WY, WZ, 0, 1>(st, a);
+        else launch_stage_k<NF, T, WX, WY, WZ, 0, 0>(st, a);
         break;
-    case 1: launch_stage_k<NF, T, WX, WY, WZ, 1, false>(st, a); break;
-    case 2: launch_stage_k<NF, T, WX, WY, WZ, 2, false>(st, a); break;
+    case 1: launch_stage_k<NF, T, WX, WY, WZ, 1, 0>(st, a); break;
+    case 2: launch_stage_k<NF, T, WX, WY, WZ, 2, 0>(st, a); break;
     case 3:
-        if (aux) launch_stage_k<NF, T, WX, WY, WZ, 3, true>(st, a);
-        else launch_stage_k<NF, T, WX, WY, WZ, 3, false>(st, a);
+        if (aux == 2) launch_stage_k<NF, T, WX, WY, WZ, 3, 2>(st, a);
+        else if (aux == 1) launch_stage_k<NF, T, WX, WY, WZ, 3, 1>(st, a);
+        else launch_stage_k<NF, T, WX, WY, WZ, 3, 0>(st, a);
         break;
-    default: launch_stage_k<NF, T, WX, WY, WZ, 4, false>(st, a); break;
+    default: launch_stage_k<NF, T, WX, WY, WZ, 4, 0>(st, a); break;
     }
 }
 
@@ -635,7 +642,11 @@ StageArgs<T> stage_args(emu_state* st, int stage, const double* gin, double* gou
     a.np = st->cap;
     a.ncell = st->ncell;
     a.stage = stage;
-    a.want_aux = want_aux ? 1 : 0;
+    // the diagonal sums for the conservative flavour bound; Im(Fbar) as well
+    // where the adaptive bound is used (max_adaptive_speedup > 1).  FP64 deposits
+    // the full set either way: measured at 64^3 x 92, its auxiliary stage ran
+    // 2% slower with the diagonal sums alone (FP32-bulk's 1% faster).
+    a.want_aux = want_aux ? (st->ctx->p.max_adaptive_speedup > 1 || sizeof(T) == 8 ? 2 : 1) : 0;
     a.blocks_per_xcd = 0;
     a.g = st->ctx->g;
     const emu_ctx* ctx = st->ctx;
@@ -673,13 +684,14 @@ int dt_from_grid(emu_state* st, const double* g, double fixed_dt)
     const emu_params& p = ctx->p;
     hipLaunchKernelGGL(k_fill_u64v, dim3(1), dim3(64), 0, ctx->stream, st->red, 2, 0ull);
     const bool use_flavor = p.flavor_cfl_factor > 0.0;
+    const int adaptive = p.max_adaptive_speedup > 1 ? 1 : 0;
     if (use_flavor && fixed_dt <= 0) {
         if (st->nf == 2)
             hipLaunchKernelGGL(k_dt_reduce<2>, dim3(grid_blocks(st->ncell)), dim3(256), 0, ctx->stream, g, st->aux,
-                               st->ncell, ctx->g.cell_volume * st->nscale, st->m00 * st->nscale, st->red);
+                               st->ncell, ctx->g.cell_volume * st->nscale, st->m00 * st->nscale, adaptive, st->red);
         else
             hipLaunchKernelGGL(k_dt_reduce<3>, dim3(grid_blocks(st->ncell)), dim3(256), 0, ctx->stream, g, st->aux,
-                               st->ncell, ctx->g.cell_volume * st->nscale, st->m00 * st->nscale, st->red);
+                               st->ncell, ctx->g.cell_volume * st->nscale, st->m00 * st->nscale, adaptive, st->red);
     }
     DtParams dp;
     dp.dt_translation = p.cfl_factor > 0.0
