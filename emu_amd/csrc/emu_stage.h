@@ -177,8 +177,9 @@ __device__ __forceinline__ void end_position(const double pos[3], const double v
 }
 
 // Three ways to map waves to cells (MODE).
-//  0: every wave is its own patch of one cell (box = its 27 neighbours), stages
-//     its own box and flushes its deposit directly: no workgroup barrier.
+//  0: every wave is its own patch of one cell (box = its 27 neighbours; two
+//     cells in FP32-bulk, below), stages its own box and flushes its deposit
+//     directly: no workgroup barrier.
 //  1: the WPB waves of a workgroup take a patch of WPB cells (2 along x, then
 //     2 along y, when the stencil is 3 wide there, else along z); the patch's
 //     neighbourhood box (PX + WX - 1) x (PY + WY - 1) x (PZ + WZ - 1) is staged
@@ -278,7 +279,11 @@ template <typename T, int S, int AUX> constexpr int stage_mode()
 {
     return S == 0 || (S == 4 && sizeof(T) == 4) ? 2 : 0;
 }
-template <int MODE> constexpr int stage_waves_per_block() { return MODE == 2 ? 4 : 2; }
+// waves per workgroup: a patch of 4 (MODE 2); otherwise 2, or 1 for the launches
+// with the compute_dt auxiliaries, whose larger LDS rows would otherwise leave
+// a CU a whole 2-wave workgroup short (FP64 at 64^3 x 92: 6 waves per CU with
+// 2-wave workgroups, 7 with single waves)
+template <int MODE, int AUX> constexpr int stage_waves_per_block() { return MODE == 2 ? 4 : (AUX != 0 ? 1 : 2); }
 
 // S: 0 = deposit only (the step's result, or the initial state), 1..4 = RK
 // stage; AUX: also deposit the compute_dt auxiliaries (S = 0 or 3; 1 or 2 as

@@ -572,7 +572,7 @@ void launch_stage_k(emu_state* st, StageArgs<T>& a)
 {
     constexpr int MODE = stage_mode<T, S, AUX>();
     constexpr bool PM = MODE >= 1;
-    constexpr int WPB = stage_waves_per_block<MODE>(); // waves per workgroup
+    constexpr int WPB = stage_waves_per_block<MODE, AUX>(); // waves per workgroup
     using PT = Patch<WX, WY, WZ, WPB, PM, stage_cells_per_wave<T, WX, MODE>()>;
     const Geom& g = st->ctx->g;
     const int64_t npatch = (int64_t)((g.n[0] + PT::PX - 1) / PT::PX) * ((g.n[1] + PT::PY - 1) / PT::PY) *
