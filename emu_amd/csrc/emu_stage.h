@@ -70,8 +70,7 @@ template <typename T> struct StageArgs {
     const unsigned char* mv;
     const int* cell_start; // first slot of each cell (a multiple of the slot alignment)
     const int* cell_cnt;   // particles of each cell
-    const double* gin;  // grid read by the gather (FP64 path)
-    const float* gin32; // its FP32 copy (FP32-bulk path)
+    const double* gin;  // grid read by the gather
     double* gout;       // grid deposited by this launch
     double* aux;        // compute_dt auxiliaries
     const double* dt;
@@ -393,17 +392,16 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
     if constexpr (s != 0) {
-        // stage the box's fields, all waves together: NB rows of QG values
-        const T* gsrc;
-        if constexpr (sizeof(T) == 8) gsrc = (const T*)a.gin;
-        else gsrc = (const T*)a.gin32;
+        // stage the box's fields, all waves together: NB rows of QG values, in
+        // the working precision (FP32-bulk rounds the FP64 deposit here)
+        const double* gsrc = a.gin;
         // a compile-time trip count, unrolled, so that every load is in flight at once
 #pragma unroll
         for (int e0 = 0; e0 < NB * QG; e0 += STRIDE) {
             const int e = e0 + tid;
             if (e < NB * QG) {
                 const int r = e / QG, q = e - r * QG;
-                Gs[e] = gsrc[(int64_t)ctab[r] * QG + q];
+                Gs[e] = T(gsrc[(int64_t)ctab[r] * QG + q]);
             }
         }
         if constexpr (PM) {

@@ -56,13 +56,6 @@ __global__ void k_fill_grid(double* g, int64_t ncell, int Q, double m00, double*
 // particle's destination is base[old cell][e] + (its rank among the particles
 // of its old cell with the same e), which makes the order deterministic.
 // ---------------------------------------------------------------------------
-// FP32 copy of a deposited grid for the FP32-bulk gather.
-__global__ void k_grid_to_f32(const double* __restrict__ g, float* __restrict__ g32, int64_t n)
-{
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
-        g32[t] = (float)g[t];
-}
-
 __device__ __forceinline__ int move1(int from, int to, int n, int* bad)
 {
     const int dd = pmod(to - from, n);
@@ -492,7 +485,6 @@ struct emu_state {
     int* mbase = nullptr;          // [ncell][27] destination of the first of them
     unsigned char* mv = nullptr;   // move index per particle
     double* grid[2] = {nullptr, nullptr};
-    float* grid32 = nullptr;       // FP32 copy for the FP32-bulk gather
     double* aux = nullptr;
     double* scal = nullptr;        // [0] time [1] dt [2] dt_next
     int64_t* steps = nullptr;
@@ -634,7 +626,6 @@ StageArgs<T> stage_args(emu_state* st, int stage, const double* gin, double* gou
     a.cell_start = st->cell_start;
     a.cell_cnt = st->cell_cnt;
     a.gin = gin;
-    a.gin32 = st->grid32;
     a.gout = gout;
     a.aux = st->aux;
     a.dt = st->scal + 1;
@@ -659,9 +650,6 @@ StageArgs<T> stage_args(emu_state* st, int stage, const double* gin, double* gou
 void launch_stage(emu_state* st, int stage, const double* gin, double* gout, bool want_aux)
 {
     if (st->precision == EMU_FP32_BULK) {
-        if (gin) // the FP32-bulk gather reads an FP32 copy of the grid
-            hipLaunchKernelGGL(k_grid_to_f32, dim3(grid_blocks(st->ncell * st->QG)), dim3(256), 0, st->ctx->stream,
-                               gin, st->grid32, st->ncell * st->QG);
         auto a = stage_args<float>(st, stage, gin, gout, want_aux);
         if (st->nf == 2) launch_stage_shape<2, float>(st, a);
         else launch_stage_shape<3, float>(st, a);
@@ -767,8 +755,6 @@ extern "C" int emu_state_create(emu_ctx* ctx, int64_t np, int precision, emu_sta
               hipMalloc(&st->grid[0], (size_t)st->ncell * st->QG * sizeof(double)) == hipSuccess &&
               hipMalloc(&st->grid[1], (size_t)st->ncell * st->QG * sizeof(double)) == hipSuccess &&
               hipMalloc(&st->aux, (size_t)st->ncell * st->naux * sizeof(double)) == hipSuccess &&
-              (precision != EMU_FP32_BULK ||
-               hipMalloc(&st->grid32, (size_t)st->ncell * st->QG * sizeof(float)) == hipSuccess) &&
               hipMalloc(&st->scal, 4 * sizeof(double)) == hipSuccess &&
               hipMalloc(&st->steps, sizeof(int64_t)) == hipSuccess &&
               hipMalloc(&st->red, 8 * sizeof(unsigned long long)) == hipSuccess;
@@ -785,7 +771,7 @@ extern "C" int emu_state_destroy(emu_state* st)
     if (!st) return EMU_OK;
     void* ptrs[] = {st->block[0], st->block[1], st->Y,        st->acc,     st->newcell, st->rank,
                     st->counts,   st->cell_start, st->new_start, st->cell_cnt, st->new_cnt, st->mcount,
-                    st->mbase,    st->mv,         st->grid[0], st->grid[1], st->grid32,  st->aux,
+                    st->mbase,    st->mv,         st->grid[0], st->grid[1], st->aux,
                     st->scal,     st->steps,      st->red,     st->cub_tmp};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
