@@ -59,7 +59,6 @@ template <int NF>
 __global__ void k_init(InitDev D, PView P, int64_t* ids)
 {
     using I = PIdx<NF>;
-    constexpr int nh = NH<NF>;
     const int64_t np = P.np;
     for (int64_t pid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; pid < np; pid += (int64_t)gridDim.x * blockDim.x) {
         const int64_t per_cell = (int64_t)D.nlocs * D.ndirs;

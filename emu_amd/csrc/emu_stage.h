@@ -155,12 +155,6 @@ __device__ __forceinline__ int renorm_fp64(double* f, double L, double maxError)
     return EMU_OK;
 }
 
-// w[i] for a wave-uniform i in 0..2, without indexing a register array
-__device__ __forceinline__ double sel3(const double w[3], int i)
-{
-    return i == 0 ? w[0] : (i == 1 ? w[1] : w[2]);
-}
-
 // End-of-step position: the four stage saxpys, then the periodic shift.
 __device__ __forceinline__ void end_position(const double pos[3], const double v[3], double dt, const Geom& g,
                                              double npos[3], int& newcell)

@@ -31,8 +31,6 @@
 namespace emu {
 namespace {
 
-constexpr int kWave = 64;
-
 // Base value of the deposit grid: zero except the matter term rho Ye/Mp dV
 // on M_00 (inside_parentheses of generate_code.py:335-342, moved into the
 // field because it enters the gather with the same weights).
