@@ -11,7 +11,6 @@ every computation happens in libemu_amd.so.
 from __future__ import annotations
 
 import ctypes
-from dataclasses import dataclass
 
 import numpy as np
 import torch

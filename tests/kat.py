@@ -7,7 +7,6 @@ snapshots directly (time and the particle attributes it inspects), so the same
 checks run on the oracle and on the HIP paths.
 """
 import json
-import math
 import os
 
 import numpy as np

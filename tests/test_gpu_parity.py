@@ -5,7 +5,6 @@ FP64 fields within 1e-12 relative to each component's max-abs (per-element
 relative error is ill-defined for cancelled fluxes); FP32-bulk within the
 tolerance stated in each test.
 """
-import math
 
 import numpy as np
 import pytest

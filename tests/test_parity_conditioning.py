@@ -5,7 +5,6 @@ initial flavour state by one ulp and measure how far the result moves after
 the same steps, for the configurations of those GPU tests.  The adaptive
 step at a speedup of 1e9 is far past the conservative flavour bound by
 design; its measured amplification is what its own, wider bar is built on."""
-import numpy as np
 import pytest
 
 import cases
