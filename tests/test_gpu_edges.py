@@ -187,3 +187,18 @@ def test_fp32_bulk_domain_shapes(core, oracle, nf, ncell):
     deck, prm, emu = make(core, oracle, d)
     parts = oracle.init_particles(deck)
     run_fused_against_oracle(core, oracle, nf, prm, emu, parts, "fp32", 6, 0.3 * 0.5 / oracle.clight, "aligned")
+
+
+@pytest.mark.parametrize("order", [0, 1])
+@pytest.mark.parametrize("nf", [2, 3])
+def test_fp32_bulk_lower_shape_orders(core, oracle, nf, order):
+    """FP32-bulk at SHAPE_FACTOR_ORDER 0 (1-point stencils: one cell per wave)
+    and 1 (3-point stencils with a zero weight: two cells per wave), 6 steps
+    with a dt that moves particles across cells, against the oracle at the same
+    order.  A tenth of the fast-flavor densities, where this dt is within the
+    flavour bound and the step is well conditioned at these orders."""
+    d = cases.fast_flavor_dict(nf=nf, ncell=(4, 3, 5), L=(2.0, 1.5, 2.5), nphi=8, shape_order=order,
+                               st4_amplitude=1e-3, st4_ndens=4.89e31, st4_ndensbar=4.89e31)
+    deck, prm, emu = make(core, oracle, d)
+    parts = oracle.init_particles(deck)
+    run_fused_against_oracle(core, oracle, nf, prm, emu, parts, "fp32", 6, 0.3 * 0.5 / oracle.clight, "aligned")
