@@ -87,13 +87,18 @@ __global__ void k_move_ranks(const int* __restrict__ newcell, const int* __restr
             if (bad) atomicMin(err, (unsigned long long)EMU_ERR_MOVE | (~0ull << 3));
             e = (ez + 1) * 9 + (ey + 1) * 3 + (ex + 1);
         }
+        // one pass per move present in the chunk (most particles stay in their
+        // cell, so there are few): the lanes of move q get ranks after the
+        // earlier chunks' particles of that move, in lane order
         int myrank = 0;
-        for (int q = 0; q < 27; q++) {
-            const unsigned long long msk = __ballot(e == q);
-            if (msk == 0) continue;
+        unsigned long long left = __ballot(act);
+        while (left) {
+            const int q = __shfl(e, __ffsll((long long)left) - 1);
+            const unsigned long long msk = __ballot(act && e == q);
             const int before = __shfl(running, q);
-            if (e == q) myrank = before + __popcll(msk & ((1ull << lane) - 1ull));
+            if (act && e == q) myrank = before + __popcll(msk & ((1ull << lane) - 1ull));
             if (lane == q) running += __popcll(msk);
+            left &= ~msk;
         }
         if (act) {
             rank[ip] = myrank;
