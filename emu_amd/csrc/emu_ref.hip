@@ -74,51 +74,93 @@ __global__ void k_zero_comps(Fab fab, int c0, int c1)
 }
 
 // Evolve.cpp:106-125 with the fill of generate_code.py:153-169.
+// The sum of v over the wave's lanes, in every lane.
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// deposit_to_mesh's particle loop (Evolve.cpp:106-124 with the fill of
+// generate_code.py:150-169): each particle adds w f N and w f N p/pupt to its
+// stencil.  The lanes of a wave whose stencils start at the same cell (most of
+// them: particles keep the cell order they are created in) first sum their
+// contributions across the wave, and one lane adds each sum to the fab, so a
+// grid value takes one atomic per wave instead of one per particle.  Every
+// lane's own terms are the reference's expressions; only the order of the
+// sum differs.
 template <int NF>
-__global__ void k_deposit(PView P, Fab fab, Geom g, unsigned long long* oob)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_deposit(PView P, Fab fab, Geom g,
+                                                                                          unsigned long long* oob)
 {
 #pragma clang fp contract(off)
     using I = PIdx<NF>;
     using G = GIdx<NF>;
+    constexpr int nh = NH<NF>;
     const int64_t ip = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (ip >= P.np) return;
+    const int lane = threadIdx.x & 63;
+    bool valid = ip < P.np;
+    const int64_t ipl = valid ? ip : 0; // (the launch has at least one particle)
     int first[3], width[3];
     double w[3][3];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        const double delta = (P.pos[d][ip] - g.plo[d]) * g.dxi[d];
+        const double delta = (P.pos[d][ipl] - g.plo[d]) * g.dxi[d];
         stencil1(delta, g.order[d], first[d], width[d], w[d]);
     }
-    if (!stencil_in_fab(fab, first, width)) {
+    if (valid && !stencil_in_fab(fab, first, width)) {
         atomicAdd(oob, 1ull);
-        return;
+        valid = false;
     }
-    const double pupx = P.r[I::pupx][ip], pupy = P.r[I::pupy][ip], pupz = P.r[I::pupz][ip],
-                 pupt = P.r[I::pupt][ip];
-    double f[2][NH<NF>], Nt[2];
-    Nt[0] = P.r[I::N][ip];
-    Nt[1] = P.r[I::Nbar][ip];
+    const double pupx = P.r[I::pupx][ipl], pupy = P.r[I::pupy][ipl], pupz = P.r[I::pupz][ipl],
+                 pupt = P.r[I::pupt][ipl];
+    double f[2][nh], Nt[2];
+    Nt[0] = P.r[I::N][ipl];
+    Nt[1] = P.r[I::Nbar][ipl];
 #pragma unroll
-    for (int c = 0; c < NH<NF>; c++) {
-        f[0][c] = P.r[I::f + c][ip];
-        f[1][c] = P.r[I::fbar + c][ip];
+    for (int c = 0; c < nh; c++) {
+        f[0][c] = P.r[I::f + c][ipl];
+        f[1][c] = P.r[I::fbar + c][ipl];
     }
-    for (int kk = 0; kk < width[2]; kk++)
-        for (int jj = 0; jj < width[1]; jj++)
-            for (int ii = 0; ii < width[0]; ii++) {
-                const int i = first[0] + ii, j = first[1] + jj, k = first[2] + kk;
-                const double sxyz = w[0][ii] * w[1][jj] * w[2][kk];
+    // the width of a stencil depends only on the shape order: the same in every lane
+    unsigned long long left = __ballot(valid);
+    while (left) {
+        const int leader = __ffsll((long long)left) - 1;
+        const int f0 = __shfl(first[0], leader), f1 = __shfl(first[1], leader), f2 = __shfl(first[2], leader);
+        const bool mine = valid && first[0] == f0 && first[1] == f1 && first[2] == f2;
+        left &= ~__ballot(mine);
+        // (the stencil loops stay rolled; the weights are picked by value, so
+        // that no register array is indexed at run time)
+        auto pick = [](double v0, double v1, double v2, int x) { return x == 0 ? v0 : (x == 1 ? v1 : v2); };
+#pragma unroll 1
+        for (int kk = 0; kk < width[2]; kk++)
+#pragma unroll 1
+            for (int jj = 0; jj < width[1]; jj++)
+#pragma unroll 1
+                for (int ii = 0; ii < width[0]; ii++) {
+                    const int i = f0 + ii, j = f1 + jj, k = f2 + kk;
+                    const double sxyz = pick(w[0][0], w[0][1], w[0][2], ii) * pick(w[1][0], w[1][1], w[1][2], jj) *
+                                        pick(w[2][0], w[2][1], w[2][2], kk);
 #pragma unroll
-                for (int bar = 0; bar < 2; bar++)
+                    for (int bar = 0; bar < 2; bar++)
 #pragma unroll
-                    for (int c = 0; c < NH<NF>; c++) {
-                        const double base = sxyz * f[bar][c] * Nt[bar];
-                        unsafeAtomicAdd(&fab.d[fab.at(i, j, k, G::var(0, bar) + c)], base);
-                        unsafeAtomicAdd(&fab.d[fab.at(i, j, k, G::var(1, bar) + c)], base * pupx / pupt);
-                        unsafeAtomicAdd(&fab.d[fab.at(i, j, k, G::var(2, bar) + c)], base * pupy / pupt);
-                        unsafeAtomicAdd(&fab.d[fab.at(i, j, k, G::var(3, bar) + c)], base * pupz / pupt);
-                    }
-            }
+                        for (int c = 0; c < nh; c++) {
+                            // other lanes' terms are selected out, not multiplied by
+                            // zero: a non-finite particle stays in its own stencil
+                            const double base = sxyz * f[bar][c] * Nt[bar];
+                            const double s0 = wave_sum(mine ? base : 0.0);
+                            const double s1 = wave_sum(mine ? base * pupx / pupt : 0.0);
+                            const double s2 = wave_sum(mine ? base * pupy / pupt : 0.0);
+                            const double s3 = wave_sum(mine ? base * pupz / pupt : 0.0);
+                            if (lane == leader) {
+                                unsafeAtomicAdd(&fab.d[fab.at(i, j, k, G::var(0, bar) + c)], s0);
+                                unsafeAtomicAdd(&fab.d[fab.at(i, j, k, G::var(1, bar) + c)], s1);
+                                unsafeAtomicAdd(&fab.d[fab.at(i, j, k, G::var(2, bar) + c)], s2);
+                                unsafeAtomicAdd(&fab.d[fab.at(i, j, k, G::var(3, bar) + c)], s3);
+                            }
+                        }
+                }
+    }
 }
 
 // SumBoundary: every valid cell adds its periodic ghost images.
