@@ -74,21 +74,15 @@ __global__ void k_zero_comps(Fab fab, int c0, int c1)
 }
 
 // Evolve.cpp:106-125 with the fill of generate_code.py:153-169.
-// The sum of v over the wave's lanes, in every lane.
-__device__ __forceinline__ double wave_sum(double v)
-{
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    return v;
-}
-
 // deposit_to_mesh's particle loop (Evolve.cpp:106-124 with the fill of
 // generate_code.py:150-169): each particle adds w f N and w f N p/pupt to its
 // stencil.  The lanes of a wave whose stencils start at the same cell (most of
 // them: particles keep the cell order they are created in) first sum their
-// contributions across the wave, and one lane adds each sum to the fab, so a
-// grid value takes one atomic per wave instead of one per particle.  Every
-// lane's own terms are the reference's expressions; only the order of the
-// sum differs.
+// contributions, and one atomic per grid value adds the sum to the fab.  The
+// sum goes through LDS: per stencil cell each lane writes its terms as a row,
+// then lane v adds up column v (value v of the 8 NH deposited ones) over the
+// wave.  Every lane's own terms are the reference's expressions; only the
+// order of the sum differs.
 template <int NF>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_deposit(PView P, Fab fab, Geom g,
                                                                                           unsigned long long* oob)
@@ -97,8 +91,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     using I = PIdx<NF>;
     using G = GIdx<NF>;
     constexpr int nh = NH<NF>;
+    constexpr int NV = 8 * nh;            // values per stencil cell: (N, Fx, Fy, Fz) x (nu, nubar) x nh
+    constexpr int HV = NV > 64 ? NV / 2 : NV; // values per pass (at most one per lane)
+    constexpr int RS = HV + 1;            // LDS row stride, odd: a column's reads hit distinct banks
+    static_assert(NV % HV == 0 && HV % 4 == 0 && HV <= 64, "passes");
+    __shared__ double rows[4][64 * RS];   // one block of rows per wave (kBlock = 256)
     const int64_t ip = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
+    double* R = rows[threadIdx.x >> 6];
     bool valid = ip < P.np;
     const int64_t ipl = valid ? ip : 0; // (the launch has at least one particle)
     int first[3], width[3];
@@ -122,6 +122,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         f[0][c] = P.r[I::f + c][ipl];
         f[1][c] = P.r[I::fbar + c][ipl];
     }
+    // value v = ((bar nh + c) 4 + gi): lane v's grid component, for the column sums
+    const int vb = lane % HV;
+    auto comp_of = [&](int v) { return G::var(v % 4, (v / 4) / nh) + (v / 4) % nh; };
     // the width of a stencil depends only on the shape order: the same in every lane
     unsigned long long left = __ballot(valid);
     while (left) {
@@ -142,23 +145,34 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                     const double sxyz = pick(w[0][0], w[0][1], w[0][2], ii) * pick(w[1][0], w[1][1], w[1][2], jj) *
                                         pick(w[2][0], w[2][1], w[2][2], kk);
 #pragma unroll
-                    for (int bar = 0; bar < 2; bar++)
+                    for (int h = 0; h < NV / HV; h++) {
 #pragma unroll
-                        for (int c = 0; c < nh; c++) {
-                            // other lanes' terms are selected out, not multiplied by
-                            // zero: a non-finite particle stays in its own stencil
-                            const double base = sxyz * f[bar][c] * Nt[bar];
-                            const double s0 = wave_sum(mine ? base : 0.0);
-                            const double s1 = wave_sum(mine ? base * pupx / pupt : 0.0);
-                            const double s2 = wave_sum(mine ? base * pupy / pupt : 0.0);
-                            const double s3 = wave_sum(mine ? base * pupz / pupt : 0.0);
-                            if (lane == leader) {
-                                unsafeAtomicAdd(&fab.d[fab.at(i, j, k, G::var(0, bar) + c)], s0);
-                                unsafeAtomicAdd(&fab.d[fab.at(i, j, k, G::var(1, bar) + c)], s1);
-                                unsafeAtomicAdd(&fab.d[fab.at(i, j, k, G::var(2, bar) + c)], s2);
-                                unsafeAtomicAdd(&fab.d[fab.at(i, j, k, G::var(3, bar) + c)], s3);
+                        for (int bar = 0; bar < 2; bar++)
+#pragma unroll
+                            for (int c = 0; c < nh; c++) {
+                                const int v = (bar * nh + c) * 4; // its 4 values sit in one pass (HV % 4 == 0)
+                                if (v / HV != h) continue;
+                                // other lanes' terms are selected out, not multiplied by
+                                // zero: a non-finite particle stays in its own stencil
+                                const double base = sxyz * f[bar][c] * Nt[bar];
+                                double* r = R + lane * RS + (v - h * HV);
+                                r[0] = mine ? base : 0.0;
+                                r[1] = mine ? base * pupx / pupt : 0.0;
+                                r[2] = mine ? base * pupy / pupt : 0.0;
+                                r[3] = mine ? base * pupz / pupt : 0.0;
                             }
+                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                        __builtin_amdgcn_wave_barrier();
+                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                        if (lane < HV) {
+                            double sum = 0.0;
+                            for (int l = 0; l < 64; l++) sum += R[l * RS + vb];
+                            unsafeAtomicAdd(&fab.d[fab.at(i, j, k, comp_of(h * HV + vb))], sum);
                         }
+                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                        __builtin_amdgcn_wave_barrier();
+                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    }
                 }
     }
 }
