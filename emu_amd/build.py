@@ -35,6 +35,7 @@ COMMON = ["-O3", f"--offload-arch={ARCH}", "-fPIC", "-std=c++17", "-munsafe-fp-a
           "-I", os.path.join(ROOT, "include"), "-Wno-unused-result"]
 SOURCES = {
     "emu_ref.hip": ["-ffp-contract=off"],
+    "emu_cells.hip": [],
     "emu_state.hip": [],
     "emu_init.hip": [],
     "emu_adjoint.hip": [],

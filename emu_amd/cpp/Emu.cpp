@@ -723,7 +723,12 @@ EvolveResult evolve_flavor(const TestParams* parms, const EvolveOptions& opt)
         for (int n = 0; n < parms->nsteps && time < parms->end_time; n++) {
             emu_check(emu_state_evolve(st, 1, 0.0), "emu_state_evolve", geom.ctx());
             const int code = emu_state_info(st, &time, &dt, &steps, &bad);
-            if (code != EMU_OK) emu_check(code, "Renormalize", geom.ctx(), bad);
+            if (code != EMU_OK)
+                emu_check(code,
+                          code == EMU_ERR_MOVE ? "fused step: a particle moved more than one cell"
+                          : code == EMU_ERR_DT ? "compute_dt"
+                                               : "Renormalize",
+                          geom.ctx(), bad);
             run_fom += (double)res.np;
             const int step = initial_step + (int)steps - 1;
             if (!opt.quiet)

@@ -447,7 +447,7 @@ __global__ void k_renorm_soa(int64_t np, double* Y, const double* Ll, double max
             for (int c = 0; c < nh; c++) f[c] = Y[(int64_t)(bar * nh + c) * np + p];
             const int e = renorm_fp64<NF>(f, Ll[(int64_t)bar * np + p], maxError);
             if (e != EMU_OK) {
-                atomicMin(err, ((unsigned long long)p << 3) | (unsigned long long)e);
+                atomicMin(err, status_word(p, e));
                 break;
             }
             for (int c = 0; c < nh; c++) Y[(int64_t)(bar * nh + c) * np + p] = f[c];
@@ -673,12 +673,7 @@ extern "C" int emu_adj_renormalize(emu_ctx* ctx, int64_t np, double* Y, const do
     unsigned long long h;
     EMU_HIP(ctx, hipMemcpyAsync(&h, ctx->d_red + 6, 8, hipMemcpyDeviceToHost, ctx->stream));
     EMU_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    if (h == ~0ull) {
-        if (bad_index) *bad_index = -1;
-        return EMU_OK;
-    }
-    if (bad_index) *bad_index = (int64_t)(h >> 3);
-    return (int)(h & 7);
+    return status_decode(h, bad_index);
 }
 
 extern "C" int emu_adj_axpy(emu_ctx* ctx, int64_t n, double a, const double* x, const double* y, double* out)

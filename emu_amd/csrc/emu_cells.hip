@@ -1,0 +1,477 @@
+// emu_cells.hip -- deposit_to_mesh and interpolate_rhs_from_mesh on the
+// reference's own data (SoA particles in PIdx order, in any order; the
+// `state` MultiFab as one ghost-cell fab), with the particles listed by the
+// box their stencil covers.
+//
+// Each particle's stencil lies in the 3 x 3 x 3 box (one cell wide in the
+// dimensions of shape order 0) around its nearest cell: shape3's window.
+// A counting sort lists the particles by that box (the key: the fab-box index
+// of the box's first cell), and one wave takes one box:
+//  - the deposit (Evolve.cpp:91-126, fill generate_code.py:150-169) sums
+//    D[t][q] = sum_p w_t(p) Q_p,q over the box's particles on the f64
+//    16x16x4 matrix cores (t: the box's cells, q: the 8 NF^2 deposited
+//    components N f, N f p/p_t for nu and nubar) and adds D to the fab with
+//    one atomic per value, 27 x 8 NF^2 per box instead of per particle;
+//  - the gather (Evolve.cpp:128-176, fills generate_code.py:239-389) reads
+//    the box's fields as wave-uniform (scalar) loads of a cell-major copy
+//    that combines each cell's nu and nubar fields as the gather uses them:
+//    per component c, (N + s Nbar + matter delta_c0) and (F_d + s Fbar_d),
+//    s = -1 for real and +1 for imaginary parts (generate_code.py:317-323).
+//
+// The per-particle formulas are the reference's; the order of the sums
+// differs (the deposit's also between runs, as with the reference's GPU
+// atomics), which moves results by rounding only.  A particle whose box
+// leaves the fab is skipped and counted, as emu_sync reports.
+#include "emu_cells.h"
+#include "emu_stage.h"
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+
+namespace emu {
+namespace {
+
+constexpr int kB = 256;
+
+unsigned nblk(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + kB - 1) / kB, 1 << 22)); }
+
+// Box of a particle in fab-box coordinates: its nearest cell minus one in
+// the 3-wide dimensions; false if the box leaves the fab.
+__device__ __forceinline__ bool particle_box(const PView& P, int64_t ip, const Fab& fab, const Geom& g, int o[3],
+                                             double w[3][3])
+{
+    bool in = true;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        int idx;
+        shape3((P.pos[d][ip] - g.plo[d]) * g.dxi[d], g.order[d], idx, w[d]);
+        const int wd = g.order[d] > 0 ? 3 : 1;
+        o[d] = idx - (wd == 3 ? 1 : 0) - fab.lo[d];
+        in = in && o[d] >= 0 && o[d] + wd <= fab.nf[d];
+    }
+    return in;
+}
+
+// Per-wave aggregation of per-lane keys: for each distinct key among the
+// active lanes, f(key, lanes holding it, leader lane).
+template <typename F> __device__ __forceinline__ void for_each_key(bool act, int key, F&& f)
+{
+    unsigned long long left = __ballot(act);
+    while (left) {
+        const int lead = __ffsll((long long)left) - 1;
+        const int k = __shfl(key, lead);
+        const unsigned long long m = __ballot(act && key == k);
+        f(k, m, lead);
+        left &= ~m;
+    }
+}
+
+__global__ void k_box_keys(PView P, Fab fab, Geom g, int nk, int* __restrict__ key, int* __restrict__ counts)
+{
+    const int64_t ip = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const bool act = ip < P.np;
+    const int lane = threadIdx.x & 63;
+    int k = nk;
+    if (act) {
+        int o[3];
+        double w[3][3];
+        if (particle_box(P, ip, fab, g, o, w)) k = (o[2] * fab.nf[1] + o[1]) * fab.nf[0] + o[0];
+        key[ip] = k;
+    }
+    for_each_key(act, k, [&](int kk, unsigned long long m, int lead) {
+        if (lane == lead) atomicAdd(&counts[kk], __popcll(m));
+    });
+}
+
+// perm lists the particles box by box; within a box, lane order within a
+// wave and waves in the order their atomics land.
+__global__ void k_box_perm(int64_t np, const int* __restrict__ key, const int* __restrict__ start, int* cursor,
+                           int* __restrict__ perm)
+{
+    const int64_t ip = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const bool act = ip < np;
+    const int lane = threadIdx.x & 63;
+    const int k = act ? key[ip] : 0;
+    for_each_key(act, k, [&](int kk, unsigned long long m, int lead) {
+        int base = 0;
+        if (lane == lead) base = atomicAdd(&cursor[kk], __popcll(m));
+        base = __shfl(base, lead);
+        if ((m >> lane) & 1ull) perm[start[kk] + base + __popcll(m & ((1ull << lane) - 1ull))] = (int)ip;
+    });
+}
+
+__global__ void k_fill_i32(int* p, int64_t n, int v)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+        p[t] = v;
+}
+
+__global__ void k_count_skipped(const int* start, int nk, unsigned long long* oob)
+{
+    const int n = start[nk + 1] - start[nk];
+    if (threadIdx.x == 0 && n > 0) atomicAdd(oob, (unsigned long long)n);
+}
+
+// The gather's combined fields of every fab cell, cell-major [cell][4][NH]:
+// v = 0: N + s Nbar (+ matter on component 0), v = 1..3: F_v + s Fbar_v.
+template <int NF> __global__ void k_combine(Fab fab, Geom g, double* __restrict__ comb)
+{
+    using G = GIdx<NF>;
+    constexpr int nh = NH<NF>;
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= fab.cstride) return;
+    const double* c0 = fab.d + t;
+    auto S = [&](int comp) { return c0[(int64_t)comp * fab.cstride]; };
+    const double matter = S(G::rho) * S(G::Ye) / phys::Mp / g.inv_cell_volume;
+    double* out = comb + t * 4 * nh;
+#pragma unroll
+    for (int v = 0; v < 4; v++)
+#pragma unroll
+        for (int c = 0; c < nh; c++) {
+            const double a = S(G::var(v, 0) + c), b = S(G::var(v, 1) + c);
+            double x = is_im_comp<NF>(c) ? a + b : a - b;
+            if (v == 0 && c == 0) x += matter;
+            out[v * nh + c] = x;
+        }
+}
+
+// XCD-aware block order: consecutive boxes share an XCD's L2.
+__device__ __forceinline__ int64_t box_block(int blocks_per_xcd)
+{
+    const int b = blockIdx.x;
+    return (int64_t)(b & 7) * blocks_per_xcd + (b >> 3);
+}
+
+constexpr int kWavesGather = 4;
+
+template <int NF, int WX, int WY, int WZ>
+__global__ __launch_bounds__(64 * kWavesGather) void k_gather_boxes(PView P, Fab fab, Geom g, const double* comb_,
+                                                                     const double* __restrict__ M2,
+                                                                     const int* __restrict__ start,
+                                                                     const int* __restrict__ perm, int nk,
+                                                                     int blocks_per_xcd)
+{
+    using I = PIdx<NF>;
+    constexpr int nh = NH<NF>, QG = 4 * nh;
+    // the box's fields are the same for every lane: read them as scalar loads
+    typedef __attribute__((address_space(4))) const double* cdp;
+    const cdp comb = (cdp)comb_;
+    const int lane = threadIdx.x & 63;
+    const int64_t box = box_block(blocks_per_xcd) * kWavesGather + (threadIdx.x >> 6);
+    if (box >= nk) return;
+    const int kb = __builtin_amdgcn_readfirstlane((int)box);
+    const int beg = __builtin_amdgcn_readfirstlane(start[kb]);
+    const int end = __builtin_amdgcn_readfirstlane(start[kb + 1]);
+    if (beg >= end) return;
+    const int64_t sy = fab.nf[0], sz = (int64_t)fab.nf[0] * fab.nf[1];
+    const double coef = M_SQRT2 * phys::GF * g.inv_cell_volume;
+    for (int base = beg; base < end; base += 64) {
+        const bool act = base + lane < end;
+        const int64_t ip = perm[act ? base + lane : beg];
+        const double pupx = P.r[I::pupx][ip], pupy = P.r[I::pupy][ip], pupz = P.r[I::pupz][ip],
+                     pupt = P.r[I::pupt][ip];
+        int o[3];
+        double w[3][3];
+        particle_box(P, ip, fab, g, o, w);
+        const double vx = pupx / pupt, vy = pupy / pupt, vz = pupz / pupt;
+        double in[nh];
+#pragma unroll
+        for (int c = 0; c < nh; c++) in[c] = 0.0;
+        // one box cell per iteration (rolled: the box's 27 x 4 NH fields would
+        // not fit the scalar registers at once): per field v the lane's
+        // coefficient w_t s_v, s = (1, -v_x, -v_y, -v_z), times the NH
+        // uniform values of that field
+        auto pick = [](const double* x, int i) { return i == 0 ? x[0] : (i == 1 ? x[1] : x[2]); };
+#pragma unroll 1
+        for (int t = 0; t < WX * WY * WZ; t++) {
+            const int ii = t % WX, jj = (t / WX) % WY, kk = t / (WX * WY);
+            const double wt = pick(w[0], WX == 3 ? ii : 1) * pick(w[1], WY == 3 ? jj : 1) * pick(w[2], WZ == 3 ? kk : 1);
+            const cdp G = comb + ((int64_t)kb + kk * sz + jj * sy + ii) * QG;
+            const double a[4] = {wt, -wt * vx, -wt * vy, -wt * vz};
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+#pragma unroll
+                for (int c = 0; c < nh; c++) in[c] += a[v] * G[v * nh + c];
+        }
+        if (!act) continue;
+        // vacuum (generate_code.py:239-255), then the gathered potential
+        double V[nh], Vb[nh];
+#pragma unroll
+        for (int i = 0; i < NF; i++)
+#pragma unroll
+            for (int j = i; j < NF; j++) {
+                const int cr = herm_re<NF>(i, j);
+                V[cr] = Vb[cr] = M2[cr] * phys::c4 / (2. * pupt);
+                if (j > i) {
+                    const int ci = herm_im<NF>(i, j);
+                    V[ci] = M2[ci] * phys::c4 / (2. * pupt);
+                    Vb[ci] = -V[ci];
+                }
+            }
+#pragma unroll
+        for (int c = 0; c < nh; c++) {
+            const double term = coef * in[c];
+            V[c] += term;
+            Vb[c] += is_im_comp<NF>(c) ? term : -term;
+        }
+        // Evolve.cpp:160-172
+        P.r[I::x][ip] = pupx / pupt * phys::c;
+        P.r[I::y][ip] = pupy / pupt * phys::c;
+        P.r[I::z][ip] = pupz / pupt * phys::c;
+        P.r[I::time][ip] = 1.0;
+        P.r[I::pupx][ip] = 0;
+        P.r[I::pupy][ip] = 0;
+        P.r[I::pupz][ip] = 0;
+        P.r[I::pupt][ip] = 0;
+        P.r[I::N][ip] = 0;
+        P.r[I::Nbar][ip] = 0;
+        P.r[I::L][ip] = 0;
+        P.r[I::Lbar][ip] = 0;
+        double f[nh], d[nh];
+#pragma unroll
+        for (int c = 0; c < nh; c++) f[c] = P.r[I::f + c][ip];
+        commutator_rhs<NF, double>(V, f, d, 1.0 / phys::hbar);
+#pragma unroll
+        for (int c = 0; c < nh; c++) P.r[I::f + c][ip] = d[c];
+#pragma unroll
+        for (int c = 0; c < nh; c++) f[c] = P.r[I::fbar + c][ip];
+        commutator_rhs<NF, double>(Vb, f, d, 1.0 / phys::hbar);
+#pragma unroll
+        for (int c = 0; c < nh; c++) P.r[I::fbar + c][ip] = d[c];
+    }
+}
+
+constexpr int kWavesDeposit = 2;
+
+// LDS row of one particle: 9 shape factors, then its sources: N f_c and
+// Nbar fbar_c (2 NH), the multipliers (1, p/p_t) of N, Fx, Fy, Fz, and a zero
+// for the padding columns; odd strides, so that four particle rows read in
+// one instruction fall on different banks.
+template <int NF> struct DepRow {
+    static constexpr int nh = NH<NF>;
+    static constexpr int W = 9;
+    static constexpr int S = 2 * nh + 5;
+    static constexpr int ZERO = 2 * nh + 4;
+    static constexpr int doubles_per_wave = 64 * (W + S);
+};
+
+template <int NF, int WX, int WY, int WZ>
+__global__ __launch_bounds__(64 * kWavesDeposit) void k_deposit_boxes(PView P, Fab fab, Geom g,
+                                                                       const int* __restrict__ start,
+                                                                       const int* __restrict__ perm, int nk,
+                                                                       int blocks_per_xcd)
+{
+    using I = PIdx<NF>;
+    using R = DepRow<NF>;
+    constexpr int nh = NH<NF>, NQ = 8 * nh;         // deposited components
+    constexpr int NT = WX * WY * WZ, MT = (NT + 15) / 16, NTL = (NQ + 15) / 16;
+    __shared__ double lds[kWavesDeposit * R::doubles_per_wave];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t box = box_block(blocks_per_xcd) * kWavesDeposit + wave;
+    if (box >= nk) return;
+    const int kb = __builtin_amdgcn_readfirstlane((int)box);
+    const int beg = __builtin_amdgcn_readfirstlane(start[kb]);
+    const int end = __builtin_amdgcn_readfirstlane(start[kb + 1]);
+    if (beg >= end) return;
+    double* Wf = lds + wave * R::doubles_per_wave;
+    double* Src = Wf + 64 * R::W;
+    // operand indices: A row t = 16m + l%16 is the product of the particle's
+    // weights at the target's (i, j, k); B column q = (2 v + bar) NH + c is
+    // source (bar, c) times multiplier v (fab component order, GIdx)
+    int tdx[MT][3];
+#pragma unroll
+    for (int m = 0; m < MT; m++) {
+        const int t = 16 * m + (lane & 15);
+        const int ii = t % WX, jj = (t / WX) % WY, kk = t / (WX * WY);
+        // (rows past the box's NT cells are padding: tdx[m][2] = -1 gives them A = 0)
+        tdx[m][0] = WX == 3 ? ii : 1;
+        tdx[m][1] = 3 + (WY == 3 ? jj : 1);
+        tdx[m][2] = t < NT ? 6 + (WZ == 3 ? kk : 1) : -1;
+    }
+    int bi1[NTL], bi2[NTL];
+#pragma unroll
+    for (int n = 0; n < NTL; n++) {
+        const int q = 16 * n + (lane & 15);
+        const int v = (q / nh) / 2, bar = (q / nh) % 2, c = q % nh;
+        bi1[n] = q < NQ ? bar * nh + c : R::ZERO;
+        bi2[n] = q < NQ ? 2 * nh + v : R::ZERO;
+    }
+    mf_d4 acc[MT][NTL];
+#pragma unroll
+    for (int m = 0; m < MT; m++)
+#pragma unroll
+        for (int n = 0; n < NTL; n++) acc[m][n] = mf_d4{0.0, 0.0, 0.0, 0.0};
+    for (int base = beg; base < end; base += 64) {
+        const bool act = base + lane < end;
+        const int64_t ip = perm[act ? base + lane : beg];
+        int o[3];
+        double w[3][3];
+        particle_box(P, ip, fab, g, o, w);
+        const double pupt = P.r[I::pupt][ip];
+        const double N = P.r[I::N][ip], Nb = P.r[I::Nbar][ip];
+        double* wr = Wf + lane * R::W;
+        double* sr = Src + lane * R::S;
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+#pragma unroll
+            for (int e = 0; e < 3; e++) wr[3 * d + e] = act ? w[d][e] : 0.0;
+#pragma unroll
+        for (int c = 0; c < nh; c++) {
+            sr[c] = act ? N * P.r[I::f + c][ip] : 0.0;
+            sr[nh + c] = act ? Nb * P.r[I::fbar + c][ip] : 0.0;
+        }
+        sr[2 * nh] = 1.0;
+        sr[2 * nh + 1] = P.r[I::pupx][ip] / pupt;
+        sr[2 * nh + 2] = P.r[I::pupy][ip] / pupt;
+        sr[2 * nh + 3] = P.r[I::pupz][ip] / pupt;
+        sr[R::ZERO] = 0.0;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int nks = (min(64, end - base) + 3) >> 2; // k-steps of 4 particles
+        for (int ks = 0; ks < nks; ks++) {
+            const int k = 4 * ks + (lane >> 4);
+            const double* wk = Wf + k * R::W;
+            const double* sk = Src + k * R::S;
+            double A[MT];
+#pragma unroll
+            for (int m = 0; m < MT; m++)
+                A[m] = tdx[m][2] < 0 ? 0.0 : wk[tdx[m][0]] * wk[tdx[m][1]] * wk[tdx[m][2]];
+#pragma unroll
+            for (int n = 0; n < NTL; n++) {
+                const double Bv = sk[bi1[n]] * sk[bi2[n]];
+#pragma unroll
+                for (int m = 0; m < MT; m++) acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[m], Bv, acc[m][n], 0, 0, 0);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    // element r of lane l holds D[16m + 4r + l/16][16n + l%16]
+    const int bx = kb % fab.nf[0], by = (kb / fab.nf[0]) % fab.nf[1], bz = kb / (fab.nf[0] * fab.nf[1]);
+#pragma unroll
+    for (int m = 0; m < MT; m++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int t = 16 * m + 4 * r + (lane >> 4);
+            if (t >= NT) continue;
+            const int ii = t % WX, jj = (t / WX) % WY, kk = t / (WX * WY);
+            const int64_t cell = ((int64_t)(bz + kk) * fab.nf[1] + (by + jj)) * fab.nf[0] + (bx + ii);
+#pragma unroll
+            for (int n = 0; n < NTL; n++) {
+                const int q = 16 * n + (lane & 15);
+                if (q < NQ) unsafeAtomicAdd(&fab.d[(int64_t)(3 + q) * fab.cstride + cell], acc[m][n][r]);
+            }
+        }
+}
+
+template <int NF, int WX, int WY, int WZ>
+void launch_boxes(bool deposit, hipStream_t s, const PView& P, const Fab& fab, const Geom& g, const double* comb,
+                  const double* M2, const int* start, const int* perm, int nk)
+{
+    const int waves = deposit ? kWavesDeposit : kWavesGather;
+    const int64_t nblock = ((int64_t)nk + waves - 1) / waves;
+    const int bpx = (int)((nblock + 7) / 8);
+    if (deposit)
+        hipLaunchKernelGGL((k_deposit_boxes<NF, WX, WY, WZ>), dim3(8 * bpx), dim3(64 * waves), 0, s, P, fab, g, start,
+                           perm, nk, bpx);
+    else
+        hipLaunchKernelGGL((k_gather_boxes<NF, WX, WY, WZ>), dim3(8 * bpx), dim3(64 * waves), 0, s, P, fab, g, comb,
+                           M2, start, perm, nk, bpx);
+}
+
+template <int NF>
+void launch_boxes_nf(bool deposit, const Geom& g, hipStream_t s, const PView& P, const Fab& fab, const double* comb,
+                     const double* M2, const int* start, const int* perm, int nk)
+{
+    const int w = (g.order[0] > 0 ? 1 : 0) | (g.order[1] > 0 ? 2 : 0) | (g.order[2] > 0 ? 4 : 0);
+    switch (w) {
+    case 0: launch_boxes<NF, 1, 1, 1>(deposit, s, P, fab, g, comb, M2, start, perm, nk); break;
+    case 1: launch_boxes<NF, 3, 1, 1>(deposit, s, P, fab, g, comb, M2, start, perm, nk); break;
+    case 2: launch_boxes<NF, 1, 3, 1>(deposit, s, P, fab, g, comb, M2, start, perm, nk); break;
+    case 3: launch_boxes<NF, 3, 3, 1>(deposit, s, P, fab, g, comb, M2, start, perm, nk); break;
+    case 4: launch_boxes<NF, 1, 1, 3>(deposit, s, P, fab, g, comb, M2, start, perm, nk); break;
+    case 5: launch_boxes<NF, 3, 1, 3>(deposit, s, P, fab, g, comb, M2, start, perm, nk); break;
+    case 6: launch_boxes<NF, 1, 3, 3>(deposit, s, P, fab, g, comb, M2, start, perm, nk); break;
+    default: launch_boxes<NF, 3, 3, 3>(deposit, s, P, fab, g, comb, M2, start, perm, nk); break;
+    }
+}
+
+size_t scan_bytes(int64_t n)
+{
+    size_t b = 0;
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, b, (const int*)nullptr, (int*)nullptr, (int)n);
+    return b;
+}
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+} // namespace
+
+BoxScratchSize box_scratch_size(int64_t np, int64_t nk, int nf, bool gather)
+{
+    BoxScratchSize s;
+    s.ints = align256(sizeof(int) * (size_t)(2 * np + 3 * (nk + 2)));
+    s.comb = gather ? align256(sizeof(double) * (size_t)nk * 4 * nf * nf) : 0;
+    s.scan = align256(scan_bytes(nk + 2));
+    s.total = s.ints + s.comb + s.scan;
+    return s;
+}
+
+int box_list_and_run(emu_ctx* ctx, const PView& P, const Fab& fab, bool deposit, void* scratch, size_t scratch_bytes)
+{
+    hipStream_t s = ctx->stream;
+    const int64_t nk64 = fab.cstride;
+    if (nk64 + 2 > (int64_t)INT32_MAX || P.np > (int64_t)INT32_MAX)
+        return set_error(ctx, EMU_ERR_UNSUPPORTED, "more than 2^31 particles or fab cells");
+    const int nk = (int)nk64;
+    const int nf = ctx->p.nf;
+    const BoxScratchSize sz = box_scratch_size(P.np, nk, nf, !deposit);
+    if (scratch_bytes < sz.total) return set_error(ctx, EMU_ERR_ARG, "box list scratch too small");
+    char* b = (char*)scratch;
+    int* key = (int*)b;
+    int* perm = key + P.np;
+    int* counts = perm + P.np;
+    int* start = counts + (nk + 2);
+    int* cursor = start + (nk + 2);
+    double* comb = (double*)(b + sz.ints);
+    void* tmp = b + sz.ints + sz.comb;
+    hipLaunchKernelGGL(k_fill_i32, dim3(nblk(2 * (nk + 2))), dim3(kB), 0, s, counts, 2 * (int64_t)(nk + 2), 0);
+    hipLaunchKernelGGL(k_fill_i32, dim3(nblk(nk + 2)), dim3(kB), 0, s, cursor, (int64_t)(nk + 2), 0);
+    if (P.np > 0)
+        hipLaunchKernelGGL(k_box_keys, dim3(nblk(P.np)), dim3(kB), 0, s, P, fab, ctx->g, nk, key, counts);
+    size_t tb = sz.scan;
+    EMU_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(tmp, tb, counts, start, nk + 2, s));
+    if (P.np > 0) hipLaunchKernelGGL(k_box_perm, dim3(nblk(P.np)), dim3(kB), 0, s, P.np, key, start, cursor, perm);
+    hipLaunchKernelGGL(k_count_skipped, dim3(1), dim3(64), 0, s, start, nk, ctx->d_red + 7);
+    if (!deposit) {
+        if (nf == 2) hipLaunchKernelGGL(k_combine<2>, dim3(nblk(nk)), dim3(kB), 0, s, fab, ctx->g, comb);
+        else hipLaunchKernelGGL(k_combine<3>, dim3(nblk(nk)), dim3(kB), 0, s, fab, ctx->g, comb);
+    }
+    if (P.np > 0) {
+        if (nf == 2) launch_boxes_nf<2>(deposit, ctx->g, s, P, fab, comb, ctx->d_M2, start, perm, nk);
+        else launch_boxes_nf<3>(deposit, ctx->g, s, P, fab, comb, ctx->d_M2, start, perm, nk);
+    }
+    EMU_LAUNCH_CHECK(ctx, deposit ? "deposit_to_mesh" : "interpolate_rhs_from_mesh");
+    return EMU_OK;
+}
+
+int box_scratch(emu_ctx* ctx, size_t need, void** out)
+{
+    if (ctx->box_scratch_bytes < need) {
+        if (ctx->box_scratch) (void)hipFree(ctx->box_scratch);
+        ctx->box_scratch = nullptr;
+        ctx->box_scratch_bytes = 0;
+        if (hipMalloc(&ctx->box_scratch, need) != hipSuccess) {
+            ctx->box_scratch = nullptr;
+            return set_error(ctx, EMU_ERR_NOMEM, "box list scratch");
+        }
+        ctx->box_scratch_bytes = need;
+    }
+    *out = ctx->box_scratch;
+    return EMU_OK;
+}
+
+} // namespace emu

@@ -37,6 +37,32 @@ struct PView {
     int64_t np;
 };
 
+// Deferred status words.  A kernel that finds an error folds
+// (index << kStatusBits) | code into one 64-bit word with atomicMin, so the
+// word ends holding the lowest offending index (its code is that particle's).
+// ~0 means no error; an all-ones index means an error that names no particle
+// (compute_dt), which any error with an index outranks.
+constexpr int kStatusBits = 4;
+static_assert(EMU_ERR_MOVE < (1 << kStatusBits), "status codes must fit the status field");
+EMU_HD unsigned long long status_word(int64_t index, int code)
+{
+    return ((unsigned long long)index << kStatusBits) | (unsigned long long)code;
+}
+EMU_HD unsigned long long status_word_noindex(int code)
+{
+    return (~0ull << kStatusBits) | (unsigned long long)code;
+}
+inline int status_decode(unsigned long long w, int64_t* bad_index)
+{
+    if (w == ~0ull) {
+        if (bad_index) *bad_index = -1;
+        return EMU_OK;
+    }
+    const unsigned long long ix = w >> kStatusBits;
+    if (bad_index) *bad_index = ix == (~0ull >> kStatusBits) ? -1 : (int64_t)ix;
+    return (int)(w & ((1ull << kStatusBits) - 1));
+}
+
 } // namespace emu
 
 struct emu_ctx {
@@ -47,6 +73,8 @@ struct emu_ctx {
     int ngrow[3];
     hipStream_t stream = nullptr;
     unsigned long long* d_red = nullptr; // reduction scratch (8 slots)
+    void* box_scratch = nullptr;         // per-function deposit/gather: box list and combined fields
+    size_t box_scratch_bytes = 0;
     std::string err;
 };
 

@@ -1,10 +1,13 @@
 // emu_ref.hip -- the per-function entry points: the reference's functions on
 // the reference's data (SoA particles in PIdx order, the MultiFab `state` as
 // one ghost-cell fab).  Built with -ffp-contract=off so that every product and
-// sum rounds as the reference writes it; deposits use hardware FP64 atomics,
-// as the reference's GPU build does (Gpu::Atomic::AddNoRet,
-// generate_code.py:154), so their summation order is not deterministic.
+// sum here rounds as the reference writes it.  The deposit and the gather
+// themselves run over particles listed by stencil box (emu_cells.hip); the
+// deposit adds to the fab with FP64 atomics, as the reference's GPU build does
+// (Gpu::Atomic::AddNoRet, generate_code.py:154), so its summation order is
+// not deterministic.
 #include "emu_internal.h"
+#include "emu_cells.h"
 
 #include <algorithm>
 #include <cmath>
@@ -31,150 +34,11 @@ constexpr int kBlock = 256;
 
 inline unsigned nblocks(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + kBlock - 1) / kBlock); }
 
-// Stencil of one dimension exactly as ParticleInterpolator (:67-107): first
-// cell, width and weights; order 1 touches two cells, order 0 one.
-__device__ __forceinline__ void stencil1(double delta, int order, int& first, int& width, double w[3])
-{
-#pragma clang fp contract(off)
-    if (order > 1) {
-        first = nearest_cell_center_index(delta) - 1;
-        const double o = nearest_cell_center_offset(delta);
-        w[0] = 0.5 * (0.5 - o) * (0.5 - o);
-        w[1] = 0.75 - o * o;
-        w[2] = 0.5 * (0.5 + o) * (0.5 + o);
-        width = 3;
-    } else if (order > 0) {
-        const int nearest = nearest_cell_center_index(delta);
-        const double o = nearest_cell_center_offset(delta);
-        if (o >= 0) { first = nearest; w[0] = 1.0 - o; w[1] = o; }
-        else { first = nearest - 1; w[0] = -o; w[1] = 1.0 + o; }
-        w[2] = 0.0;
-        width = 2;
-    } else {
-        first = nearest_cell_center_index(delta);
-        w[0] = 1.0; w[1] = w[2] = 0.0;
-        width = 1;
-    }
-}
-
-// A stencil outside the fab (a particle outside the domain) is skipped and
-// counted in *oob; the reference would write outside its fab.
-__device__ __forceinline__ bool stencil_in_fab(const Fab& fab, const int first[3], const int width[3])
-{
-    for (int d = 0; d < 3; d++)
-        if (first[d] < fab.lo[d] || first[d] + width[d] > fab.lo[d] + fab.nf[d]) return false;
-    return true;
-}
-
 __global__ void k_zero_comps(Fab fab, int c0, int c1)
 {
     const int64_t n = fab.cstride * (c1 - c0);
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
         fab.d[(int64_t)c0 * fab.cstride + t] = 0.0;
-}
-
-// Evolve.cpp:106-125 with the fill of generate_code.py:153-169.
-// deposit_to_mesh's particle loop (Evolve.cpp:106-124 with the fill of
-// generate_code.py:150-169): each particle adds w f N and w f N p/pupt to its
-// stencil.  The lanes of a wave whose stencils start at the same cell (most of
-// them: particles keep the cell order they are created in) first sum their
-// contributions, and one atomic per grid value adds the sum to the fab.  The
-// sum goes through LDS: per stencil cell each lane writes its terms as a row,
-// then lane v adds up column v (value v of the 8 NH deposited ones) over the
-// wave.  Every lane's own terms are the reference's expressions; only the
-// order of the sum differs.
-template <int NF>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_deposit(PView P, Fab fab, Geom g,
-                                                                                          unsigned long long* oob)
-{
-#pragma clang fp contract(off)
-    using I = PIdx<NF>;
-    using G = GIdx<NF>;
-    constexpr int nh = NH<NF>;
-    constexpr int NV = 8 * nh;            // values per stencil cell: (N, Fx, Fy, Fz) x (nu, nubar) x nh
-    constexpr int HV = NV > 64 ? NV / 2 : NV; // values per pass (at most one per lane)
-    constexpr int RS = HV + 1;            // LDS row stride, odd: a column's reads hit distinct banks
-    static_assert(NV % HV == 0 && HV % 4 == 0 && HV <= 64, "passes");
-    __shared__ double rows[4][64 * RS];   // one block of rows per wave (kBlock = 256)
-    const int64_t ip = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    const int lane = threadIdx.x & 63;
-    double* R = rows[threadIdx.x >> 6];
-    bool valid = ip < P.np;
-    const int64_t ipl = valid ? ip : 0; // (the launch has at least one particle)
-    int first[3], width[3];
-    double w[3][3];
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-        const double delta = (P.pos[d][ipl] - g.plo[d]) * g.dxi[d];
-        stencil1(delta, g.order[d], first[d], width[d], w[d]);
-    }
-    if (valid && !stencil_in_fab(fab, first, width)) {
-        atomicAdd(oob, 1ull);
-        valid = false;
-    }
-    const double pupx = P.r[I::pupx][ipl], pupy = P.r[I::pupy][ipl], pupz = P.r[I::pupz][ipl],
-                 pupt = P.r[I::pupt][ipl];
-    double f[2][nh], Nt[2];
-    Nt[0] = P.r[I::N][ipl];
-    Nt[1] = P.r[I::Nbar][ipl];
-#pragma unroll
-    for (int c = 0; c < nh; c++) {
-        f[0][c] = P.r[I::f + c][ipl];
-        f[1][c] = P.r[I::fbar + c][ipl];
-    }
-    // value v = ((bar nh + c) 4 + gi): lane v's grid component, for the column sums
-    const int vb = lane % HV;
-    auto comp_of = [&](int v) { return G::var(v % 4, (v / 4) / nh) + (v / 4) % nh; };
-    // the width of a stencil depends only on the shape order: the same in every lane
-    unsigned long long left = __ballot(valid);
-    while (left) {
-        const int leader = __ffsll((long long)left) - 1;
-        const int f0 = __shfl(first[0], leader), f1 = __shfl(first[1], leader), f2 = __shfl(first[2], leader);
-        const bool mine = valid && first[0] == f0 && first[1] == f1 && first[2] == f2;
-        left &= ~__ballot(mine);
-        // (the stencil loops stay rolled; the weights are picked by value, so
-        // that no register array is indexed at run time)
-        auto pick = [](double v0, double v1, double v2, int x) { return x == 0 ? v0 : (x == 1 ? v1 : v2); };
-#pragma unroll 1
-        for (int kk = 0; kk < width[2]; kk++)
-#pragma unroll 1
-            for (int jj = 0; jj < width[1]; jj++)
-#pragma unroll 1
-                for (int ii = 0; ii < width[0]; ii++) {
-                    const int i = f0 + ii, j = f1 + jj, k = f2 + kk;
-                    const double sxyz = pick(w[0][0], w[0][1], w[0][2], ii) * pick(w[1][0], w[1][1], w[1][2], jj) *
-                                        pick(w[2][0], w[2][1], w[2][2], kk);
-#pragma unroll
-                    for (int h = 0; h < NV / HV; h++) {
-#pragma unroll
-                        for (int bar = 0; bar < 2; bar++)
-#pragma unroll
-                            for (int c = 0; c < nh; c++) {
-                                const int v = (bar * nh + c) * 4; // its 4 values sit in one pass (HV % 4 == 0)
-                                if (v / HV != h) continue;
-                                // other lanes' terms are selected out, not multiplied by
-                                // zero: a non-finite particle stays in its own stencil
-                                const double base = sxyz * f[bar][c] * Nt[bar];
-                                double* r = R + lane * RS + (v - h * HV);
-                                r[0] = mine ? base : 0.0;
-                                r[1] = mine ? base * pupx / pupt : 0.0;
-                                r[2] = mine ? base * pupy / pupt : 0.0;
-                                r[3] = mine ? base * pupz / pupt : 0.0;
-                            }
-                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                        __builtin_amdgcn_wave_barrier();
-                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                        if (lane < HV) {
-                            double sum = 0.0;
-                            for (int l = 0; l < 64; l++) sum += R[l * RS + vb];
-                            unsafeAtomicAdd(&fab.d[fab.at(i, j, k, comp_of(h * HV + vb))], sum);
-                        }
-                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                        __builtin_amdgcn_wave_barrier();
-                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                    }
-                }
-    }
 }
 
 // SumBoundary: every valid cell adds its periodic ghost images.
@@ -213,99 +77,6 @@ __global__ void k_fill_boundary(Fab fab, Geom g, int ncomp)
     if (i >= 0 && i < g.n[0] && j >= 0 && j < g.n[1] && k >= 0 && k < g.n[2]) return;
     const int si = pmod(i, g.n[0]), sj = pmod(j, g.n[1]), sk = pmod(k, g.n[2]);
     for (int c = 0; c < ncomp; c++) fab.d[fab.at(i, j, k, c)] = fab.d[fab.at(si, sj, sk, c)];
-}
-
-// Evolve.cpp:138-175 with Vvac (generate_code.py:239-255), the gather
-// (:296-357) and dfdt (:359-389).
-template <int NF>
-__global__ void k_interpolate_rhs(PView P, Fab fab, Geom g, const double* __restrict__ M2, unsigned long long* oob)
-{
-#pragma clang fp contract(off)
-    using I = PIdx<NF>;
-    using G = GIdx<NF>;
-    constexpr int nh = NH<NF>;
-    const int64_t ip = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (ip >= P.np) return;
-    const double pupx = P.r[I::pupx][ip], pupy = P.r[I::pupy][ip], pupz = P.r[I::pupz][ip],
-                 pupt = P.r[I::pupt][ip];
-    double V[nh], Vb[nh];
-#pragma unroll
-    for (int i = 0; i < NF; i++)
-#pragma unroll
-        for (int j = i; j < NF; j++) {
-            const int cr = herm_re<NF>(i, j);
-            V[cr] = 1 * (M2[cr]) * phys::c4 / (2. * pupt);
-            Vb[cr] = 1 * (M2[cr]) * phys::c4 / (2. * pupt);
-            if (j > i) {
-                const int ci = herm_im<NF>(i, j);
-                V[ci] = 1 * (M2[ci]) * phys::c4 / (2. * pupt);
-                Vb[ci] = -1 * (M2[ci]) * phys::c4 / (2. * pupt);
-            }
-        }
-    int first[3], width[3];
-    double w[3][3];
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-        const double delta = (P.pos[d][ip] - g.plo[d]) * g.dxi[d];
-        stencil1(delta, g.order[d], first[d], width[d], w[d]);
-    }
-    if (!stencil_in_fab(fab, first, width)) {
-        atomicAdd(oob, 1ull);
-        return;
-    }
-    for (int kk = 0; kk < width[2]; kk++)
-        for (int jj = 0; jj < width[1]; jj++)
-            for (int ii = 0; ii < width[0]; ii++) {
-                const int i = first[0] + ii, j = first[1] + jj, k = first[2] + kk;
-#define S(comp) fab.d[fab.at(i, j, k, (comp))]
-#pragma unroll
-                for (int c = 0; c < nh; c++) {
-                    // Re components of nubar enter with -1, Im with +1 (generate_code.py:317-323)
-                    bool is_im = false;
-#pragma unroll
-                    for (int a = 0; a < NF; a++)
-#pragma unroll
-                        for (int b = a + 1; b < NF; b++)
-                            if (c == herm_im<NF>(a, b)) is_im = true;
-                    const double SI = 1 * (S(G::var(0, 0) + c) - S(G::var(1, 0) + c) * pupx / pupt -
-                                           S(G::var(2, 0) + c) * pupy / pupt - S(G::var(3, 0) + c) * pupz / pupt);
-                    const double SIb = (is_im ? 1 : -1) *
-                                       (S(G::var(0, 1) + c) - S(G::var(1, 1) + c) * pupx / pupt -
-                                        S(G::var(2, 1) + c) * pupy / pupt - S(G::var(3, 1) + c) * pupz / pupt);
-                    double inside = SI + SIb;
-                    if (c == 0) inside = SI + SIb + S(G::rho) * S(G::Ye) / phys::Mp / g.inv_cell_volume;
-                    const double term =
-                        sqrt(2.) * phys::GF * g.inv_cell_volume * w[0][ii] * w[1][jj] * w[2][kk] * (inside);
-                    V[c] += term;
-                    if (is_im) Vb[c] += term;
-                    else Vb[c] -= term;
-                }
-#undef S
-            }
-    // Evolve.cpp:160-172
-    P.r[I::x][ip] = pupx / pupt * phys::c;
-    P.r[I::y][ip] = pupy / pupt * phys::c;
-    P.r[I::z][ip] = pupz / pupt * phys::c;
-    P.r[I::time][ip] = 1.0;
-    P.r[I::pupx][ip] = 0;
-    P.r[I::pupy][ip] = 0;
-    P.r[I::pupz][ip] = 0;
-    P.r[I::pupt][ip] = 0;
-    P.r[I::N][ip] = 0;
-    P.r[I::Nbar][ip] = 0;
-    P.r[I::L][ip] = 0;
-    P.r[I::Lbar][ip] = 0;
-    double f[nh], d[nh];
-#pragma unroll
-    for (int c = 0; c < nh; c++) f[c] = P.r[I::f + c][ip];
-    commutator_rhs<NF, double>(V, f, d, 1.0 / phys::hbar);
-#pragma unroll
-    for (int c = 0; c < nh; c++) P.r[I::f + c][ip] = d[c];
-#pragma unroll
-    for (int c = 0; c < nh; c++) f[c] = P.r[I::fbar + c][ip];
-    commutator_rhs<NF, double>(Vb, f, d, 1.0 / phys::hbar);
-#pragma unroll
-    for (int c = 0; c < nh; c++) P.r[I::fbar + c][ip] = d[c];
 }
 
 __device__ __forceinline__ void atomic_max_nonneg(unsigned long long* a, double v)
@@ -430,7 +201,7 @@ __global__ void k_renormalize(PView P, double maxError, unsigned long long* red)
             for (int c = 0; c < nh; c++) P.r[f0 + c][ip] = f[c];
         }
     }
-    if (err != EMU_OK) atomicMin(red, ((unsigned long long)ip << 3) | (unsigned long long)err);
+    if (err != EMU_OK) atomicMin(red, status_word(ip, err));
 }
 
 __global__ void k_sync(PView P, int type, int x0)
@@ -623,6 +394,7 @@ extern "C" int emu_destroy(emu_ctx* ctx)
     if (!ctx) return EMU_OK;
     if (ctx->d_red) (void)hipFree(ctx->d_red);
     if (ctx->d_M2) (void)hipFree(ctx->d_M2);
+    if (ctx->box_scratch) (void)hipFree(ctx->box_scratch);
     delete ctx;
     return EMU_OK;
 }
@@ -667,10 +439,11 @@ extern "C" int emu_deposit_to_mesh(emu_ctx* ctx, const emu_particles* parts, emu
     if (!make_fab(ctx, state, fab) || !make_pview(ctx, parts, P)) return set_error(ctx, EMU_ERR_ARG, "bad grid or particles");
     const int nc = 3 + 8 * ctx->p.nf * ctx->p.nf;
     hipLaunchKernelGGL(k_zero_comps, dim3(1024), dim3(kBlock), 0, ctx->stream, fab, 3, nc);
-    if (P.np > 0) {
-        if (ctx->p.nf == 2) hipLaunchKernelGGL(k_deposit<2>, dim3(nblocks(P.np)), dim3(kBlock), 0, ctx->stream, P, fab, ctx->g, ctx->d_red + 7);
-        else hipLaunchKernelGGL(k_deposit<3>, dim3(nblocks(P.np)), dim3(kBlock), 0, ctx->stream, P, fab, ctx->g, ctx->d_red + 7);
-    }
+    const size_t need = box_scratch_size(P.np, fab.cstride, ctx->p.nf, false).total;
+    void* scratch = nullptr;
+    int e = box_scratch(ctx, need, &scratch);
+    if (e == EMU_OK) e = box_list_and_run(ctx, P, fab, true, scratch, need);
+    if (e != EMU_OK) return e;
     const int64_t ncell = (int64_t)ctx->g.n[0] * ctx->g.n[1] * ctx->g.n[2];
     hipLaunchKernelGGL(k_sum_boundary, dim3(nblocks(ncell)), dim3(kBlock), 0, ctx->stream, fab, ctx->g,
                        ctx->ngrow[0], ctx->ngrow[1], ctx->ngrow[2], 3, nc);
@@ -696,10 +469,11 @@ extern "C" int emu_interpolate_rhs_from_mesh(emu_ctx* ctx, emu_particles* rhs, c
     PView P;
     if (!make_fab(ctx, state, fab) || !make_pview(ctx, rhs, P)) return set_error(ctx, EMU_ERR_ARG, "bad grid or particles");
     if (P.np == 0) return EMU_OK;
-    if (ctx->p.nf == 2) hipLaunchKernelGGL(k_interpolate_rhs<2>, dim3(nblocks(P.np)), dim3(kBlock), 0, ctx->stream, P, fab, ctx->g, ctx->d_M2, ctx->d_red + 7);
-    else hipLaunchKernelGGL(k_interpolate_rhs<3>, dim3(nblocks(P.np)), dim3(kBlock), 0, ctx->stream, P, fab, ctx->g, ctx->d_M2, ctx->d_red + 7);
-    EMU_LAUNCH_CHECK(ctx, "interpolate_rhs_from_mesh");
-    return EMU_OK;
+    const size_t need = box_scratch_size(P.np, fab.cstride, ctx->p.nf, true).total;
+    void* scratch = nullptr;
+    const int e = box_scratch(ctx, need, &scratch);
+    if (e != EMU_OK) return e;
+    return box_list_and_run(ctx, P, fab, false, scratch, need);
 }
 
 extern "C" int emu_compute_dt(emu_ctx* ctx, const emu_grid* state, double Vvac_max, double* dt_out)
@@ -782,12 +556,7 @@ extern "C" int emu_renormalize(emu_ctx* ctx, emu_particles* parts, int64_t* bad_
     unsigned long long h;
     EMU_HIP(ctx, hipMemcpyAsync(&h, ctx->d_red, 8, hipMemcpyDeviceToHost, ctx->stream));
     EMU_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    if (h == ~0ull) {
-        if (bad_index) *bad_index = -1;
-        return EMU_OK;
-    }
-    if (bad_index) *bad_index = (int64_t)(h >> 3);
-    return (int)(h & 7);
+    return status_decode(h, bad_index);
 }
 
 extern "C" int emu_sync_location(emu_ctx* ctx, emu_particles* parts, int type)

@@ -733,7 +733,7 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
                 const int64_t id = a.cur.id[ip];
                 int err = renorm_fp64<NF>(y, double(L), a.maxError);
                 if (err == EMU_OK) err = renorm_fp64<NF>(y + nh, double(Lb), a.maxError);
-                if (err != EMU_OK) atomicMin(a.err, ((unsigned long long)id << 3) | (unsigned long long)err);
+                if (err != EMU_OK) atomicMin(a.err, status_word(id, err));
                 double v[3], npos[3];
 #pragma unroll
                 for (int d = 0; d < 3; d++) v[d] = double(pupv[d]) / double(pupt) * phys::c;
@@ -754,7 +754,7 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
 #pragma unroll
                     for (int c = 0; c < 2 * nh; c++) a.dst.f[(int64_t)c * a.np + j] = T(y[c]);
                 } else {
-                    atomicMin(a.err, (unsigned long long)EMU_ERR_MOVE | (~0ull << 3));
+                    atomicMin(a.err, status_word(id, EMU_ERR_MOVE));
                 }
             }
         }

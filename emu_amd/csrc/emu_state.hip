@@ -65,9 +65,12 @@ __device__ __forceinline__ int move1(int from, int to, int n, int* bad)
 }
 
 // One wave per old cell: move index and rank within (cell, move); m[c][e].
+// A move of more than one cell (only a particle with |p| > p_t can outrun
+// c dt < dx) is reported as EMU_ERR_MOVE with the particle's id.
 __global__ void k_move_ranks(const int* __restrict__ newcell, const int* __restrict__ cell_start,
-                             const int* __restrict__ cell_cnt, Geom g, int* __restrict__ rank,
-                             unsigned char* __restrict__ mv, int* __restrict__ m, unsigned long long* err)
+                             const int* __restrict__ cell_cnt, const int64_t* __restrict__ ids, Geom g,
+                             int* __restrict__ rank, unsigned char* __restrict__ mv, int* __restrict__ m,
+                             unsigned long long* err)
 {
     const int c = blockIdx.x;
     const int lane = threadIdx.x;
@@ -84,7 +87,7 @@ __global__ void k_move_ranks(const int* __restrict__ newcell, const int* __restr
             int bad = 0;
             const int ex = move1(cx, tx, g.n[0], &bad), ey = move1(cy, ty, g.n[1], &bad),
                       ez = move1(cz, tz, g.n[2], &bad);
-            if (bad) atomicMin(err, (unsigned long long)EMU_ERR_MOVE | (~0ull << 3));
+            if (bad) atomicMin(err, status_word(ids[ip], EMU_ERR_MOVE));
             e = (ez + 1) * 9 + (ey + 1) * 3 + (ex + 1);
         }
         // one pass per move present in the chunk (most particles stay in their
@@ -242,7 +245,7 @@ __global__ void k_dt_finalize(const unsigned long long* red, DtParams p, double*
     if (p.dt_translation != 0.0 && dt_flavor != 0.0) dt = fmin(p.dt_translation, dt_flavor);
     else if (p.dt_translation != 0.0) dt = p.dt_translation;
     else if (dt_flavor != 0.0) dt = dt_flavor;
-    else atomicMin(err, (unsigned long long)EMU_ERR_DT | (~0ull << 3));
+    else atomicMin(err, status_word_noindex(EMU_ERR_DT));
     *dt_next = dt;
 }
 
@@ -527,6 +530,9 @@ template <typename T> Arrays<T> arrays_of(emu_state* st, int copy)
     A.f = (T*)b;
     return A;
 }
+
+// The ids of the current copy (placed after the positions in either precision).
+const int64_t* current_ids(emu_state* st) { return arrays_of<double>(st, st->cur).id; }
 
 size_t persistent_bytes(int64_t np, int nh, size_t tsize)
 {
@@ -962,7 +968,7 @@ extern "C" int emu_state_evolve(emu_state* st, int nsteps, double fixed_dt)
         // stable counting-sort plan by the new cell
         ev_record(st, 2, true);
         hipLaunchKernelGGL(k_move_ranks, dim3((unsigned)st->ncell), dim3(64), 0, s, st->newcell, st->cell_start,
-                           st->cell_cnt, ctx->g, st->rank, st->mv, st->mcount, st->red + 2);
+                           st->cell_cnt, current_ids(st), ctx->g, st->rank, st->mv, st->mcount, st->red + 2);
         hipLaunchKernelGGL(k_new_counts, dim3(grid_blocks(st->ncell + 1)), dim3(256), 0, s, st->mcount, ctx->g,
                            st->ncell, st->align, st->counts, st->new_cnt);
         int e = scan_counts(st, st->counts, st->new_start);
@@ -1020,12 +1026,7 @@ extern "C" int emu_state_info(emu_state* st, double* time, double* dt, int64_t* 
     if (time) *time = sc[0];
     if (dt) *dt = sc[1];
     if (steps) *steps = n;
-    if (e == ~0ull) {
-        if (bad_index) *bad_index = -1;
-        return EMU_OK;
-    }
-    if (bad_index) *bad_index = (e >> 3) == (~0ull >> 3) ? -1 : (int64_t)(e >> 3);
-    return (int)(e & 7);
+    return status_decode(e, bad_index);
 }
 
 

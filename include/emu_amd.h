@@ -158,7 +158,10 @@ int emu_state_store(emu_state* st, emu_particles* parts, int64_t* ids);
 /* Advance nsteps RK4 steps (asynchronous; dt stays on the device between
  * steps).  A fixed_dt > 0 replaces compute_dt's value for every step. */
 int emu_state_evolve(emu_state* st, int nsteps, double fixed_dt);
-/* Time, last dt, steps taken and the deferred Renormalize status.  Synchronous. */
+/* Time, last dt, steps taken and the deferred status of the steps since load:
+ * EMU_ERR_TRACE/NEGDIAG/LENGTH (Renormalize) or EMU_ERR_MOVE (a particle that
+ * moved more than one cell, which only |p| > p_t allows), with *bad_index = the
+ * lowest offending particle id; EMU_ERR_DT with *bad_index = -1.  Synchronous. */
 int emu_state_info(emu_state* st, double* time, double* dt, int64_t* steps, int64_t* bad_index);
 /* Per-launch timing on the context stream: enable = 1 records a HIP event pair
  * around every launch of the following emu_state_evolve calls (and clears

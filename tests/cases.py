@@ -30,7 +30,23 @@ def deck_dict(name, nf=2):
 
 def oracle_deck(oracle, d):
     fields = oracle.Deck.__dataclass_fields__
-    return oracle.Deck(**{k: v for k, v in d.items() if k in fields})
+    kw = {k: v for k, v in d.items() if k in fields}
+    st5 = {k[4:]: v for k, v in d.items() if k.startswith("st5_")}
+    if st5:
+        kw["st5"] = st5
+    return oracle.Deck(**kw)
+
+
+def minerbo_dict(nf=3, ncell=(2, 2, 2), nphi=32, **kw):
+    """Simulation type 5 (the Minerbo closure, FlavoredNeutrinoContainerInit.cpp
+    :574-654): nu_e isotropic, nubar_e with flux factor 0.33 along +z, heavy
+    leptons with 0.1 along a tilted axis."""
+    d = fast_flavor_dict(nf=nf, ncell=ncell, nphi=nphi, simulation_type=5)
+    d.update(st5_nnue=4.89e32, st5_nnua=4.0e32, st5_nnux=2.0e32, st5_fxnue=0.0, st5_fynue=0.0, st5_fznue=0.0,
+             st5_fxnua=0.0, st5_fynua=0.0, st5_fznua=0.33, st5_fxnux=0.06, st5_fynux=-0.048, st5_fznux=0.064,
+             st5_amplitude=1e-6)
+    d.update(kw)
+    return d
 
 
 def product_params(d):

@@ -1,0 +1,261 @@
+"""The five named workloads at their full sizes on the GPU.
+
+- cpu16 (2 flavours, 16^3 cells x 26 cube directions, 106,496 particles, 10
+  RK4 steps at CFL 0.4): the plain per-function path, the fused FP64 step and
+  FP32-bulk against the oracle over the whole run.
+- fiducial1d (3 flavours, 1 x 1 x 1024 cells over 64 cm, 378 directions,
+  387,072 particles): the fused FP64 step against the oracle for 20 steps on
+  the full grid; then 2000 steps of FP64 and FP32-bulk through the
+  instability's saturation, comparing the growth rate of |N_emu|, the
+  saturation, the cell-averaged N_ab and the trace drift.
+- large128 (3 flavours, 128^3 cells x 92 directions, 192,937,984 particles):
+  the forward step in both precisions through properties that hold at any
+  size (trace, conserved number, the two precisions' growth rates).
+
+The oracle's runs here take seconds on the host's cores.  Its conditioning at
+these settings (a one-ulp change of the initial flavour state moves the
+result by 3e-15 after cpu16's 10 steps and 5e-14 after fiducial1d's 20) is
+what makes the 1e-12 bar meaningful.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import cases
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def mods():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    from emu_amd import configs, core
+    return core, configs
+
+
+def deck_of(oracle, parms):
+    d = {k: getattr(parms, k) for k in parms.__dataclass_fields__ if k != "extra"}
+    return cases.oracle_deck(oracle, d)
+
+
+def oracle_run(oracle, deck, parts, nsteps):
+    """nsteps oracle RK4 steps with compute_dt's dt; returns the dts used."""
+    nf = deck.nf
+    prm = deck.params(nthreads=min(16, len(os.sched_getaffinity(0))))
+    g = oracle.Grid(nf, prm)
+    vv = oracle.vvac_max(nf, prm, parts)
+    oracle.deposit_to_mesh(nf, prm, parts, g)
+    err, dt = oracle.compute_dt(nf, prm, g, vv)
+    assert err == 0
+    dts = []
+    for _ in range(nsteps):
+        dts.append(dt)
+        err, dt, bad = oracle.rk4_step(nf, prm, parts, g, dt, vv)
+        assert err == 0, (err, bad)
+    return dts
+
+
+def compare(got_c, want, prec):
+    got = cases.by_id(got_c.to_aos())
+    want = cases.by_id(want)
+    np.testing.assert_array_equal(got["id"], want["id"])
+    if prec == "fp64":
+        np.testing.assert_array_equal(got["pos"], want["pos"])
+        return cases.max_rel_err(got["rdata"], want["rdata"])
+    np.testing.assert_allclose(got["pos"], want["pos"], rtol=1e-7)
+    return cases.max_rel_err(got["rdata"], want["rdata"])
+
+
+@pytest.mark.parametrize("path,tol", [("plain", 1e-12), ("fp64", 1e-12), ("fp32", 2e-5)])
+def test_cpu16_full_run(mods, oracle, path, tol):
+    """All 106,496 particles, the deck's 10 steps with compute_dt's dt."""
+    core, configs = mods
+    parms = configs.cpu16()
+    deck = deck_of(oracle, parms)
+    emu = core.Emu(parms)
+    c = core.FlavoredNeutrinoContainer.init_particles(emu)
+    assert c.np == 106496
+    want = c.to_aos()
+    oracle_run(oracle, deck, want, 10)
+    if path == "plain":
+        integ = core.Integrator(emu, c)
+        integ.integrate(10)
+        out = integ.old
+        assert integ.step_number == 10
+    else:
+        st = core.FusedState(emu, c.np, path)
+        st.load(c)
+        st.evolve(10)
+        st.store(c)
+        assert st.info()["steps"] == 10
+        st.close()
+        out = c
+    err = compare(out, want, "fp64" if path != "fp32" else "fp32")
+    print(f"cpu16 {path}: max rel err vs oracle after 10 steps {err:.2e}")
+    assert err < tol
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 2e-5)])
+def test_fiducial1d_oracle_window(mods, oracle, prec, tol):
+    """The full 1 x 1 x 1024 x 378 grid for 20 steps with compute_dt's dt."""
+    core, configs = mods
+    parms = configs.fiducial1d()
+    deck = deck_of(oracle, parms)
+    emu = core.Emu(parms)
+    c = core.FlavoredNeutrinoContainer.init_particles(emu)
+    assert c.np == 387072
+    want = c.to_aos()
+    dts = oracle_run(oracle, deck, want, 20)
+    st = core.FusedState(emu, c.np, prec)
+    st.load(c)
+    assert st.info()["dt"] == pytest.approx(dts[0], rel=1e-14 if prec == "fp64" else 1e-6)
+    st.evolve(20)
+    st.store(c)
+    info = st.info()
+    st.close()
+    assert info["time"] == pytest.approx(sum(dts), rel=1e-13 if prec == "fp64" else 1e-5)
+    err = compare(c, want, prec)
+    print(f"fiducial1d {prec}: max rel err vs oracle after 20 steps {err:.2e}")
+    assert err < tol
+
+
+def growth_fit(t, a):
+    """Growth rate of |N_emu| over its exponential phase: the least-squares
+    slope of ln a(t) between e^3 above the starting amplitude (past the
+    transient in which the non-growing modes mix) and e^3 below the
+    saturation maximum."""
+    t, la = np.asarray(t), np.log(np.asarray(a))
+    lo, hi = la[0] + 3.0, la.max() - 3.0
+    isat = int(np.argmax(la))
+    sel = np.arange(len(la)) <= isat
+    sel &= (la >= lo) & (la <= hi)
+    assert sel.sum() >= 8, (lo, hi, sel.sum())
+    return np.polyfit(t[sel], la[sel], 1)[0], (float(t[sel][0]), float(t[sel][-1]))
+
+
+def grid_nab(core, emu, c):
+    """The deposited nu N_ab of every cell (reference deposit of the stored
+    particles), [NF^2 components][cells]."""
+    state = core.MultiFab(emu)
+    core.deposit_to_mesh(c, state)
+    v = state.valid()  # [comp][k][j][i]
+    nh = emu.nh
+    return v[3:3 + nh].reshape(nh, -1).double()
+
+
+def test_fiducial1d_through_saturation(mods):
+    """2000 steps of the fused FP64 step and of FP32-bulk from the same
+    particles, with compute_dt's dt.  The FP64 step is the reference here
+    (it equals the oracle to 1e-12 over the window above; 2000 oracle steps
+    would take minutes).  FP32-bulk must give the growth rate of |N_emu| in
+    the exponential phase within 1%, saturate at the same amplitude within
+    5% and at the same time within 2%, keep every particle's trace within
+    2 maxError of 1 (Renormalize fixes the trace beyond maxError; its length
+    rescaling can then move it by about maxError again), and give the
+    cell-averaged N_ab within 1% of the largest diagonal in the linear
+    phase."""
+    core, configs = mods
+    parms = configs.fiducial1d()
+    emu = core.Emu(parms)
+    c0 = core.FlavoredNeutrinoContainer.init_particles(emu)
+    K = emu.K
+    nh, nf = emu.nh, parms.nf
+    diag = [K[f"f{i}{i}_Re"] for i in range(nf)]
+    out = {}
+    for prec in ("fp64", "fp32"):
+        c = core.FlavoredNeutrinoContainer(emu, c0.np)
+        c.copyParticles(c0)
+        st = core.FusedState(emu, c.np, prec)
+        st.load(c)
+        ts, amps, nabs, drift = [], [], {}, 0.0
+        info = st.info()
+        for n in range(0, 2001, 10):
+            if n:
+                st.evolve(10)
+                info = st.info()
+            ts.append(info["time"])
+            amps.append(st.flavor_amplitude()[0])
+            if n % 200 == 0 and n:
+                st.store(c)
+                tr = c.rdata[diag].sum(dim=0)
+                drift = max(drift, float((tr - 1).abs().max().item()))
+                nabs[n] = grid_nab(core, emu, c)
+        assert info["steps"] == 2000
+        st.close()
+        out[prec] = (np.array(ts), np.array(amps), nabs, drift)
+    (t64, a64, n64, d64), (t32, a32, n32, d32) = out["fp64"], out["fp32"]
+    r64, w64 = growth_fit(t64, a64)
+    r32, w32 = growth_fit(t32, a32)
+    rel = abs(r32 - r64) / abs(r64)
+    i64, i32 = int(np.argmax(a64)), int(np.argmax(a32))
+    sat_amp = abs(a32[i32] - a64[i64]) / a64[i64]
+    sat_time = abs(t32[i32] - t64[i64]) / t64[i64]
+    # the cell-averaged N_ab at the last snapshot of the linear phase (|N_emu|
+    # below 1% of its maximum) and at the end, over the largest N_00
+    lin = max([n for n in n64 if a64[n // 10] < 1e-2 * a64.max()], default=None)
+    assert lin is not None
+    scale = float(n64[lin][0].abs().max().item())
+    nab_lin = float((n32[lin].mean(dim=1) - n64[lin].mean(dim=1)).abs().max().item()) / scale
+    nab_end = float((n32[2000].mean(dim=1) - n64[2000].mean(dim=1)).abs().max().item()) / scale
+    print(f"fiducial1d 2000 steps: growth fp64 {r64:.6e} /s over {w64}, fp32 {r32:.6e} /s over {w32}, "
+          f"rel err {rel:.2e}; |N_emu| x {a64.max() / a64[0]:.3e} to saturation at t = {t64[i64]:.4e} s; "
+          f"saturation amplitude rel diff {sat_amp:.2e}, time rel diff {sat_time:.2e}; "
+          f"cell-averaged N_ab diff / max N_00: step {lin} {nab_lin:.2e}, step 2000 {nab_end:.2e}; "
+          f"trace drift fp64 {d64:.2e}, fp32 {d32:.2e}")
+    assert a64.max() / a64[0] > 1e4, "the run must reach the nonlinear phase"
+    assert i64 < len(a64) - 5, "the amplitude must saturate within the run"
+    assert rel < 1e-2
+    assert sat_amp < 5e-2 and sat_time < 2e-2
+    assert nab_lin < 1e-2
+    assert d64 <= 2 * parms.maxError and d32 <= 2 * parms.maxError
+
+
+def conserved_sums(c, K, nf):
+    """(sum_p N_p tr f_p, sum_p Nbar_p tr fbar_p) in FP64 on the device."""
+    tr = c.rdata[[K[f"f{i}{i}_Re"] for i in range(nf)]].sum(dim=0)
+    trb = c.rdata[[K[f"f{i}{i}_Rebar"] for i in range(nf)]].sum(dim=0)
+    return (float((c.rdata[K["N"]] * tr).sum().item()), float((c.rdata[K["Nbar"]] * trb).sum().item()),
+            float((tr - 1).abs().max().item()), float((trb - 1).abs().max().item()))
+
+
+def test_large128_forward_properties(mods):
+    """All 192,937,984 particles, 12 steps in each precision, one state at a
+    time (FP64's is 156 GB, beside the 51 GB container).  Each particle's
+    trace stays within 2 maxError of 1 (Renormalize in FP64), so the total
+    number sum_p N_p tr f_p changes by less than that fraction; the growth
+    rate of |N_emu| agrees between the two precisions to 1e-3 in the linear
+    phase."""
+    import torch
+    core, configs = mods
+    parms = configs.large128()
+    emu = core.Emu(parms)
+    K, nf = emu.K, parms.nf
+    rates = {}
+    for prec in ("fp32", "fp64"):
+        # the initial conditions are generated again for each precision (the
+        # generator is deterministic): one container at a time
+        c = core.FlavoredNeutrinoContainer.init_particles(emu)
+        assert c.np == 192937984
+        s0 = conserved_sums(c, K, nf)
+        st = core.FusedState(emu, c.np, prec)
+        st.load(c)
+        i0, a0 = st.info(), st.flavor_amplitude()[0]
+        st.evolve(12)
+        i1, a1 = st.info(), st.flavor_amplitude()[0]
+        assert i1["steps"] == 12
+        rates[prec] = math.log(a1 / a0) / (i1["time"] - i0["time"])
+        st.store(c)
+        st.close()
+        del st
+        s1 = conserved_sums(c, K, nf)
+        del c
+        torch.cuda.empty_cache()
+        print(f"large128 {prec}: growth {rates[prec]:.6e} /s, number change nu {s1[0] / s0[0] - 1:.2e} "
+              f"nubar {s1[1] / s0[1] - 1:.2e}, max |tr f - 1| {s1[2]:.2e} / {s1[3]:.2e}")
+        assert abs(s1[0] / s0[0] - 1) < 2 * parms.maxError and abs(s1[1] / s0[1] - 1) < 2 * parms.maxError
+        assert max(s1[2], s1[3]) <= 2 * parms.maxError
+    assert rates["fp64"] > 0
+    assert abs(rates["fp32"] - rates["fp64"]) / rates["fp64"] < 1e-3

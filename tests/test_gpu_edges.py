@@ -202,3 +202,47 @@ def test_fp32_bulk_lower_shape_orders(core, oracle, nf, order):
     deck, prm, emu = make(core, oracle, d)
     parts = oracle.init_particles(deck)
     run_fused_against_oracle(core, oracle, nf, prm, emu, parts, "fp32", 6, 0.3 * 0.5 / oracle.clight, "aligned")
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("kind", ["trace", "negdiag", "length", "move"])
+def test_fused_deferred_errors(core, oracle, kind, prec):
+    """The fused step defers its errors to emu_state_info, which names the
+    lowest offending particle id: Renormalize's three aborts
+    (generate_code.py:404-409, 418-423, 433-438 inside
+    FlavoredNeutrinoContainer.cpp:89-113) at the end of the step, and a
+    particle that outruns the one-cell move the cell sort allows (only
+    |p| > p_t can: here p_x = 3 p_t with c dt = 0.9 dx, a move of 2 or 3
+    cells along x in a 6-cell dimension).  Two particles are made bad, the
+    higher id in the lower slot, so the lowest id has to win."""
+    nf, ncell, L = 2, (6, 4, 4), (3.0, 2.0, 2.0)
+    d = cases.fast_flavor_dict(nf=nf, ncell=ncell, L=L, nphi=4)
+    deck, prm, emu = make(core, oracle, d)
+    parts = placed(oracle, deck, ncell, L, {c: 6 for c in range(96)}, seed=3)
+    K = {nm: i for i, nm in enumerate(oracle.attribute_names(nf))}
+    bad_ids = (41, 17)  # slots 40 and 16
+    want_code = {"trace": 1, "negdiag": 2, "length": 3, "move": 14}[kind]
+    for bid in bad_ids:
+        r = parts["rdata"][bid - 1]
+        if kind == "trace":
+            r[K["f00_Re"]] += 1e-3
+        elif kind == "negdiag":
+            r[K["f00_Re"]], r[K["f11_Re"]] = -0.01, 1.01
+        elif kind == "length":
+            r[K["L"]] *= 2.0
+        else:
+            r[K["pupx"]], r[K["pupy"]], r[K["pupz"]] = 3.0 * r[K["pupt"]], 0.0, 0.0
+    c = core.FlavoredNeutrinoContainer.from_aos(emu, parts)
+    st = core.FusedState(emu, c.np, prec, "aligned")
+    st.load(c)
+    assert st.info()["code"] == 0
+    st.evolve(1, fixed_dt=0.9 * 0.5 / oracle.clight)
+    info = st.info(raise_on_error=False)
+    with pytest.raises(Exception) as ei:
+        st.info()
+    assert ei.value.code == want_code and ei.value.bad_index == min(bad_ids)
+    st.store(c)
+    st.close()
+    assert info["code"] == want_code, info
+    assert info["bad_index"] == min(bad_ids), info
+    assert info["steps"] == 1

@@ -68,6 +68,40 @@ def test_init_matches_oracle(core, oracle, nf):
     assert cases.max_rel_err(got["rdata"], want["rdata"]) < 1e-14
 
 
+def sim_type_dict(st, nf):
+    """A deck for each initial condition: the reference's own sample decks for
+    types 0-3 (MSW, bipolar, fast flavor, fast flavor with k != 0), a
+    fast-flavor box with a tilted flux and a single mode for 4, and the
+    Minerbo closure for 5."""
+    if st == 4:
+        return cases.fast_flavor_dict(nf=nf, ncell=(3, 2, 4), nphi=8, st4_fluxfac=0.2, st4_theta=0.7, st4_phi=1.9,
+                                      st3_amplitude=1e-5)
+    if st == 5:
+        return cases.minerbo_dict(nf=nf, ncell=(2, 3, 2), nphi=16)
+    name = {0: "inputs_msw_test", 1: "inputs_bipolar_test", 2: "inputs_fast_flavor",
+            3: "inputs_fast_flavor_nonzerok"}[st]
+    d = cases.deck_dict(name, nf)
+    assert d["simulation_type"] == st
+    return d
+
+
+@pytest.mark.parametrize("st", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("nf", [2, 3])
+def test_init_sim_types_match_oracle(core, oracle, nf, st):
+    """InitParticles (FlavoredNeutrinoContainerInit.cpp:136-661) for every
+    simulation type: the GPU generator equals the oracle's bit for bit in
+    ids and positions, and to 1e-14 in the attributes (the generator's
+    transcendental functions are the device's, not glibc's)."""
+    d = sim_type_dict(st, nf)
+    deck, prm, emu = make(core, oracle, d)
+    want = oracle.init_particles(deck)
+    got = core.FlavoredNeutrinoContainer.init_particles(emu).to_aos()
+    assert len(got) == len(want) > 0
+    np.testing.assert_array_equal(got["id"], want["id"])
+    np.testing.assert_array_equal(got["pos"], want["pos"])
+    assert cases.max_rel_err(got["rdata"], want["rdata"]) < 1e-14
+
+
 @pytest.mark.parametrize("nf", [2, 3])
 @pytest.mark.parametrize("ncell", [(4, 3, 5), (1, 1, 6), (2, 2, 2)])
 def test_deposit_fill_interpolate_parity(core, oracle, nf, ncell):
@@ -103,6 +137,55 @@ def test_deposit_fill_interpolate_parity(core, oracle, nf, ncell):
     err, dt_or = oracle.compute_dt(nf, prm, g_or, vv)
     assert err == 0
     assert core.compute_dt(state, vv) == pytest.approx(dt_or, rel=1e-14)
+
+
+@pytest.mark.parametrize("nf", [2, 3])
+def test_deposit_interpolate_ghost_shell_and_skipped(core, oracle, nf):
+    """Particles between SyncLocation and RedistributeLocal may sit outside the
+    domain.  Those within one cell of it (a third of them here, in random
+    order) deposit into the fab's ghost cells, which SumBoundary folds, and
+    gather from them, as in the oracle.  Two particles three cells out have
+    stencils that leave the fab: both calls skip them, and emu_sync reports
+    them (the reference would write outside its fab)."""
+    ncell, L = (4, 3, 5), (2.0, 1.5, 2.5)
+    d = cases.fast_flavor_dict(nf=nf, ncell=ncell, L=L, rho_g_ccm=1e10, Ye=0.45)
+    deck, prm, emu = make(core, oracle, d)
+    parts, K = random_case(oracle, nf, ncell, L, 400, seed=17 + nf)
+    rng = np.random.default_rng(5)
+    dx = np.array(L) / np.array(ncell)
+    shell = rng.choice(len(parts), size=len(parts) // 3, replace=False)
+    for i in shell:
+        dim = rng.integers(3)
+        side = rng.choice([-1.0, 1.0])
+        x = -rng.uniform(0, 1) * dx[dim] if side < 0 else L[dim] + rng.uniform(0, 0.999) * dx[dim]
+        parts["pos"][i, dim] = x
+        parts["rdata"][i, 1 + dim] = x
+    far = parts[:2].copy()
+    far["pos"][:, 0] = -3.2 * dx[0]
+    far["rdata"][:, 1] = -3.2 * dx[0]
+    g_or = oracle.Grid(nf, prm)
+    oracle.deposit_to_mesh(nf, prm, parts, g_or)
+    c = core.FlavoredNeutrinoContainer.from_aos(emu, np.concatenate([parts, far]))
+    state = core.MultiFab(emu)
+    core.deposit_to_mesh(c, state)
+    with pytest.raises(Exception) as ei:
+        emu.sync()
+    assert ei.value.code == 10 and "2 particle" in str(ei.value)
+    fab = state.data.cpu().numpy().reshape(emu.ncomp, -1).T
+    want = g_or.fab().reshape(emu.ncomp, -1).T
+    assert cases.max_rel_err(fab[:, 3:], want[:, 3:]) < 1e-12
+    oracle.fill_boundary(nf, prm, g_or)
+    state.data.copy_(__import__("torch").from_numpy(g_or.fab().copy()))
+    rhs_or = parts.copy()
+    oracle.interpolate_rhs_from_mesh(nf, prm, rhs_or, g_or)
+    rhs = core.FlavoredNeutrinoContainer.from_aos(emu, np.concatenate([parts, far]))
+    core.interpolate_rhs_from_mesh(rhs, state)
+    with pytest.raises(Exception):
+        emu.sync()
+    got = rhs.to_aos()
+    assert cases.max_rel_err(got["rdata"][:len(parts)], rhs_or["rdata"]) < 1e-12
+    # the skipped particles keep their attributes (the rhs container's copy)
+    np.testing.assert_array_equal(got["rdata"][len(parts):], far["rdata"])
 
 
 @pytest.mark.parametrize("nf", [2, 3])

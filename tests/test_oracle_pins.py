@@ -671,3 +671,69 @@ def test_sync_location_and_apply_rhs(oracle):
     dF["rdata"] = np.random.default_rng(1).standard_normal(dF["rdata"].shape)
     oracle.apply_rhs(2, r, 0.25, dF)
     np.testing.assert_array_equal(r["rdata"], 0.25 * dF["rdata"] + parts["rdata"])
+
+
+# --------------------------------------------------------------------------
+# initial conditions: angular moments of the generated distributions
+# --------------------------------------------------------------------------
+def deposited_moments(oracle, deck, parts, bar, comp):
+    """(total N_comp, total F_comp vector) over the domain's cells, per unit
+    volume, from the oracle's deposit."""
+    nf = deck.nf
+    p = deck.params()
+    g = oracle.Grid(nf, p)
+    oracle.deposit_to_mesh(nf, p, parts, g)
+    v = g.valid()
+    nh = nf * nf
+    vol = deck.Lx * deck.Ly * deck.Lz
+    tot = [v[3 + (2 * var + bar) * nh + comp].sum() / vol for var in range(4)]
+    return tot[0], np.array(tot[1:])
+
+
+@pytest.mark.parametrize("directions,tol", [("cube26", 1e-13), ("sphere", 2e-3)])
+def test_init_type4_moments(oracle, directions, tol):
+    """Sim type 4 weights each direction by n dV/ndirs (1 + 3 ff cos theta)
+    (FlavoredNeutrinoContainerInit.cpp:506-569).  For a direction set whose
+    second moment is isotropic, <u u> = 1/3, the deposit's number density is
+    n and its flux F = n ff along the flux axis: |F|/N is the flux factor.
+    The 26 cube directions satisfy <u u> = 1/3 exactly; the 378 of
+    uniform_sphere_xyz(32) to 1e-3."""
+    d = cases.fast_flavor_dict(nf=2, ncell=(2, 2, 2), nphi=32, directions=directions, st4_fluxfac=0.2,
+                               st4_theta=0.7, st4_phi=1.9, st4_fluxfacbar=0.33, st4_amplitude=0.0)
+    deck = cases.oracle_deck(oracle, d)
+    parts = oracle.init_particles(deck)
+    for bar, n, ff, th, ph in ((0, 4.89e32, 0.2, 0.7, 1.9), (1, 4.89e32, 0.33, 0.0, 0.0)):
+        N, F = deposited_moments(oracle, deck, parts, bar, 0)
+        axis = np.array([math.cos(ph) * math.sin(th), math.sin(ph) * math.sin(th), math.cos(th)])
+        assert abs(N / n - 1) < 1e-13
+        np.testing.assert_allclose(F / N, ff * axis, atol=tol * ff)
+
+
+@pytest.mark.parametrize("nf", [2, 3])
+def test_init_type5_minerbo_moments(oracle, nf):
+    """Sim type 5 weights each direction with the Minerbo closure
+    exp(Z mu) Z/sinh(Z) (FlavoredNeutrinoContainerInit.cpp:574-654), whose
+    integral over the sphere is 1 and whose mean mu is coth(Z) - 1/Z = ff.
+    So the deposit of nu_e N_ee is n_nue with flux n_nue ff_e, the same for
+    nubar_e, and each heavy-lepton diagonal holds n_nux/4 with its flux --
+    to the 378-direction quadrature's accuracy (1e-3)."""
+    d = cases.minerbo_dict(nf=nf, st5_amplitude=0.0)
+    deck = cases.oracle_deck(oracle, d)
+    parts = oracle.init_particles(deck)
+    herm = herm_index(nf)
+    flux = {"nue": np.array([d["st5_fxnue"], d["st5_fynue"], d["st5_fznue"]]),
+            "nua": np.array([d["st5_fxnua"], d["st5_fynua"], d["st5_fznua"]]),
+            "nux": np.array([d["st5_fxnux"], d["st5_fynux"], d["st5_fznux"]])}
+    checks = [(0, "00_Re", "nue", 1.0), (1, "00_Re", "nua", 1.0), (0, "11_Re", "nux", 0.25), (1, "11_Re", "nux", 0.25)]
+    if nf == 3:
+        checks += [(0, "22_Re", "nux", 0.25), (1, "22_Re", "nux", 0.25)]
+    for bar, comp, fl, frac in checks:
+        N, F = deposited_moments(oracle, deck, parts, bar, herm[comp])
+        n = d["st5_n" + fl] * frac
+        assert abs(N / n - 1) < 2e-3, (bar, comp, N / n)
+        np.testing.assert_allclose(F / N, flux[fl], atol=2e-3 * max(np.linalg.norm(flux[fl]), 1e-3))
+    # each particle's flavour state is a density matrix: trace 1
+    K = {nm: i for i, nm in enumerate(oracle.attribute_names(nf))}
+    for t in ("", "bar"):
+        tr = sum(parts["rdata"][:, K[f"f{i}{i}_Re{t}"]] for i in range(nf))
+        np.testing.assert_allclose(tr, 1.0, atol=1e-15)
