@@ -7,8 +7,18 @@ volume-integrated |N_emu|^2; its backward runs the discrete adjoint of those
 steps.  Every right-hand side and its vector-Jacobian product run in
 libemu_amd.so (include/emu_adjoint.h): the backward of the deposit is a
 gather, the backward of the gather is a scatter, and the commutator has its
-own adjoint.  Forward states are kept at checkpoints and recomputed segment by
-segment (every ``checkpoint_every`` steps; ``sqrt(nsteps)`` by default).
+own adjoint.  Forward states are kept at checkpoints (every
+``checkpoint_every`` steps; ``sqrt(nsteps)`` by default) and recomputed
+segment by segment, either keeping the segment's step states
+(``segment="store"``: (nsteps/cp + cp) states in memory) or recomputing each
+step from the segment's start (``segment="recompute"``: nsteps/cp + 1 states,
+about cp/2 more forward steps per step).  The footprint is checked against
+``memory_budget`` bytes before anything runs.
+
+Each forward step is one library call (emu_adj_step: one cell list, the four
+stages with their deposits, Renormalize with a deferred status, the position
+update) and each reverse step another (emu_adj_step_vjp: the stage states
+recomputed, then the four stage VJPs).
 
 Conventions: dt is held fixed over the run (compute_dt's maximum is not
 differentiated); Renormalize is applied in the forward and passed straight
@@ -33,13 +43,23 @@ def _p(t):
 class AdjointProblem:
     """Fixed-order particle data and scratch for forward/adjoint sweeps."""
 
-    def __init__(self, emu, parts, dt: float, nsteps: int, checkpoint_every: int | None = None):
+    def __init__(self, emu, parts, dt: float, nsteps: int, checkpoint_every: int | None = None,
+                 segment: str = "store", memory_budget: float | None = None):
         self.emu = emu
         self.np = parts.np
         self.nh = emu.nh
         self.dt = float(dt)
         self.nsteps = int(nsteps)
         self.cp = int(checkpoint_every or max(1, int(math.sqrt(max(nsteps, 1)))))
+        if segment not in ("store", "recompute"):
+            raise ValueError("segment is 'store' or 'recompute'")
+        self.segment = segment
+        need = self.footprint_bytes()
+        if memory_budget is not None and need > memory_budget:
+            from .capi import EmuError
+            raise EmuError(13, "AdjointProblem",
+                           f"the checkpoints and work arrays need {need / 2**30:.1f} GiB, above the budget of "
+                           f"{memory_budget / 2**30:.1f} GiB (fewer checkpoints or segment='recompute')")
         K = emu.K
         r = parts.rdata
         self.pos0 = parts.pos.clone()
@@ -51,7 +71,24 @@ class AdjointProblem:
         self.f0 = torch.cat([r[f:f + self.nh], r[fb:fb + self.nh]]).clone()
         nwork = lib().emu_adj_work_doubles(emu.ctx, self.np)
         self.work = torch.empty(nwork, dtype=torch.float64, device=emu.device)
+        self.step_work = torch.empty(lib().emu_adj_step_work_doubles(emu.ctx, self.np), dtype=torch.float64,
+                                     device=emu.device)
         self.lam_nbar = torch.zeros(self.np, dtype=torch.float64, device=emu.device)
+
+    def footprint_bytes(self) -> float:
+        """Device bytes of a gradient: the run's checkpoints (flavour state and
+        positions), the segment's step states (segment='store'), the adjoint
+        and its next value, the particle constants and the work arrays."""
+        nh, np_ = self.nh, self.np
+        state = 8.0 * np_ * (2 * nh + 3)
+        ncp = -(-self.nsteps // self.cp)
+        held = ncp + (self.cp if self.segment == "store" else 1)
+        ncell = 1
+        for n in self.emu.parms.ncell:
+            ncell *= n
+        work = 8.0 * (5 * ncell * 4 * nh + 11 * nh * np_ + 2 * np_) + 8.0 * (3 * ncell * 4 * nh + nh * np_ + np_)
+        consts = 8.0 * np_ * (3 + 4 + 2 + 2 + 1 + 2 * nh)
+        return held * state + 2 * 8.0 * 2 * nh * np_ + consts + work
 
     # ---- kernels ---------------------------------------------------------
     def rhs(self, pos, Nn, Y):
@@ -99,7 +136,8 @@ class AdjointProblem:
         Y4 = self.axpy(dt * 1.0, k3, y)
         return (y, Y2, Y3, Y4), (k1, k2, k3)
 
-    def step(self, pos, Nn, y):
+    def step_unfused(self, pos, Nn, y):
+        """The same step from the per-RHS calls (kept as the check of step)."""
         dt = self.dt
         (Y1, Y2, Y3, Y4), (k1, k2, k3) = self.stages(pos, Nn, y)
         k4 = self.rhs(pos, Nn, Y4)
@@ -111,20 +149,28 @@ class AdjointProblem:
         self.advance_positions(pos)
         return acc
 
+    def step(self, pos, Nn, y):
+        """y_{n+1} from y_n; pos advanced in place; Renormalize's status is
+        checked by check_status()."""
+        out = torch.empty_like(y)
+        check(lib().emu_adj_step(self.emu.ctx, self.np, _p(pos), _p(self.pup), _p(Nn), _p(self.Ll), _p(y), _p(out),
+                                 self.dt, _p(self.step_work)), "emu_adj_step", self.emu.ctx)
+        return out
+
+    def check_status(self):
+        bad = ctypes.c_int64(-1)
+        code = lib().emu_adj_status(self.emu.ctx, ctypes.byref(bad))
+        check(code, "emu_adj_step (deferred Renormalize)", self.emu.ctx, bad.value)
+
     def step_adjoint(self, pos, Nn, y, lam):
-        """lam_n from lam_{n+1} for the step that started at (pos, y)."""
-        dt = self.dt
-        (Y1, Y2, Y3, Y4), _ = self.stages(pos, Nn, y)
-        zero = torch.zeros_like(lam)
-        # lambda_k4 = dt/6 lam; lambda_k3 = dt/3 lam + dt lY4; lambda_k2 = dt/3 lam + dt/2 lY3;
-        # lambda_k1 = dt/6 lam + dt/2 lY2; lam_n = lam + sum_i lY_i
-        lY4 = self.vjp(pos, Nn, Y4, self.axpy(dt * (1. / 6.), lam, zero))
-        lY3 = self.vjp(pos, Nn, Y3, self.axpy(dt * 1.0, lY4, self.axpy(dt * (1. / 3.), lam, zero)))
-        lY2 = self.vjp(pos, Nn, Y2, self.axpy(dt * 0.5, lY3, self.axpy(dt * (1. / 3.), lam, zero)))
-        lY1 = self.vjp(pos, Nn, Y1, self.axpy(dt * 0.5, lY2, self.axpy(dt * (1. / 6.), lam, zero)))
-        out = self.axpy(1.0, lY1, lam)
-        for l in (lY2, lY3, lY4):
-            self.axpy(1.0, l, out, out)
+        """lam_n from lam_{n+1} for the step that started at (pos, y):
+        lambda_k4 = dt/6 lam, lambda_k3 = dt/3 lam + dt lY4, lambda_k2 =
+        dt/3 lam + dt/2 lY3, lambda_k1 = dt/6 lam + dt/2 lY2, lam_n = lam +
+        sum_i lY_i (in emu_adj_step_vjp)."""
+        out = torch.empty_like(lam)
+        check(lib().emu_adj_step_vjp(self.emu.ctx, self.np, _p(pos), _p(self.pup), _p(Nn), _p(y), _p(lam), _p(out),
+                                     _p(self.lam_nbar), self.dt, _p(self.step_work)), "emu_adj_step_vjp",
+              self.emu.ctx)
         return out
 
     # ---- whole runs ------------------------------------------------------
@@ -143,6 +189,7 @@ class AdjointProblem:
                 cps.append((n, pos.clone(), y.clone()))
             y = self.step(pos, Nn, y)
         J, dJ = self.objective(pos, Nn, y, grad=keep_checkpoints)
+        self.check_status()
         return J, dJ, cps, Nn
 
     def adjoint_sweep(self, lam, cps, Nn):
@@ -151,15 +198,24 @@ class AdjointProblem:
         self.lam_nbar.zero_()
         for (n0, pos0, y0) in reversed(cps):
             n1 = min(n0 + self.cp, self.nsteps)
-            states = []  # the segment's step-start states, recomputed
-            pos = pos0.clone()
-            y = y0.clone()
-            for n in range(n0, n1):
-                states.append((pos.clone(), y.clone()))
-                if n + 1 < n1:
-                    y = self.step(pos, Nn, y)
-            for (pos_n, y_n) in reversed(states):
-                lam = self.step_adjoint(pos_n, Nn, y_n, lam)
+            if self.segment == "store":
+                states = []  # the segment's step-start states, recomputed
+                pos = pos0.clone()
+                y = y0.clone()
+                for n in range(n0, n1):
+                    states.append((pos.clone(), y.clone()))
+                    if n + 1 < n1:
+                        y = self.step(pos, Nn, y)
+                for (pos_n, y_n) in reversed(states):
+                    lam = self.step_adjoint(pos_n, Nn, y_n, lam)
+            else:
+                for n in range(n1 - 1, n0 - 1, -1):  # each step's start, from the checkpoint
+                    pos = pos0.clone()
+                    y = y0
+                    for _ in range(n0, n):
+                        y = self.step(pos, Nn, y)
+                    lam = self.step_adjoint(pos, Nn, y, lam)
+        self.check_status()
         return lam
 
     def gradients(self, f0, ratio: float):

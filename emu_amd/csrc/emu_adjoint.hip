@@ -455,6 +455,163 @@ __global__ void k_renorm_soa(int64_t np, double* Y, const double* Ll, double max
     }
 }
 
+// ---------------------------------------------------------------------------
+// Step-level kernels: one launch per RK4 stage after that stage's deposit.
+// The stage algebra is step()'s (AMReX RK4 with the reference's saxpys,
+// FlavoredNeutrinoContainer.H:39-46), each a x + y rounded as written.
+// ---------------------------------------------------------------------------
+template <int NF>
+__device__ __forceinline__ void stage_rhs(const AdjIn& in, int64_t p, const Geom& g, const double* grid,
+                                          const HamParams& hp, double* k)
+{
+    constexpr int nh = NH<NF>;
+    const int64_t np = in.np;
+    double v[3], H[nh], Hb[nh], f[nh];
+    hamiltonians<NF>(in, p, g, grid, hp, v, H, Hb);
+#pragma unroll
+    for (int c = 0; c < nh; c++) f[c] = in.Y[(int64_t)c * np + p];
+    commutator_rhs<NF, double>(H, f, k, 1.0);
+#pragma unroll
+    for (int c = 0; c < nh; c++) f[c] = in.Y[(int64_t)(nh + c) * np + p];
+    commutator_rhs<NF, double>(Hb, f, k + nh, 1.0);
+}
+
+// Stage S of one step, in.Y = Y_S, grid = its deposit.
+//  STORE 0 (the forward step): S = 1..3 write Y_{S+1} into ynext (which may be
+//  in.Y: each particle reads its own Y_S first) and the accumulator into acc;
+//  S = 4 finishes acc = y_{n+1}, renormalizes it (status deferred into err)
+//  and advances the particle's position.
+//  STORE 1 (the VJP's recompute): S = 1..3 write Y_{S+1} into ynext only.
+template <int NF, int S, int STORE>
+__global__ __launch_bounds__(256) void k_fwd_stage(AdjIn in, Geom g, const int* perm, const double* grid,
+                                                   HamParams hp, const double* y0, double* ynext, double* acc,
+                                                   const double* Ll, double maxError, double dt, double* pos_out,
+                                                   unsigned long long* err)
+{
+#pragma clang fp contract(off)
+    constexpr int nh = NH<NF>, S2 = 2 * nh;
+    const int64_t np = in.np;
+    const double a_next = S == 3 ? dt * 1.0 : dt * 0.5;
+    const double a_acc = S == 1 || S == 4 ? dt * (1. / 6.) : dt * (1. / 3.);
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < np; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t p = perm[i];
+        double k[S2];
+        stage_rhs<NF>(in, p, g, grid, hp, k);
+        if constexpr (S < 4) {
+#pragma unroll
+            for (int c = 0; c < S2; c++) {
+                const double y = y0[(int64_t)c * np + p];
+                ynext[(int64_t)c * np + p] = a_next * k[c] + y;
+                if constexpr (STORE == 0)
+                    acc[(int64_t)c * np + p] = a_acc * k[c] + (S == 1 ? y : acc[(int64_t)c * np + p]);
+            }
+        } else {
+            double y[S2];
+#pragma unroll
+            for (int c = 0; c < S2; c++) y[c] = a_acc * k[c] + acc[(int64_t)c * np + p];
+            // Renormalize (FlavoredNeutrinoContainer.cpp:89-113): nu, then nubar
+            // unless nu failed; a failing half is left as it is
+            int e = renorm_fp64<NF>(y, Ll[p], maxError);
+            if (e == EMU_OK) e = renorm_fp64<NF>(y + nh, Ll[np + p], maxError);
+            if (e != EMU_OK) atomicMin(err, status_word(p, e));
+#pragma unroll
+            for (int c = 0; c < S2; c++) acc[(int64_t)c * np + p] = y[c];
+            double x[3], v[3], nx[3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                x[d] = in.pos[(int64_t)d * np + p];
+                v[d] = in.pup[(int64_t)d * np + p] / in.pup[3 * np + p] * phys::c;
+            }
+            int nc;
+            end_position(x, v, dt, g, nx, nc);
+#pragma unroll
+            for (int d = 0; d < 3; d++) pos_out[(int64_t)d * np + p] = nx[d];
+        }
+    }
+}
+
+// VJP of stage S, part 1: lambda_k = alpha lambda + beta lY_{S+1} (the
+// reverse of the stage algebra; beta = 0 at stage 4), then the commutator
+// adjoints: lY_S (commutator part) and mu, as k_vjp_comm.
+template <int NF>
+__global__ __launch_bounds__(256) void k_vjp_stage_comm(AdjIn in, Geom g, const int* perm, const double* grid,
+                                                        HamParams hp, const double* lam, const double* lY_prev,
+                                                        double alpha, double beta, double* lY, double* mu)
+{
+#pragma clang fp contract(off)
+    constexpr int nh = NH<NF>;
+    const int64_t np = in.np;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < np; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t p = perm[i];
+        double v[3], H[nh], Hb[nh];
+        hamiltonians<NF>(in, p, g, grid, hp, v, H, Hb);
+        double gH[2][nh];
+#pragma unroll
+        for (int bar = 0; bar < 2; bar++) {
+            double lk[nh], L[nh], f[nh], gy[nh];
+#pragma unroll
+            for (int c = 0; c < nh; c++) {
+                const int64_t o = (int64_t)(bar * nh + c) * np + p;
+                const double t = alpha * lam[o];
+                lk[c] = lY_prev ? beta * lY_prev[o] + t : t;
+                f[c] = in.Y[o];
+            }
+            lam_matrix<NF>(lk, L);
+            commutator_rhs<NF, double>(L, bar ? Hb : H, gy, 1.0); // -i[Lam, A]
+            double_offdiag<NF>(gy);
+#pragma unroll
+            for (int c = 0; c < nh; c++) lY[(int64_t)(bar * nh + c) * np + p] = gy[c];
+            commutator_rhs<NF, double>(f, L, gH[bar], 1.0);       // -i[B, Lam]
+            double_offdiag<NF>(gH[bar]);
+        }
+#pragma unroll
+        for (int c = 0; c < nh; c++)
+            mu[(int64_t)c * np + p] = hp.coef * (gH[0][c] + (is_im_comp<NF>(c) ? gH[1][c] : -gH[1][c]));
+    }
+}
+
+// VJP of stage S, part 3: the gather of the grid adjoint into lY_S (as
+// k_vjp_gather), then the running sum lam_out = lY_S + (lam, at the first
+// reversed stage, else lam_out).
+template <int NF>
+__global__ __launch_bounds__(256) void k_vjp_stage_gather(AdjIn in, Geom g, const int* perm, const double* gadj,
+                                                          double* lY, double* lamNbar, const double* lam,
+                                                          double* lam_out, int first)
+{
+#pragma clang fp contract(off)
+    constexpr int nh = NH<NF>, QG = 4 * nh;
+    const int64_t np = in.np;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < np; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t p = perm[i];
+        const double pos[3] = {in.pos[p], in.pos[np + p], in.pos[2 * np + p]};
+        const double pt = in.pup[3 * np + p];
+        const double v[3] = {in.pup[p] / pt, in.pup[np + p] / pt, in.pup[2 * np + p] / pt};
+        double lq[nh];
+#pragma unroll
+        for (int c = 0; c < nh; c++) lq[c] = 0.0;
+        for_stencil(g, pos, [&](int64_t cell, double w) {
+            const double* G = gadj + cell * QG;
+#pragma unroll
+            for (int c = 0; c < nh; c++)
+                lq[c] += w * (G[c] + v[0] * G[nh + c] + v[1] * G[2 * nh + c] + v[2] * G[3 * nh + c]);
+        });
+        const double N = in.Nn[p], Nb = in.Nn[np + p];
+        double ln = 0.0;
+#pragma unroll
+        for (int c = 0; c < nh; c++) {
+            const double sg = is_im_comp<NF>(c) ? 1.0 : -1.0;
+            const int64_t o0 = (int64_t)c * np + p, o1 = (int64_t)(nh + c) * np + p;
+            const double l0 = lY[o0] + N * lq[c], l1 = lY[o1] + sg * Nb * lq[c];
+            lY[o0] = l0;
+            lY[o1] = l1;
+            lam_out[o0] = 1.0 * l0 + (first ? lam[o0] : lam_out[o0]);
+            lam_out[o1] = 1.0 * l1 + (first ? lam[o1] : lam_out[o1]);
+            ln += sg * in.Y[o1] * lq[c];
+        }
+        if (lamNbar) lamNbar[p] += ln;
+    }
+}
+
 __global__ void k_axpy(int64_t n, double a, const double* x, const double* y, double* out)
 {
 #pragma clang fp contract(off)
@@ -510,6 +667,29 @@ struct CellSort {
     void* tmp;
     size_t tmp_bytes;
 };
+
+static CellSort cell_sort_at(const emu_ctx* ctx, int64_t np, double* region)
+{
+    const int64_t nc = ncell_of(ctx);
+    CellSort cs;
+    int* base = (int*)region;
+    cs.key = base;
+    cs.perm = cs.key + np;
+    cs.start = cs.perm + np;
+    cs.cursor = cs.start + (nc + 1);
+    const int64_t ints = 2 * np + 2 * nc + 1;
+    const uintptr_t t = (uintptr_t)(region + (ints + 1) / 2);
+    cs.tmp = (void*)((t + 255) & ~(uintptr_t)255);
+    cs.tmp_bytes = scan_bytes(ctx);
+    return cs;
+}
+
+// doubles of a cell sort region (ints, the scan's storage and its alignment)
+static int64_t cell_sort_doubles(const emu_ctx* ctx, int64_t np)
+{
+    const int64_t ints = 2 * np + 2 * ncell_of(ctx) + 1;
+    return (ints + 1) / 2 + 32 + (int64_t)(scan_bytes(ctx) + 7) / 8;
+}
 
 static CellSort cell_sort_space(const emu_ctx* ctx, int64_t np, double* work)
 {
@@ -683,4 +863,162 @@ extern "C" int emu_adj_axpy(emu_ctx* ctx, int64_t n, double a, const double* x, 
     hipLaunchKernelGGL(k_axpy, dim3(nblk(n)), dim3(kB), 0, ctx->stream, n, a, x, y, out);
     EMU_LAUNCH_CHECK(ctx, "adj_axpy");
     return EMU_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Step-level calls.  Work layout (doubles): the four stage grids
+// [4][ncell][4 NH], the grid adjoint [ncell][4 NH], the stage states
+// [3][2 NH][np] (Y2, Y3, Y4; the forward step uses the first as its stage
+// buffer), the stage adjoints [2][2 NH][np], mu [NH][np], then the cell sort.
+// ---------------------------------------------------------------------------
+namespace {
+struct StepWork {
+    double* G[4];
+    double* gadj;
+    double* Ys[3];
+    double* lY[2];
+    double* mu;
+    CellSort cs;
+};
+
+StepWork step_work(const emu_ctx* ctx, int64_t np, double* work)
+{
+    const int nh = ctx->p.nf * ctx->p.nf;
+    const int64_t ng = ncell_of(ctx) * 4 * nh, ns = 2 * nh * np;
+    StepWork w;
+    double* b = work;
+    for (int s = 0; s < 4; s++, b += ng) w.G[s] = b;
+    w.gadj = b;
+    b += ng;
+    for (int s = 0; s < 3; s++, b += ns) w.Ys[s] = b;
+    for (int s = 0; s < 2; s++, b += ns) w.lY[s] = b;
+    w.mu = b;
+    b += (int64_t)nh * np;
+    w.cs = cell_sort_at(ctx, np, b);
+    return w;
+}
+
+int arm_status(emu_ctx* ctx)
+{
+    if (!ctx->adj_status_armed) {
+        hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, ctx->stream, ctx->d_red + 5, ~0ull);
+        ctx->adj_status_armed = true;
+    }
+    return EMU_OK;
+}
+
+template <int NF, int S, int STORE>
+void fwd_stage(emu_ctx* ctx, const AdjIn& in, const StepWork& w, const double* grid, const double* y0, double* ynext,
+               double* acc, const double* Ll, double dt, double* pos)
+{
+    const int64_t np = in.np;
+    hipLaunchKernelGGL((k_fwd_stage<NF, S, STORE>), dim3(nblk(np)), dim3(kB), 0, ctx->stream, in, ctx->g, w.cs.perm,
+                       grid, ham_params(ctx), y0, ynext, acc, Ll, ctx->p.maxError, dt, pos, ctx->d_red + 5);
+}
+
+template <int NF>
+int adj_step_nf(emu_ctx* ctx, int64_t np, double* pos, const double* pup, const double* Nn, const double* Ll,
+                const double* y_in, double* y_out, double dt, double* work)
+{
+    const StepWork w = step_work(ctx, np, work);
+    AdjIn in{np, pos, pup, Nn, y_in};
+    if (int e = cell_sort(ctx, in, w.cs)) return e;
+    double* Ys = w.Ys[0];
+    deposit(ctx, in, w.cs, w.G[0]);
+    fwd_stage<NF, 1, 0>(ctx, in, w, w.G[0], y_in, Ys, y_out, Ll, dt, pos);
+    in.Y = Ys;
+    deposit(ctx, in, w.cs, w.G[0]);
+    fwd_stage<NF, 2, 0>(ctx, in, w, w.G[0], y_in, Ys, y_out, Ll, dt, pos);
+    deposit(ctx, in, w.cs, w.G[0]);
+    fwd_stage<NF, 3, 0>(ctx, in, w, w.G[0], y_in, Ys, y_out, Ll, dt, pos);
+    deposit(ctx, in, w.cs, w.G[0]);
+    fwd_stage<NF, 4, 0>(ctx, in, w, w.G[0], y_in, Ys, y_out, Ll, dt, pos);
+    return EMU_OK;
+}
+
+template <int NF>
+int adj_step_vjp_nf(emu_ctx* ctx, int64_t np, const double* pos, const double* pup, const double* Nn,
+                    const double* y_in, const double* lam_in, double* lam_out, double* lam_Nbar, double dt,
+                    double* work)
+{
+    const int nh = NF * NF;
+    const int64_t ng = ncell_of(ctx) * 4 * nh;
+    const StepWork w = step_work(ctx, np, work);
+    AdjIn in{np, pos, pup, Nn, y_in};
+    if (int e = cell_sort(ctx, in, w.cs)) return e;
+    // recompute the stage states and their deposits
+    const double* Yst[4] = {y_in, w.Ys[0], w.Ys[1], w.Ys[2]};
+    deposit(ctx, in, w.cs, w.G[0]);
+    fwd_stage<NF, 1, 1>(ctx, in, w, w.G[0], y_in, w.Ys[0], nullptr, nullptr, dt, nullptr);
+    in.Y = w.Ys[0];
+    deposit(ctx, in, w.cs, w.G[1]);
+    fwd_stage<NF, 2, 1>(ctx, in, w, w.G[1], y_in, w.Ys[1], nullptr, nullptr, dt, nullptr);
+    in.Y = w.Ys[1];
+    deposit(ctx, in, w.cs, w.G[2]);
+    fwd_stage<NF, 3, 1>(ctx, in, w, w.G[2], y_in, w.Ys[2], nullptr, nullptr, dt, nullptr);
+    in.Y = w.Ys[2];
+    deposit(ctx, in, w.cs, w.G[3]);
+    // reverse: lambda_k4 = dt/6 lam; lambda_k3 = dt/3 lam + dt lY4;
+    // lambda_k2 = dt/3 lam + dt/2 lY3; lambda_k1 = dt/6 lam + dt/2 lY2;
+    // lam_out = lam + lY4 + lY3 + lY2 + lY1
+    const double alpha[4] = {dt * (1. / 6.), dt * (1. / 3.), dt * (1. / 3.), dt * (1. / 6.)};
+    const double beta[4] = {dt * 0.5, dt * 0.5, dt * 1.0, 0.0};
+    for (int s = 3; s >= 0; s--) {
+        in.Y = Yst[s];
+        double* lY = w.lY[s & 1];
+        const double* lYp = s == 3 ? nullptr : w.lY[(s + 1) & 1];
+        hipLaunchKernelGGL(k_vjp_stage_comm<NF>, dim3(nblk(np)), dim3(kB), 0, ctx->stream, in, ctx->g, w.cs.perm,
+                           w.G[s], ham_params(ctx), lam_in, lYp, alpha[s], beta[s], lY, w.mu);
+        hipLaunchKernelGGL(k_zero, dim3(nblk(ng)), dim3(kB), 0, ctx->stream, w.gadj, ng);
+        seg_deposit<1>(ctx, in, w.cs.perm, w.mu, w.gadj);
+        hipLaunchKernelGGL(k_vjp_stage_gather<NF>, dim3(nblk(np)), dim3(kB), 0, ctx->stream, in, ctx->g, w.cs.perm,
+                           w.gadj, lY, lam_Nbar, lam_in, lam_out, s == 3 ? 1 : 0);
+    }
+    return EMU_OK;
+}
+} // namespace
+
+extern "C" int64_t emu_adj_step_work_doubles(const emu_ctx* ctx, int64_t np)
+{
+    if (!ctx || np < 0) return -1;
+    const int nh = ctx->p.nf * ctx->p.nf;
+    return 5 * ncell_of(ctx) * 4 * nh + 5 * 2 * nh * np + (int64_t)nh * np + cell_sort_doubles(ctx, np);
+}
+
+extern "C" int emu_adj_step(emu_ctx* ctx, int64_t np, double* pos, const double* pup, const double* Nn,
+                            const double* Ll, const double* y_in, double* y_out, double dt, double* work)
+{
+    if (!ctx || np < 0 || !work || y_in == y_out) return EMU_ERR_ARG;
+    arm_status(ctx);
+    if (np == 0) return EMU_OK;
+    const int e = ctx->p.nf == 2 ? adj_step_nf<2>(ctx, np, pos, pup, Nn, Ll, y_in, y_out, dt, work)
+                                 : adj_step_nf<3>(ctx, np, pos, pup, Nn, Ll, y_in, y_out, dt, work);
+    if (e) return e;
+    EMU_LAUNCH_CHECK(ctx, "adj_step");
+    return EMU_OK;
+}
+
+extern "C" int emu_adj_step_vjp(emu_ctx* ctx, int64_t np, const double* pos, const double* pup, const double* Nn,
+                                const double* y_in, const double* lam_in, double* lam_out, double* lam_Nbar, double dt,
+                                double* work)
+{
+    if (!ctx || np < 0 || !work || lam_in == lam_out) return EMU_ERR_ARG;
+    if (np == 0) return EMU_OK;
+    const int e = ctx->p.nf == 2
+                      ? adj_step_vjp_nf<2>(ctx, np, pos, pup, Nn, y_in, lam_in, lam_out, lam_Nbar, dt, work)
+                      : adj_step_vjp_nf<3>(ctx, np, pos, pup, Nn, y_in, lam_in, lam_out, lam_Nbar, dt, work);
+    if (e) return e;
+    EMU_LAUNCH_CHECK(ctx, "adj_step_vjp");
+    return EMU_OK;
+}
+
+extern "C" int emu_adj_status(emu_ctx* ctx, int64_t* bad_index)
+{
+    if (!ctx) return EMU_ERR_ARG;
+    arm_status(ctx);
+    unsigned long long h;
+    EMU_HIP(ctx, hipMemcpyAsync(&h, ctx->d_red + 5, 8, hipMemcpyDeviceToHost, ctx->stream));
+    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, ctx->stream, ctx->d_red + 5, ~0ull);
+    EMU_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return status_decode(h, bad_index);
 }

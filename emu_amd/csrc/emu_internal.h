@@ -75,6 +75,7 @@ struct emu_ctx {
     unsigned long long* d_red = nullptr; // reduction scratch (8 slots)
     void* box_scratch = nullptr;         // per-function deposit/gather: box list and combined fields
     size_t box_scratch_bytes = 0;
+    bool adj_status_armed = false;       // d_red[5] holds the adjoint steps' deferred status
     std::string err;
 };
 

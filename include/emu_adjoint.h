@@ -46,6 +46,31 @@ int emu_adj_renormalize(emu_ctx* ctx, int64_t np, double* Y, const double* Ll, i
 /* out = a x + y, elementwise over n doubles. */
 int emu_adj_axpy(emu_ctx* ctx, int64_t n, double a, const double* x, const double* y, double* out);
 
+/* Step-level calls: 'work' holds emu_adj_step_work_doubles(ctx, np) doubles. */
+int64_t emu_adj_step_work_doubles(const emu_ctx* ctx, int64_t np);
+
+/* One RK4 step of length dt at the step's positions (main.cpp:129-213 with
+ * the RK hook of FlavoredNeutrinoContainer.H:33-47; the stage states and
+ * accumulator rounded as the AMReX RK4 saxpys): y_out = y_{n+1} from
+ * y_in = y_n, Renormalize on y_out (FlavoredNeutrinoContainer.cpp:89-113)
+ * with its status deferred to emu_adj_status, then pos advanced in place.
+ * y_out must not alias y_in.  Asynchronous. */
+int emu_adj_step(emu_ctx* ctx, int64_t np, double* pos, const double* pup, const double* Nn, const double* Ll,
+                 const double* y_in, double* y_out, double dt, double* work);
+
+/* The vector-Jacobian product of that step's flavour map at its starting
+ * positions (Renormalize passed through): lam_out = (dy_out/dy_in)^T lam_in,
+ * lam_Nbar += (dy_out/dNbar)^T lam_in when lam_Nbar is not NULL.  lam_out must
+ * not alias lam_in.  Asynchronous. */
+int emu_adj_step_vjp(emu_ctx* ctx, int64_t np, const double* pos, const double* pup, const double* Nn,
+                     const double* y_in, const double* lam_in, double* lam_out, double* lam_Nbar, double dt,
+                     double* work);
+
+/* The first Renormalize error of the emu_adj_step calls since the previous
+ * emu_adj_status (then cleared), with the lowest particle index in
+ * *bad_index.  Synchronous. */
+int emu_adj_status(emu_ctx* ctx, int64_t* bad_index);
+
 #ifdef __cplusplus
 }
 #endif

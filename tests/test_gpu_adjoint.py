@@ -113,3 +113,79 @@ def test_adjoint_gradient_matches_finite_differences(mods, oracle, nf):
     Jm, _, _, _ = prob.run(prob.f0, 1.0 - h, keep_checkpoints=False)
     fd = (Jp - Jm) / (2 * h)
     assert float(ratio.grad) == pytest.approx(fd, rel=1e-5)
+
+
+@pytest.mark.parametrize("nf", [2, 3])
+def test_fused_step_matches_per_rhs_step(mods, oracle, nf):
+    """emu_adj_step (one call: the stages with their deposits, the
+    accumulator, Renormalize, the positions) against the same step composed
+    from the per-RHS calls and axpys, over 3 steps; and its VJP against the
+    central difference of <lam, step(y)> along a random direction (the step is
+    a polynomial in y, so the O(h^2) error is far below the bar)."""
+    torch, core, adjoint = mods
+    emu, c, deck, parts = setup(mods, oracle, nf, amp=1e-3)
+    dt = 0.4 * 0.5 / 2.99792458e10 * 0.5
+    prob = adjoint.AdjointProblem(emu, c, dt=dt, nsteps=3)
+    pa, pb = prob.pos0.clone(), prob.pos0.clone()
+    ya, yb = prob.f0.clone(), prob.f0.clone()
+    for _ in range(3):
+        ya = prob.step(pa, prob.Nn_base, ya)
+        yb = prob.step_unfused(pb, prob.Nn_base, yb)
+    prob.check_status()
+    assert torch.equal(pa, pb)
+    scale = yb.abs().amax(dim=1, keepdim=True).clamp_min(1e-300)
+    assert float(((ya - yb).abs() / scale).max()) < 1e-12
+    gen = torch.Generator(device="cpu").manual_seed(5)
+    y = prob.f0
+    lam = torch.randn(y.shape, generator=gen, dtype=torch.float64).to(y.device)
+    dy = torch.randn(y.shape, generator=gen, dtype=torch.float64).to(y.device) * 1e-4
+    prob.lam_nbar.zero_()
+    g = prob.step_adjoint(prob.pos0, prob.Nn_base, y, lam)
+    h = 1e-2
+
+    def J(yy):
+        p = prob.pos0.clone()
+        return prob.step(p, prob.Nn_base, yy)
+    fd = float((lam * (J(y + h * dy) - J(y - h * dy))).sum() / (2 * h))
+    assert float((g * dy).sum()) == pytest.approx(fd, rel=1e-8)
+
+
+def test_adjoint32_gradient_matches_finite_differences(mods):
+    """The two-flavour 32^3 x 92 workload (3,014,656 particles), 200 RK4
+    steps with checkpoints every 20: dJ/dratio and directional derivatives of
+    J along two random directions of the initial flavour state, each against
+    a central difference with h = 1e-4 (relative to the ratio, and to the
+    direction's own scale of the 1e-6 perturbations), to 1e-5."""
+    torch, core, adjoint = mods
+    from emu_amd import configs
+    parms = configs.adjoint32()
+    emu = core.Emu(parms)
+    c = core.FlavoredNeutrinoContainer.init_particles(emu)
+    assert c.np == 3014656
+    st = core.FusedState(emu, c.np, "fp64")
+    st.load(c)
+    dt = st.info()["dt"]
+    st.close()
+    prob = adjoint.AdjointProblem(emu, c, dt=dt, nsteps=200, checkpoint_every=20)
+    J0, gf, gr = prob.gradients(prob.f0.clone(), 1.0)
+    h = 1e-4
+    Jp, _, _, _ = prob.run(prob.f0, 1.0 + h, keep_checkpoints=False)
+    Jm, _, _, _ = prob.run(prob.f0, 1.0 - h, keep_checkpoints=False)
+    fd = (Jp - Jm) / (2 * h)
+    print(f"adjoint32: J {J0:.6e}, dJ/dratio adjoint {gr:.10e} fd {fd:.10e}")
+    assert gr == pytest.approx(fd, rel=1e-5)
+    gen = torch.Generator(device="cpu").manual_seed(11)
+    K = emu.K
+    nh = emu.nh
+    i01 = K["f01_Re"] - K["f00_Re"]
+    for trial in range(2):
+        v = torch.zeros_like(prob.f0)
+        # the off-diagonal components the perturbation lives in, nu and nubar
+        for comp in (i01, i01 + 1, nh + i01, nh + i01 + 1):
+            v[comp] = torch.randn(c.np, generator=gen, dtype=torch.float64).to(v.device) * 1e-6
+        Jp, _, _, _ = prob.run(prob.f0 + h * v, 1.0, keep_checkpoints=False)
+        Jm, _, _, _ = prob.run(prob.f0 - h * v, 1.0, keep_checkpoints=False)
+        fd = (Jp - Jm) / (2 * h)
+        ad = float((gf * v).sum())
+        print(f"adjoint32: direction {trial}: adjoint {ad:.10e} fd {fd:.10e}")
+        assert ad == pytest.approx(fd, rel=1e-5)

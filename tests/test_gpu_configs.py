@@ -58,14 +58,17 @@ def oracle_run(oracle, deck, parts, nsteps):
     return dts
 
 
-def compare(got_c, want, prec):
+def compare(got_c, want, prec, L):
+    """Positions: the dt of each step comes from compute_dt, whose maxima
+    come from FP64 atomic sums in a different order than the oracle's, so dt
+    and with it the positions may differ in the last bit (bit-exact positions
+    at a fixed dt are the fused-step tests' business); FP32-bulk's direction
+    is rounded to 2^-24.  Returns the fields' max relative error."""
     got = cases.by_id(got_c.to_aos())
     want = cases.by_id(want)
     np.testing.assert_array_equal(got["id"], want["id"])
-    if prec == "fp64":
-        np.testing.assert_array_equal(got["pos"], want["pos"])
-        return cases.max_rel_err(got["rdata"], want["rdata"])
-    np.testing.assert_allclose(got["pos"], want["pos"], rtol=1e-7)
+    tol = 1e-14 if prec == "fp64" else 1e-7
+    np.testing.assert_allclose(got["pos"], want["pos"], rtol=tol, atol=tol * max(L))
     return cases.max_rel_err(got["rdata"], want["rdata"])
 
 
@@ -93,7 +96,7 @@ def test_cpu16_full_run(mods, oracle, path, tol):
         assert st.info()["steps"] == 10
         st.close()
         out = c
-    err = compare(out, want, "fp64" if path != "fp32" else "fp32")
+    err = compare(out, want, "fp64" if path != "fp32" else "fp32", (parms.Lx, parms.Ly, parms.Lz))
     print(f"cpu16 {path}: max rel err vs oracle after 10 steps {err:.2e}")
     assert err < tol
 
@@ -117,7 +120,7 @@ def test_fiducial1d_oracle_window(mods, oracle, prec, tol):
     info = st.info()
     st.close()
     assert info["time"] == pytest.approx(sum(dts), rel=1e-13 if prec == "fp64" else 1e-5)
-    err = compare(c, want, prec)
+    err = compare(c, want, prec, (parms.Lx, parms.Ly, parms.Lz))
     print(f"fiducial1d {prec}: max rel err vs oracle after 20 steps {err:.2e}")
     assert err < tol
 
