@@ -60,6 +60,7 @@ _SIGS = [
     ("emu_last_error", ctypes.c_char_p, [_P]),
     ("emu_grid_ngrow", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int)]),
     ("emu_deposit_to_mesh", ctypes.c_int, [_P, ctypes.POINTER(EmuParticles), ctypes.POINTER(EmuGrid)]),
+    ("emu_deposit_to_mesh_static", ctypes.c_int, [_P, ctypes.POINTER(EmuParticles), ctypes.POINTER(EmuGrid)]),
     ("emu_fill_boundary", ctypes.c_int, [_P, ctypes.POINTER(EmuGrid)]),
     ("emu_interpolate_rhs_from_mesh", ctypes.c_int, [_P, ctypes.POINTER(EmuParticles), ctypes.POINTER(EmuGrid)]),
     ("emu_compute_dt", ctypes.c_int, [_P, ctypes.POINTER(EmuGrid), ctypes.c_double, ctypes.POINTER(ctypes.c_double)]),

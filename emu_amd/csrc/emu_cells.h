@@ -22,4 +22,10 @@ int box_list_and_run(emu_ctx* ctx, const PView& P, const Fab& fab, bool deposit,
 // The context's scratch, grown to `need` bytes (hipMalloc) on first use.
 int box_scratch(emu_ctx* ctx, size_t need, void** out);
 
+// The static scratch of the current device: static_scratch_init (at
+// emu_create, with the device current) looks up its address once per device;
+// static_scratch checks a call's sizes against its capacity and returns it.
+int static_scratch_init();
+int static_scratch(emu_ctx* ctx, int64_t np, int64_t ncells, void** out, size_t* bytes);
+
 } // namespace emu

@@ -28,6 +28,8 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <mutex>
+#include <string>
 
 namespace emu {
 namespace {
@@ -455,6 +457,55 @@ int box_list_and_run(emu_ctx* ctx, const PView& P, const Fab& fab, bool deposit,
         else launch_boxes_nf<3>(deposit, ctx->g, s, P, fab, comb, ctx->d_M2, start, perm, nk);
     }
     EMU_LAUNCH_CHECK(ctx, deposit ? "deposit_to_mesh" : "interpolate_rhs_from_mesh");
+    return EMU_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Scratch for the deposit that allocates nothing (emu_deposit_to_mesh_static):
+// one array per GPU, in the code object's device memory, for at most
+// kStaticMaxParticles particles and kStaticMaxCells fab cells (ghosts
+// included).  Host code uses the per-device address from hipGetSymbolAddress,
+// never the array's name.  Every call on one GPU shares the array.
+// ---------------------------------------------------------------------------
+constexpr int64_t kStaticMaxParticles = int64_t(1) << 22;
+constexpr int64_t kStaticMaxCells = int64_t(1) << 20;
+constexpr size_t kStaticScanBytes = size_t(1) << 20;
+constexpr size_t kStaticScratchBytes =
+    ((sizeof(int) * (size_t)(2 * kStaticMaxParticles + 3 * (kStaticMaxCells + 2)) + 255) & ~(size_t)255) +
+    kStaticScanBytes;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static void* g_static_scratch[kMaxDevices] = {};
+static std::mutex g_static_mu;
+
+int static_scratch_init()
+{
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return EMU_ERR_HIP;
+    std::lock_guard<std::mutex> lock(g_static_mu);
+    if (g_static_scratch[dev]) return EMU_OK;
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return EMU_ERR_HIP;
+    g_static_scratch[dev] = p;
+    return EMU_OK;
+}
+
+int static_scratch(emu_ctx* ctx, int64_t np, int64_t ncells, void** out, size_t* bytes)
+{
+    if (np > kStaticMaxParticles || ncells > kStaticMaxCells)
+        return set_error(ctx, EMU_ERR_UNSUPPORTED,
+                         "emu_deposit_to_mesh_static holds at most " + std::to_string(kStaticMaxParticles) +
+                             " particles and " + std::to_string(kStaticMaxCells) + " fab cells (got " +
+                             std::to_string(np) + " and " + std::to_string(ncells) + ")");
+    const BoxScratchSize sz = box_scratch_size(np, ncells, ctx->p.nf, false);
+    if (sz.total > kStaticScratchBytes || sz.scan > kStaticScanBytes)
+        return set_error(ctx, EMU_ERR_UNSUPPORTED, "static scratch too small for this grid");
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices || !g_static_scratch[dev])
+        return set_error(ctx, EMU_ERR_HIP, "static scratch not set up on this device (emu_create sets it up)");
+    *out = g_static_scratch[dev];
+    *bytes = sz.total;
     return EMU_OK;
 }
 

@@ -344,6 +344,26 @@ using namespace emu;
 
 extern "C" int emu_destroy(emu_ctx* ctx);
 
+namespace emu {
+namespace {
+// deposit_to_mesh (Evolve.cpp:91-126) into `fab` with the given box-list
+// scratch: zero the deposited components (valid and ghost cells), deposit,
+// then SumBoundary.  Launches kernels only.
+int deposit_into(emu_ctx* ctx, const PView& P, const Fab& fab, void* scratch, size_t bytes)
+{
+    const int nc = 3 + 8 * ctx->p.nf * ctx->p.nf;
+    hipLaunchKernelGGL(k_zero_comps, dim3(1024), dim3(kBlock), 0, ctx->stream, fab, 3, nc);
+    const int e = box_list_and_run(ctx, P, fab, true, scratch, bytes);
+    if (e != EMU_OK) return e;
+    const int64_t ncell = (int64_t)ctx->g.n[0] * ctx->g.n[1] * ctx->g.n[2];
+    hipLaunchKernelGGL(k_sum_boundary, dim3(nblocks(ncell)), dim3(kBlock), 0, ctx->stream, fab, ctx->g,
+                       ctx->ngrow[0], ctx->ngrow[1], ctx->ngrow[2], 3, nc);
+    EMU_LAUNCH_CHECK(ctx, "deposit_to_mesh");
+    return EMU_OK;
+}
+} // namespace
+} // namespace emu
+
 // ---------------------------------------------------------------------------
 // context
 // ---------------------------------------------------------------------------
@@ -380,6 +400,9 @@ extern "C" int emu_create(const emu_params* params, emu_ctx** out)
         emu_destroy(ctx);
         return EMU_ERR_NOMEM;
     }
+    // the static scratch of emu_deposit_to_mesh_static, looked up once per
+    // device; if that fails, the static entry point reports it on use
+    (void)static_scratch_init();
     if (hipMemset(ctx->d_red, 0, 8 * sizeof(unsigned long long)) != hipSuccess ||
         hipMemcpy(ctx->d_M2, ctx->M2, 9 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
         emu_destroy(ctx);
@@ -437,18 +460,24 @@ extern "C" int emu_deposit_to_mesh(emu_ctx* ctx, const emu_particles* parts, emu
     Fab fab;
     PView P;
     if (!make_fab(ctx, state, fab) || !make_pview(ctx, parts, P)) return set_error(ctx, EMU_ERR_ARG, "bad grid or particles");
-    const int nc = 3 + 8 * ctx->p.nf * ctx->p.nf;
-    hipLaunchKernelGGL(k_zero_comps, dim3(1024), dim3(kBlock), 0, ctx->stream, fab, 3, nc);
     const size_t need = box_scratch_size(P.np, fab.cstride, ctx->p.nf, false).total;
     void* scratch = nullptr;
-    int e = box_scratch(ctx, need, &scratch);
-    if (e == EMU_OK) e = box_list_and_run(ctx, P, fab, true, scratch, need);
+    const int e = box_scratch(ctx, need, &scratch);
     if (e != EMU_OK) return e;
-    const int64_t ncell = (int64_t)ctx->g.n[0] * ctx->g.n[1] * ctx->g.n[2];
-    hipLaunchKernelGGL(k_sum_boundary, dim3(nblocks(ncell)), dim3(kBlock), 0, ctx->stream, fab, ctx->g,
-                       ctx->ngrow[0], ctx->ngrow[1], ctx->ngrow[2], 3, nc);
-    EMU_LAUNCH_CHECK(ctx, "deposit_to_mesh");
-    return EMU_OK;
+    return deposit_into(ctx, P, fab, scratch, need);
+}
+
+extern "C" int emu_deposit_to_mesh_static(emu_ctx* ctx, const emu_particles* parts, emu_grid* state)
+{
+    if (!ctx) return EMU_ERR_ARG;
+    Fab fab;
+    PView P;
+    if (!make_fab(ctx, state, fab) || !make_pview(ctx, parts, P)) return set_error(ctx, EMU_ERR_ARG, "bad grid or particles");
+    void* scratch = nullptr;
+    size_t bytes = 0;
+    const int e = static_scratch(ctx, P.np, fab.cstride, &scratch, &bytes);
+    if (e != EMU_OK) return e;
+    return deposit_into(ctx, P, fab, scratch, bytes);
 }
 
 extern "C" int emu_fill_boundary(emu_ctx* ctx, emu_grid* state)

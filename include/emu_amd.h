@@ -106,6 +106,15 @@ int emu_grid_ngrow(const emu_ctx* ctx, int ngrow[3]);
  * ghosts keep their partial sums). */
 int emu_deposit_to_mesh(emu_ctx* ctx, const emu_particles* parts, emu_grid* state);
 
+/* emu_deposit_to_mesh without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops).  Its scratch is a fixed array in
+ * the library's device memory (47 MB per GPU, held until the process ends),
+ * for at most 4,194,304 particles and 1,048,576 fab cells, ghosts included;
+ * larger calls return EMU_ERR_UNSUPPORTED.  All calls on one GPU share that
+ * array: calls that overlap on one GPU (other streams, other contexts) race. */
+int emu_deposit_to_mesh_static(emu_ctx* ctx, const emu_particles* parts, emu_grid* state);
+
 /* state.FillBoundary(geom.periodicity()) (main.cpp:85,137). */
 int emu_fill_boundary(emu_ctx* ctx, emu_grid* state);
 
