@@ -44,10 +44,11 @@ __device__ __forceinline__ bool particle_box(const PView& P, int64_t ip, const F
                                              double w[3][3])
 {
     bool in = true;
+    const double x[3] = {P.pos[0][ip], P.pos[1][ip], P.pos[2][ip]}; // loads issued together
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         int idx;
-        shape3((P.pos[d][ip] - g.plo[d]) * g.dxi[d], g.order[d], idx, w[d]);
+        shape3((x[d] - g.plo[d]) * g.dxi[d], g.order[d], idx, w[d]);
         const int wd = g.order[d] > 0 ? 3 : 1;
         o[d] = idx - (wd == 3 ? 1 : 0) - fab.lo[d];
         in = in && o[d] >= 0 && o[d] + wd <= fab.nf[d];
@@ -244,94 +245,146 @@ __global__ __launch_bounds__(64 * kWavesGather) void k_gather_boxes(PView P, Fab
     }
 }
 
-constexpr int kWavesDeposit = 2;
+// The deposit takes boxes a patch at a time: PX x PY x PZ boxes (2 x 2 x 2 for
+// 3-D stencils, 1 x 1 x 8 along z for 1-D ones), one wave each.  Their
+// stencil regions overlap, so the waves add their sums into the patch's
+// region in LDS first (4 x 4 x 4 cells instead of 8 x 27) and the workgroup
+// adds that to the fab: 3.4x fewer device-scope atomics, which are what
+// bounds this kernel.
+template <int WX, int WY, int WZ> struct DepPatch {
+    static constexpr int PX = WX == 3 ? 2 : 1, PY = WY == 3 ? 2 : 1;
+    static constexpr int PZ = WZ == 3 ? 8 / (PX * PY) : 1;
+    static constexpr int WAVES = PX * PY * PZ;
+    static constexpr int RX = PX + WX - 1, RY = PY + WY - 1, RZ = PZ + WZ - 1, NR = RX * RY * RZ;
+};
 
-// LDS row of one particle: 9 shape factors, then its sources: N f_c and
-// Nbar fbar_c (2 NH), the multipliers (1, p/p_t) of N, Fx, Fy, Fz, and a zero
-// for the padding columns; odd strides, so that four particle rows read in
-// one instruction fall on different banks.
+// Particles staged per chunk, and the LDS row of one: 9 shape factors, then
+// its sources: N f_c and Nbar fbar_c (2 NH), the multipliers (1, p/p_t) of
+// N, Fx, Fy, Fz, and a zero for the padding columns; odd strides, so that four
+// particle rows read in one instruction fall on different banks.
+constexpr int kDepChunk = 32;
 template <int NF> struct DepRow {
     static constexpr int nh = NH<NF>;
     static constexpr int W = 9;
     static constexpr int S = 2 * nh + 5;
     static constexpr int ZERO = 2 * nh + 4;
-    static constexpr int doubles_per_wave = 64 * (W + S);
+    static constexpr int doubles_per_wave = kDepChunk * (W + S);
 };
 
 template <int NF, int WX, int WY, int WZ>
-__global__ __launch_bounds__(64 * kWavesDeposit) void k_deposit_boxes(PView P, Fab fab, Geom g,
-                                                                       const int* __restrict__ start,
-                                                                       const int* __restrict__ perm, int nk,
-                                                                       int blocks_per_xcd)
+constexpr int dep_lds_doubles()
+{
+    using PT = DepPatch<WX, WY, WZ>;
+    const int stage = PT::WAVES * DepRow<NF>::doubles_per_wave, merge = PT::NR * 8 * NH<NF>;
+    return stage > merge ? stage : merge;
+}
+
+template <int NF, int WX, int WY, int WZ>
+__global__ __launch_bounds__((64 * DepPatch<WX, WY, WZ>::WAVES)) void k_deposit_boxes(
+    PView P, Fab fab, Geom g, const int* __restrict__ start, const int* __restrict__ perm, int npx, int npy,
+    int64_t npatch, int blocks_per_xcd)
 {
     using I = PIdx<NF>;
     using R = DepRow<NF>;
+    using PT = DepPatch<WX, WY, WZ>;
     constexpr int nh = NH<NF>, NQ = 8 * nh;         // deposited components
-    constexpr int NT = WX * WY * WZ, MT = (NT + 15) / 16, NTL = (NQ + 15) / 16;
-    __shared__ double lds[kWavesDeposit * R::doubles_per_wave];
+    // full 16-column tiles (f64 16x16x4), then 4-column blocks (f64 4x4x4,
+    // four row blocks) for the rest: NF = 3's 72 columns are 4 tiles + 2 blocks
+    constexpr int NT = WX * WY * WZ, MT = (NT + 15) / 16, NTF = NQ / 16, NSM = (NQ - 16 * NTF + 3) / 4;
+    __shared__ double lds[dep_lds_doubles<NF, WX, WY, WZ>()];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int64_t box = box_block(blocks_per_xcd) * kWavesDeposit + wave;
-    if (box >= nk) return;
-    const int kb = __builtin_amdgcn_readfirstlane((int)box);
-    const int beg = __builtin_amdgcn_readfirstlane(start[kb]);
-    const int end = __builtin_amdgcn_readfirstlane(start[kb + 1]);
-    if (beg >= end) return;
+    const int64_t patch = box_block(blocks_per_xcd);
+    if (patch >= npatch) return; // the whole workgroup
+    const int ppx = (int)(patch % npx), ppy = (int)((patch / npx) % npy), ppz = (int)(patch / ((int64_t)npx * npy));
+    // this wave's box, in fab-box coordinates (none past the fab's edge)
+    const int wx = wave % PT::PX, wy = (wave / PT::PX) % PT::PY, wz = wave / (PT::PX * PT::PY);
+    const int bx = ppx * PT::PX + wx, by = ppy * PT::PY + wy, bz = ppz * PT::PZ + wz;
+    const bool have = bx < fab.nf[0] && by < fab.nf[1] && bz < fab.nf[2];
+    const int kb = have ? (bz * fab.nf[1] + by) * fab.nf[0] + bx : 0;
+    const int beg = have ? __builtin_amdgcn_readfirstlane(start[kb]) : 0;
+    const int end = have ? __builtin_amdgcn_readfirstlane(start[kb + 1]) : 0;
     double* Wf = lds + wave * R::doubles_per_wave;
-    double* Src = Wf + 64 * R::W;
+    double* Src = Wf + kDepChunk * R::W;
     // operand indices: A row t = 16m + l%16 is the product of the particle's
-    // weights at the target's (i, j, k); B column q = (2 v + bar) NH + c is
-    // source (bar, c) times multiplier v (fab component order, GIdx)
+    // weights at the target's (i, j, k) (rows past the box's NT cells are
+    // padding: tdx[m][2] = -1 gives them A = 0); B column q = (2 v + bar) NH + c
+    // is source (bar, c) times multiplier v (fab component order, GIdx)
     int tdx[MT][3];
 #pragma unroll
     for (int m = 0; m < MT; m++) {
         const int t = 16 * m + (lane & 15);
         const int ii = t % WX, jj = (t / WX) % WY, kk = t / (WX * WY);
-        // (rows past the box's NT cells are padding: tdx[m][2] = -1 gives them A = 0)
         tdx[m][0] = WX == 3 ? ii : 1;
         tdx[m][1] = 3 + (WY == 3 ? jj : 1);
         tdx[m][2] = t < NT ? 6 + (WZ == 3 ? kk : 1) : -1;
     }
-    int bi1[NTL], bi2[NTL];
+    auto src1 = [&](int q) { return q < NQ ? ((q / nh) % 2) * nh + q % nh : R::ZERO; };
+    auto src2 = [&](int q) { return q < NQ ? 2 * nh + (q / nh) / 2 : R::ZERO; };
+    int bi1[NTF], bi2[NTF], bs1[NSM > 0 ? NSM : 1], bs2[NSM > 0 ? NSM : 1];
 #pragma unroll
-    for (int n = 0; n < NTL; n++) {
-        const int q = 16 * n + (lane & 15);
-        const int v = (q / nh) / 2, bar = (q / nh) % 2, c = q % nh;
-        bi1[n] = q < NQ ? bar * nh + c : R::ZERO;
-        bi2[n] = q < NQ ? 2 * nh + v : R::ZERO;
+    for (int n = 0; n < NTF; n++) {
+        bi1[n] = src1(16 * n + (lane & 15));
+        bi2[n] = src2(16 * n + (lane & 15));
     }
-    mf_d4 acc[MT][NTL];
 #pragma unroll
-    for (int m = 0; m < MT; m++)
+    for (int k = 0; k < NSM; k++) {
+        bs1[k] = src1(16 * NTF + 4 * k + (lane & 3));
+        bs2[k] = src2(16 * NTF + 4 * k + (lane & 3));
+    }
+    mf_d4 acc[MT][NTF];
+    double accs[MT][NSM > 0 ? NSM : 1];
 #pragma unroll
-        for (int n = 0; n < NTL; n++) acc[m][n] = mf_d4{0.0, 0.0, 0.0, 0.0};
-    for (int base = beg; base < end; base += 64) {
-        const bool act = base + lane < end;
-        const int64_t ip = perm[act ? base + lane : beg];
+    for (int m = 0; m < MT; m++) {
+#pragma unroll
+        for (int n = 0; n < NTF; n++) acc[m][n] = mf_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int k = 0; k < NSM; k++) accs[m][k] = 0.0;
+    }
+    const int cl = lane & (kDepChunk - 1); // the chunk row this lane stages (lanes 32-63 repeat 0-31)
+    for (int base = beg; base < end; base += kDepChunk) {
+        const bool act = base + cl < end;
+        const int64_t ip = perm[act ? base + cl : beg];
         int o[3];
         double w[3][3];
         particle_box(P, ip, fab, g, o, w);
-        const double pupt = P.r[I::pupt][ip];
-        const double N = P.r[I::N][ip], Nb = P.r[I::Nbar][ip];
-        double* wr = Wf + lane * R::W;
-        double* sr = Src + lane * R::S;
+        // every lane loads (inactive ones a valid particle), so that the loads
+        // issue together, nu's batch then nubar's; inactive lanes stage zeros
+        double* wr = Wf + cl * R::W;
+        double* sr = Src + cl * R::S;
+        const bool row = lane < kDepChunk;
+        {
+            const double pupt = P.r[I::pupt][ip], pv[3] = {P.r[I::pupx][ip], P.r[I::pupy][ip], P.r[I::pupz][ip]};
+            const double N = P.r[I::N][ip];
+            double fv[nh];
 #pragma unroll
-        for (int d = 0; d < 3; d++)
+            for (int c = 0; c < nh; c++) fv[c] = P.r[I::f + c][ip];
+            if (row) {
 #pragma unroll
-            for (int e = 0; e < 3; e++) wr[3 * d + e] = act ? w[d][e] : 0.0;
+                for (int d = 0; d < 3; d++)
 #pragma unroll
-        for (int c = 0; c < nh; c++) {
-            sr[c] = act ? N * P.r[I::f + c][ip] : 0.0;
-            sr[nh + c] = act ? Nb * P.r[I::fbar + c][ip] : 0.0;
+                    for (int e = 0; e < 3; e++) wr[3 * d + e] = act ? w[d][e] : 0.0;
+#pragma unroll
+                for (int c = 0; c < nh; c++) sr[c] = act ? N * fv[c] : 0.0;
+                sr[2 * nh] = 1.0;
+#pragma unroll
+                for (int d = 0; d < 3; d++) sr[2 * nh + 1 + d] = pv[d] / pupt;
+                sr[R::ZERO] = 0.0;
+            }
         }
-        sr[2 * nh] = 1.0;
-        sr[2 * nh + 1] = P.r[I::pupx][ip] / pupt;
-        sr[2 * nh + 2] = P.r[I::pupy][ip] / pupt;
-        sr[2 * nh + 3] = P.r[I::pupz][ip] / pupt;
-        sr[R::ZERO] = 0.0;
+        {
+            const double Nb = P.r[I::Nbar][ip];
+            double fv[nh];
+#pragma unroll
+            for (int c = 0; c < nh; c++) fv[c] = P.r[I::fbar + c][ip];
+            if (row) {
+#pragma unroll
+                for (int c = 0; c < nh; c++) sr[nh + c] = act ? Nb * fv[c] : 0.0;
+            }
+        }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const int nks = (min(64, end - base) + 3) >> 2; // k-steps of 4 particles
+        const int nks = (min(kDepChunk, end - base) + 3) >> 2; // k-steps of 4 particles
         for (int ks = 0; ks < nks; ks++) {
             const int k = 4 * ks + (lane >> 4);
             const double* wk = Wf + k * R::W;
@@ -341,47 +394,84 @@ __global__ __launch_bounds__(64 * kWavesDeposit) void k_deposit_boxes(PView P, F
             for (int m = 0; m < MT; m++)
                 A[m] = tdx[m][2] < 0 ? 0.0 : wk[tdx[m][0]] * wk[tdx[m][1]] * wk[tdx[m][2]];
 #pragma unroll
-            for (int n = 0; n < NTL; n++) {
+            for (int n = 0; n < NTF; n++) {
                 const double Bv = sk[bi1[n]] * sk[bi2[n]];
 #pragma unroll
                 for (int m = 0; m < MT; m++) acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[m], Bv, acc[m][n], 0, 0, 0);
+            }
+#pragma unroll
+            for (int k = 0; k < NSM; k++) {
+                const double Bs = sk[bs1[k]] * sk[bs2[k]];
+#pragma unroll
+                for (int m = 0; m < MT; m++) accs[m][k] = __builtin_amdgcn_mfma_f64_4x4x4f64(A[m], Bs, accs[m][k], 0, 0, 0);
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-    // element r of lane l holds D[16m + 4r + l/16][16n + l%16]
-    const int bx = kb % fab.nf[0], by = (kb / fab.nf[0]) % fab.nf[1], bz = kb / (fab.nf[0] * fab.nf[1]);
+    // ---- merge the patch's boxes in LDS (the staging rows are free now), then
+    // add the region to the fab; the region's first cell is the patch's first box
+    double* Mb = lds; // [NR][NQ]
+    __syncthreads();
+    for (int e = threadIdx.x; e < PT::NR * NQ; e += 64 * PT::WAVES) Mb[e] = 0.0;
+    __syncthreads();
+    if (beg < end) {
+        // element r of lane l holds D[16m + 4r + l/16][16n + l%16]
 #pragma unroll
-    for (int m = 0; m < MT; m++)
+        for (int m = 0; m < MT; m++)
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int t = 16 * m + 4 * r + (lane >> 4);
-            if (t >= NT) continue;
-            const int ii = t % WX, jj = (t / WX) % WY, kk = t / (WX * WY);
-            const int64_t cell = ((int64_t)(bz + kk) * fab.nf[1] + (by + jj)) * fab.nf[0] + (bx + ii);
+            for (int r = 0; r < 4; r++) {
+                const int t = 16 * m + 4 * r + (lane >> 4);
+                if (t >= NT) continue;
+                const int ii = t % WX, jj = (t / WX) % WY, kk = t / (WX * WY);
+                const int rc = ((wz + kk) * PT::RY + (wy + jj)) * PT::RX + (wx + ii);
 #pragma unroll
-            for (int n = 0; n < NTL; n++) {
-                const int q = 16 * n + (lane & 15);
-                if (q < NQ) unsafeAtomicAdd(&fab.d[(int64_t)(3 + q) * fab.cstride + cell], acc[m][n][r]);
+                for (int n = 0; n < NTF; n++) unsafeAtomicAdd(&Mb[rc * NQ + 16 * n + (lane & 15)], acc[m][n][r]);
             }
-        }
+        // 4x4x4 blocks: element of lane l is D[16m + 4((l >> 2) & 3) + l/16][16 NTF + 4k + l%4]
+#pragma unroll
+        for (int m = 0; m < MT; m++)
+#pragma unroll
+            for (int k = 0; k < NSM; k++) {
+                const int t = 16 * m + 4 * ((lane >> 2) & 3) + (lane >> 4), q = 16 * NTF + 4 * k + (lane & 3);
+                if (t >= NT || q >= NQ) continue;
+                const int ii = t % WX, jj = (t / WX) % WY, kk = t / (WX * WY);
+                const int rc = ((wz + kk) * PT::RY + (wy + jj)) * PT::RX + (wx + ii);
+                unsafeAtomicAdd(&Mb[rc * NQ + q], accs[m][k]);
+            }
+    }
+    __syncthreads();
+    const int ox = ppx * PT::PX, oy = ppy * PT::PY, oz = ppz * PT::PZ;
+    for (int e = threadIdx.x; e < PT::NR * NQ; e += 64 * PT::WAVES) {
+        const double v = Mb[e];
+        if (v == 0.0) continue;
+        const int rc = e / NQ, q = e - rc * NQ;
+        const int cx = ox + rc % PT::RX, cy = oy + (rc / PT::RX) % PT::RY, cz = oz + rc / (PT::RX * PT::RY);
+        if (cx >= fab.nf[0] || cy >= fab.nf[1] || cz >= fab.nf[2]) continue;
+        const int64_t cell = ((int64_t)cz * fab.nf[1] + cy) * fab.nf[0] + cx;
+        unsafeAtomicAdd(&fab.d[(int64_t)(3 + q) * fab.cstride + cell], v);
+    }
 }
 
 template <int NF, int WX, int WY, int WZ>
 void launch_boxes(bool deposit, hipStream_t s, const PView& P, const Fab& fab, const Geom& g, const double* comb,
                   const double* M2, const int* start, const int* perm, int nk)
 {
-    const int waves = deposit ? kWavesDeposit : kWavesGather;
-    const int64_t nblock = ((int64_t)nk + waves - 1) / waves;
-    const int bpx = (int)((nblock + 7) / 8);
-    if (deposit)
-        hipLaunchKernelGGL((k_deposit_boxes<NF, WX, WY, WZ>), dim3(8 * bpx), dim3(64 * waves), 0, s, P, fab, g, start,
-                           perm, nk, bpx);
-    else
-        hipLaunchKernelGGL((k_gather_boxes<NF, WX, WY, WZ>), dim3(8 * bpx), dim3(64 * waves), 0, s, P, fab, g, comb,
-                           M2, start, perm, nk, bpx);
+    if (deposit) {
+        using PT = DepPatch<WX, WY, WZ>;
+        const int npx = (fab.nf[0] + PT::PX - 1) / PT::PX, npy = (fab.nf[1] + PT::PY - 1) / PT::PY;
+        const int npz = (fab.nf[2] + PT::PZ - 1) / PT::PZ;
+        const int64_t npatch = (int64_t)npx * npy * npz;
+        const int bpx = (int)((npatch + 7) / 8);
+        hipLaunchKernelGGL((k_deposit_boxes<NF, WX, WY, WZ>), dim3(8 * bpx), dim3(64 * PT::WAVES), 0, s, P, fab, g,
+                           start, perm, npx, npy, npatch, bpx);
+    } else {
+        const int64_t nblock = ((int64_t)nk + kWavesGather - 1) / kWavesGather;
+        const int bpx = (int)((nblock + 7) / 8);
+        hipLaunchKernelGGL((k_gather_boxes<NF, WX, WY, WZ>), dim3(8 * bpx), dim3(64 * kWavesGather), 0, s, P, fab, g,
+                           comb, M2, start, perm, nk, bpx);
+    }
 }
 
 template <int NF>
