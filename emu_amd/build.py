@@ -36,6 +36,7 @@ COMMON = ["-O3", f"--offload-arch={ARCH}", "-fPIC", "-std=c++17", "-munsafe-fp-a
 SOURCES = {
     "emu_ref.hip": ["-ffp-contract=off"],
     "emu_cells.hip": [],
+    "emu_rk.hip": ["-ffp-contract=off"],
     "emu_state.hip": [],
     "emu_init.hip": [],
     "emu_adjoint.hip": [],

@@ -70,6 +70,8 @@ _SIGS = [
     ("emu_redistribute_local", ctypes.c_int, [_P, ctypes.POINTER(EmuParticles)]),
     ("emu_apply_rhs", ctypes.c_int, [_P, ctypes.POINTER(EmuParticles), ctypes.c_double, ctypes.POINTER(EmuParticles)]),
     ("emu_copy_particles", ctypes.c_int, [_P, ctypes.POINTER(EmuParticles), ctypes.POINTER(EmuParticles)]),
+    ("emu_rk_combine", ctypes.c_int, [_P, ctypes.POINTER(EmuParticles), ctypes.POINTER(EmuParticles), ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_double), ctypes.POINTER(EmuParticles)]),
     ("emu_state_create", ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int, ctypes.POINTER(_P)]),
     ("emu_state_destroy", ctypes.c_int, [_P]),
     ("emu_state_load", ctypes.c_int, [_P, ctypes.POINTER(EmuParticles), _P]),

@@ -139,6 +139,20 @@ class FlavoredNeutrinoContainer:
         v, d = self.view(), dFdt.view()
         check(lib().emu_apply_rhs(self.emu.ctx, ctypes.byref(v), a, ctypes.byref(d)), "apply_rhs", self.emu.ctx)
 
+    def rk_combine(self, src: "FlavoredNeutrinoContainer", coeffs, Fs):
+        """self = src, then self = a_j F_j + self for each (a_j, F_j), in one
+        pass: the stage combination of the RK integrator (emu_rk_combine)."""
+        coeffs = [float(a) for a in coeffs]
+        if len(coeffs) != len(Fs) or any(F.np != self.np for F in Fs) or src.np != self.np:
+            raise ValueError("size mismatch")
+        n = len(coeffs)
+        a = (ctypes.c_double * max(n, 1))(*coeffs)
+        views = (capi.EmuParticles * max(n, 1))(*[F.view() for F in Fs])
+        d, sv = self.view(), src.view()
+        check(lib().emu_rk_combine(self.emu.ctx, ctypes.byref(d), ctypes.byref(sv), n, a, views), "rk_combine",
+              self.emu.ctx)
+        self.ids.copy_(src.ids)
+
     def Vvac_max(self) -> float:
         v = self.view()
         out = ctypes.c_double()
@@ -247,15 +261,13 @@ class Integrator:
         interpolate_rhs_from_mesh(F, self.state)           # main.cpp:151
 
     def advance(self):
+        # each stage state and the result: copyParticles(old), then one
+        # ApplyFlavoredNeutrinoRHS per earlier stage -- one pass (rk_combine)
         dt = self.dt
         for i in range(len(self.B)):
-            self.new.copyParticles(self.old)
-            for j, a in enumerate(self.A[i]):
-                self.new.apply_rhs(dt * a, self.F[j])
+            self.new.rk_combine(self.old, [dt * a for a in self.A[i]], self.F[:len(self.A[i])])
             self.rhs(self.F[i], self.new)
-        self.new.copyParticles(self.old)
-        for i in range(len(self.B)):
-            self.new.apply_rhs(dt * self.B[i], self.F[i])
+        self.new.rk_combine(self.old, [dt * b for b in self.B], self.F[:len(self.B)])
         self.time += dt
         self.step_number += 1
         # post_timestep_fun, main.cpp:155-199

@@ -409,6 +409,19 @@ void FlavoredNeutrinoContainer::copyParticles(const FlavoredNeutrinoContainer& o
     hip_check(hipMemcpy(ids_, other.ids_, (size_t)np_ * sizeof(int64_t), hipMemcpyDeviceToDevice), "copy ids");
 }
 
+void FlavoredNeutrinoContainer::rk_combine(const FlavoredNeutrinoContainer& src, int n, const double* a,
+                                           const FlavoredNeutrinoContainer* const* F)
+{
+    resize(src.np_);
+    if (np_ == 0) return;
+    std::vector<emu_particles> views((size_t)std::max(n, 1));
+    for (int j = 0; j < n; j++) views[j] = F[j]->view_;
+    emu_check(emu_rk_combine(geom_.ctx(), &view_, &src.view_, n, a, views.data()), "rk_combine", geom_.ctx());
+    // the ids travel with the copy, in stream order (the context's stream is the null stream)
+    hip_check(hipMemcpyAsync(ids_, src.ids_, (size_t)np_ * sizeof(int64_t), hipMemcpyDeviceToDevice, nullptr),
+              "copy ids");
+}
+
 void FlavoredNeutrinoContainer::SyncLocation(int type)
 {
     if (type != Sync::CoordinateToPosition && type != Sync::PositionToCoordinate) Abort("SyncLocation: bad type");
@@ -564,13 +577,16 @@ TimeIntegrator::~TimeIntegrator()
 
 void TimeIntegrator::advance(double dt)
 {
+    // each stage state and the result: copyParticles(old) followed by one
+    // ApplyFlavoredNeutrinoRHS per earlier stage, as one pass (rk_combine)
+    double a[4];
     for (int i = 0; i < tab_.nstages; i++) {
-        new_->copyParticles(*old_, true);
-        for (int j = 0; j < i; j++) new_->apply_rhs(dt * tab_.A[i][j], *F_[j]);
+        for (int j = 0; j < i; j++) a[j] = dt * tab_.A[i][j];
+        new_->rk_combine(*old_, i, a, F_.data());
         rhs_(*F_[i], *new_, time_ + tab_.c[i] * dt);
     }
-    new_->copyParticles(*old_, true);
-    for (int i = 0; i < tab_.nstages; i++) new_->apply_rhs(dt * tab_.b[i], *F_[i]);
+    for (int i = 0; i < tab_.nstages; i++) a[i] = dt * tab_.b[i];
+    new_->rk_combine(*old_, tab_.nstages, a, F_.data());
     time_ += dt;
 }
 

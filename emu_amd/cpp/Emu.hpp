@@ -177,6 +177,9 @@ public:
 
     // ApplyFlavoredNeutrinoRHS over the container (FlavoredNeutrinoContainer.H:33-47)
     void apply_rhs(double a, const FlavoredNeutrinoContainer& dFdt);
+    // copyParticles(src) followed by apply_rhs(a[j], *F[j]) for j < n, in one pass
+    void rk_combine(const FlavoredNeutrinoContainer& src, int n, const double* a,
+                    const FlavoredNeutrinoContainer* const* F);
     // host copies (SoA, particle order), for output and restart
     void to_host(std::vector<double>& pos, std::vector<double>& rdata, std::vector<int64_t>& ids) const;
     void from_host(int64_t np, const std::vector<double>& pos, const std::vector<double>& rdata,

@@ -275,7 +275,7 @@ template <int NF, int WX, int WY, int WZ>
 constexpr int dep_lds_doubles()
 {
     using PT = DepPatch<WX, WY, WZ>;
-    const int stage = PT::WAVES * DepRow<NF>::doubles_per_wave, merge = PT::NR * 8 * NH<NF>;
+    const int stage = PT::WAVES * DepRow<NF>::doubles_per_wave, merge = PT::NR * (8 * NH<NF> + 1);
     return stage > merge ? stage : merge;
 }
 
@@ -412,9 +412,10 @@ __global__ __launch_bounds__((64 * DepPatch<WX, WY, WZ>::WAVES)) void k_deposit_
     }
     // ---- merge the patch's boxes in LDS (the staging rows are free now), then
     // add the region to the fab; the region's first cell is the patch's first box
-    double* Mb = lds; // [NR][NQ]
+    constexpr int NQS = NQ + 1; // odd row stride: a wave reading one component over the cells spreads over the banks
+    double* Mb = lds; // [NR][NQS]
     __syncthreads();
-    for (int e = threadIdx.x; e < PT::NR * NQ; e += 64 * PT::WAVES) Mb[e] = 0.0;
+    for (int e = threadIdx.x; e < PT::NR * NQS; e += 64 * PT::WAVES) Mb[e] = 0.0;
     __syncthreads();
     if (beg < end) {
         // element r of lane l holds D[16m + 4r + l/16][16n + l%16]
@@ -427,7 +428,7 @@ __global__ __launch_bounds__((64 * DepPatch<WX, WY, WZ>::WAVES)) void k_deposit_
                 const int ii = t % WX, jj = (t / WX) % WY, kk = t / (WX * WY);
                 const int rc = ((wz + kk) * PT::RY + (wy + jj)) * PT::RX + (wx + ii);
 #pragma unroll
-                for (int n = 0; n < NTF; n++) unsafeAtomicAdd(&Mb[rc * NQ + 16 * n + (lane & 15)], acc[m][n][r]);
+                for (int n = 0; n < NTF; n++) unsafeAtomicAdd(&Mb[rc * NQS + 16 * n + (lane & 15)], acc[m][n][r]);
             }
         // 4x4x4 blocks: element of lane l is D[16m + 4((l >> 2) & 3) + l/16][16 NTF + 4k + l%4]
 #pragma unroll
@@ -438,15 +439,17 @@ __global__ __launch_bounds__((64 * DepPatch<WX, WY, WZ>::WAVES)) void k_deposit_
                 if (t >= NT || q >= NQ) continue;
                 const int ii = t % WX, jj = (t / WX) % WY, kk = t / (WX * WY);
                 const int rc = ((wz + kk) * PT::RY + (wy + jj)) * PT::RX + (wx + ii);
-                unsafeAtomicAdd(&Mb[rc * NQ + q], accs[m][k]);
+                unsafeAtomicAdd(&Mb[rc * NQS + q], accs[m][k]);
             }
     }
     __syncthreads();
+    // one component per wave-wide pass over the region's cells, so that a wave's
+    // atomics fall on runs of neighbouring cells of one fab component
     const int ox = ppx * PT::PX, oy = ppy * PT::PY, oz = ppz * PT::PZ;
     for (int e = threadIdx.x; e < PT::NR * NQ; e += 64 * PT::WAVES) {
-        const double v = Mb[e];
+        const int q = e / PT::NR, rc = e - q * PT::NR;
+        const double v = Mb[rc * NQS + q];
         if (v == 0.0) continue;
-        const int rc = e / NQ, q = e - rc * NQ;
         const int cx = ox + rc % PT::RX, cy = oy + (rc / PT::RX) % PT::RY, cz = oz + rc / (PT::RX * PT::RY);
         if (cx >= fab.nf[0] || cy >= fab.nf[1] || cz >= fab.nf[2]) continue;
         const int64_t cell = ((int64_t)cz * fab.nf[1] + cy) * fab.nf[0] + cx;

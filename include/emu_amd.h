@@ -153,6 +153,15 @@ int emu_apply_rhs(emu_ctx* ctx, emu_particles* parts, double a, const emu_partic
 /* copyParticles(src, true) (main.cpp:115,148): attribute-wise device copy. */
 int emu_copy_particles(emu_ctx* ctx, emu_particles* dst, const emu_particles* src);
 
+/* One stage combination of the AMReX Runge-Kutta integrator the reference
+ * drives (main.cpp:129,202-213): dst = src (copyParticles), then for j = 0 ..
+ * n-1 dst = a[j] F[j] + dst (ApplyFlavoredNeutrinoRHS,
+ * FlavoredNeutrinoContainer.H:33-47), in one pass and bit for bit that
+ * sequence of calls.  F is a host array of n (<= 4) particle views; dst must
+ * not alias src or any F[j]. */
+int emu_rk_combine(emu_ctx* ctx, emu_particles* dst, const emu_particles* src, int n, const double* a,
+                   const emu_particles* F);
+
 /* ---- fused state ----------------------------------------------------------
  * The integrator loop of main.cpp:129-213 (AMReX TimeIntegrator RK4 + the
  * post-step hook) on a library-owned SoA state sorted by cell.  precision is

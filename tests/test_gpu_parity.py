@@ -247,6 +247,33 @@ def oracle_steps(oracle, nf, prm, parts, nsteps, fixed_dt=None):
     return dts, dt
 
 
+@pytest.mark.parametrize("n", [0, 1, 3, 4])
+def test_rk_combine_is_copy_then_saxpys(core, oracle, n):
+    """emu_rk_combine equals copyParticles followed by one apply_rhs per stage,
+    bit for bit (the same roundings in the same order)."""
+    import torch
+    nf = 3
+    d = cases.fast_flavor_dict(nf=nf, ncell=(3, 3, 3), nphi=4)
+    deck, prm, emu = make(core, oracle, d)
+    parts, K = random_case(oracle, nf, (3, 3, 3), (1.5, 1.5, 1.5), 500, seed=21)
+    src = core.FlavoredNeutrinoContainer.from_aos(emu, parts)
+    gen = torch.Generator(device="cpu").manual_seed(n)
+    Fs = []
+    for _ in range(n):
+        F = core.FlavoredNeutrinoContainer(emu, src.np)
+        F.rdata.copy_(torch.randn(F.rdata.shape, generator=gen, dtype=torch.float64))
+        Fs.append(F)
+    a = [1.3e-12 * (j + 1) for j in range(n)]
+    want = core.FlavoredNeutrinoContainer(emu, src.np)
+    want.copyParticles(src)
+    for aj, F in zip(a, Fs):
+        want.apply_rhs(aj, F)
+    got = core.FlavoredNeutrinoContainer(emu, src.np)
+    got.rk_combine(src, a, Fs)
+    assert torch.equal(got.rdata, want.rdata) and torch.equal(got.pos, want.pos)
+    assert torch.equal(got.ids, src.ids)
+
+
 @pytest.mark.parametrize("nf", [2, 3])
 def test_integrator_parity(core, oracle, nf):
     d = cases.fast_flavor_dict(nf=nf, ncell=(4, 4, 4), nphi=4)
