@@ -459,21 +459,99 @@ __global__ void k_renorm_soa(int64_t np, double* Y, const double* Ll, double max
 // Step-level kernels: one launch per RK4 stage after that stage's deposit.
 // The stage algebra is step()'s (AMReX RK4 with the reference's saxpys,
 // FlavoredNeutrinoContainer.H:39-46), each a x + y rounded as written.
+//
+// They take one cell per wave (the cell list's runs): every particle of the
+// wave has its stencil in the cell's 27-cell neighbourhood, so the grid rows
+// are the same for all lanes and are read as scalar loads, one neighbour at
+// a time, while each lane forms its own weights.
 // ---------------------------------------------------------------------------
+constexpr int kCellWaves = 4; // waves (cells) per workgroup
+
+// The wave's cell, in an XCD-aware order (consecutive cells share an XCD's
+// L2); -1 past the last cell.
+__device__ __forceinline__ int wave_cell(int64_t ncell, int blocks_per_xcd)
+{
+    const int b = blockIdx.x;
+    const int64_t c = ((int64_t)(b & 7) * blocks_per_xcd + (b >> 3)) * kCellWaves + (threadIdx.x >> 6);
+    return c < ncell ? __builtin_amdgcn_readfirstlane((int)c) : -1;
+}
+
+// out[c] += sum over the neighbourhood of `cell` (wave-uniform) of
+// w_t (G_t[0][c] + s0 G_t[1][c] + s1 G_t[2][c] + s2 G_t[3][c]), with the
+// particle's shape factors w (shape3's windows) and multipliers s.
 template <int NF>
-__device__ __forceinline__ void stage_rhs(const AdjIn& in, int64_t p, const Geom& g, const double* grid,
-                                          const HamParams& hp, double* k)
+__device__ __forceinline__ void gather_cell_box(const Geom& g, int cell, const double* grid_, const double w[3][3],
+                                                const double s[3], double* out)
+{
+    constexpr int nh = NH<NF>, QG = 4 * nh;
+    typedef __attribute__((address_space(4))) const double* cdp;
+    const cdp grid = (cdp)grid_;
+    const int cx = cell % g.n[0], cy = (cell / g.n[0]) % g.n[1], cz = cell / (g.n[0] * g.n[1]);
+    int lo[3], hi[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        lo[d] = g.order[d] > 0 ? 0 : 1;
+        hi[d] = g.order[d] > 0 ? 3 : 2;
+    }
+    auto pick = [](const double* x, int i) { return i == 0 ? x[0] : (i == 1 ? x[1] : x[2]); };
+#pragma unroll 1
+    for (int kk = lo[2]; kk < hi[2]; kk++) {
+        const int nz = pmod(cz + kk - 1, g.n[2]);
+#pragma unroll 1
+        for (int jj = lo[1]; jj < hi[1]; jj++) {
+            const int ny = pmod(cy + jj - 1, g.n[1]);
+            const double wyz = pick(w[1], jj) * pick(w[2], kk);
+#pragma unroll 1
+            for (int ii = lo[0]; ii < hi[0]; ii++) {
+                const int64_t nb = ((int64_t)nz * g.n[1] + ny) * g.n[0] + pmod(cx + ii - 1, g.n[0]);
+                const double wt = pick(w[0], ii) * wyz;
+                const double a[4] = {wt, wt * s[0], wt * s[1], wt * s[2]};
+                const cdp G = grid + nb * QG;
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+#pragma unroll
+                    for (int c = 0; c < nh; c++) out[c] += a[v] * G[v * nh + c];
+            }
+        }
+    }
+}
+
+// The particle's shape factors and direction (positions frozen within a step).
+__device__ __forceinline__ void particle_shape(const AdjIn& in, int64_t p, const Geom& g, double w[3][3], double v[3])
+{
+    const int64_t np = in.np;
+    const double x[3] = {in.pos[p], in.pos[np + p], in.pos[2 * np + p]};
+    const double pt = in.pup[3 * np + p];
+    v[0] = in.pup[p] / pt;
+    v[1] = in.pup[np + p] / pt;
+    v[2] = in.pup[2 * np + p] / pt;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        int idx;
+        shape3((x[d] - g.plo[d]) * g.dxi[d], g.order[d], idx, w[d]);
+    }
+}
+
+// H/hbar and Hbar/hbar of a particle of the wave's cell (as hamiltonians()).
+template <int NF>
+__device__ __forceinline__ void cell_hamiltonians(const AdjIn& in, int64_t p, const Geom& g, int cell,
+                                                  const double* grid, const HamParams& hp, double* H, double* Hb)
 {
     constexpr int nh = NH<NF>;
-    const int64_t np = in.np;
-    double v[3], H[nh], Hb[nh], f[nh];
-    hamiltonians<NF>(in, p, g, grid, hp, v, H, Hb);
+    double w[3][3], v[3];
+    particle_shape(in, p, g, w, v);
+    const double s[3] = {-v[0], -v[1], -v[2]};
+    double inside[nh];
 #pragma unroll
-    for (int c = 0; c < nh; c++) f[c] = in.Y[(int64_t)c * np + p];
-    commutator_rhs<NF, double>(H, f, k, 1.0);
+    for (int c = 0; c < nh; c++) inside[c] = 0.0;
+    gather_cell_box<NF>(g, cell, grid, w, s, inside);
+    const double pt = in.pup[3 * in.np + p];
 #pragma unroll
-    for (int c = 0; c < nh; c++) f[c] = in.Y[(int64_t)(nh + c) * np + p];
-    commutator_rhs<NF, double>(Hb, f, k + nh, 1.0);
+    for (int c = 0; c < nh; c++) {
+        const double hv = hp.vac[c] / pt, hs = hp.coef * inside[c];
+        H[c] = hv + hs;
+        Hb[c] = is_im_comp<NF>(c) ? hs - hv : hv - hs;
+    }
 }
 
 // Stage S of one step, in.Y = Y_S, grid = its deposit.
@@ -483,49 +561,64 @@ __device__ __forceinline__ void stage_rhs(const AdjIn& in, int64_t p, const Geom
 //  and advances the particle's position.
 //  STORE 1 (the VJP's recompute): S = 1..3 write Y_{S+1} into ynext only.
 template <int NF, int S, int STORE>
-__global__ __launch_bounds__(256) void k_fwd_stage(AdjIn in, Geom g, const int* perm, const double* grid,
-                                                   HamParams hp, const double* y0, double* ynext, double* acc,
-                                                   const double* Ll, double maxError, double dt, double* pos_out,
-                                                   unsigned long long* err)
+__global__ __launch_bounds__(64 * kCellWaves) void k_fwd_stage(AdjIn in, Geom g, const int* start, const int* perm,
+                                                               int64_t ncell, int bpx, const double* grid,
+                                                               HamParams hp, const double* y0, double* ynext,
+                                                               double* acc, const double* Ll, double maxError,
+                                                               double dt, double* pos_out, unsigned long long* err)
 {
-#pragma clang fp contract(off)
     constexpr int nh = NH<NF>, S2 = 2 * nh;
     const int64_t np = in.np;
+    const int cell = wave_cell(ncell, bpx);
+    if (cell < 0) return;
+    const int beg = __builtin_amdgcn_readfirstlane(start[cell]), end = __builtin_amdgcn_readfirstlane(start[cell + 1]);
+    const int lane = threadIdx.x & 63;
     const double a_next = S == 3 ? dt * 1.0 : dt * 0.5;
     const double a_acc = S == 1 || S == 4 ? dt * (1. / 6.) : dt * (1. / 3.);
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < np; i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t p = perm[i];
-        double k[S2];
-        stage_rhs<NF>(in, p, g, grid, hp, k);
-        if constexpr (S < 4) {
+    for (int base = beg; base < end; base += 64) {
+        const bool act = base + lane < end;
+        const int64_t p = perm[act ? base + lane : beg];
+        double H[nh], Hb[nh], f[nh], k[S2];
+        cell_hamiltonians<NF>(in, p, g, cell, grid, hp, H, Hb);
+        if (!act) continue;
 #pragma unroll
-            for (int c = 0; c < S2; c++) {
-                const double y = y0[(int64_t)c * np + p];
-                ynext[(int64_t)c * np + p] = a_next * k[c] + y;
-                if constexpr (STORE == 0)
-                    acc[(int64_t)c * np + p] = a_acc * k[c] + (S == 1 ? y : acc[(int64_t)c * np + p]);
+        for (int c = 0; c < nh; c++) f[c] = in.Y[(int64_t)c * np + p];
+        commutator_rhs<NF, double>(H, f, k, 1.0);
+#pragma unroll
+        for (int c = 0; c < nh; c++) f[c] = in.Y[(int64_t)(nh + c) * np + p];
+        commutator_rhs<NF, double>(Hb, f, k + nh, 1.0);
+        {
+#pragma clang fp contract(off)
+            if constexpr (S < 4) {
+#pragma unroll
+                for (int c = 0; c < S2; c++) {
+                    const double y = y0[(int64_t)c * np + p];
+                    ynext[(int64_t)c * np + p] = a_next * k[c] + y;
+                    if constexpr (STORE == 0)
+                        acc[(int64_t)c * np + p] = a_acc * k[c] + (S == 1 ? y : acc[(int64_t)c * np + p]);
+                }
+            } else {
+                double y[S2];
+#pragma unroll
+                for (int c = 0; c < S2; c++) y[c] = a_acc * k[c] + acc[(int64_t)c * np + p];
+                // Renormalize (FlavoredNeutrinoContainer.cpp:89-113): nu, then nubar
+                // unless nu failed; a failing half is left as it is
+                int e = renorm_fp64<NF>(y, Ll[p], maxError);
+                if (e == EMU_OK) e = renorm_fp64<NF>(y + nh, Ll[np + p], maxError);
+                if (e != EMU_OK) atomicMin(err, status_word(p, e));
+#pragma unroll
+                for (int c = 0; c < S2; c++) acc[(int64_t)c * np + p] = y[c];
+                double x[3], v[3], nx[3];
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    x[d] = in.pos[(int64_t)d * np + p];
+                    v[d] = in.pup[(int64_t)d * np + p] / in.pup[3 * np + p] * phys::c;
+                }
+                int nc;
+                end_position(x, v, dt, g, nx, nc);
+#pragma unroll
+                for (int d = 0; d < 3; d++) pos_out[(int64_t)d * np + p] = nx[d];
             }
-        } else {
-            double y[S2];
-#pragma unroll
-            for (int c = 0; c < S2; c++) y[c] = a_acc * k[c] + acc[(int64_t)c * np + p];
-            // Renormalize (FlavoredNeutrinoContainer.cpp:89-113): nu, then nubar
-            // unless nu failed; a failing half is left as it is
-            int e = renorm_fp64<NF>(y, Ll[p], maxError);
-            if (e == EMU_OK) e = renorm_fp64<NF>(y + nh, Ll[np + p], maxError);
-            if (e != EMU_OK) atomicMin(err, status_word(p, e));
-#pragma unroll
-            for (int c = 0; c < S2; c++) acc[(int64_t)c * np + p] = y[c];
-            double x[3], v[3], nx[3];
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-                x[d] = in.pos[(int64_t)d * np + p];
-                v[d] = in.pup[(int64_t)d * np + p] / in.pup[3 * np + p] * phys::c;
-            }
-            int nc;
-            end_position(x, v, dt, g, nx, nc);
-#pragma unroll
-            for (int d = 0; d < 3; d++) pos_out[(int64_t)d * np + p] = nx[d];
         }
     }
 }
@@ -534,23 +627,31 @@ __global__ __launch_bounds__(256) void k_fwd_stage(AdjIn in, Geom g, const int* 
 // reverse of the stage algebra; beta = 0 at stage 4), then the commutator
 // adjoints: lY_S (commutator part) and mu, as k_vjp_comm.
 template <int NF>
-__global__ __launch_bounds__(256) void k_vjp_stage_comm(AdjIn in, Geom g, const int* perm, const double* grid,
-                                                        HamParams hp, const double* lam, const double* lY_prev,
-                                                        double alpha, double beta, double* lY, double* mu)
+__global__ __launch_bounds__(64 * kCellWaves) void k_vjp_stage_comm(AdjIn in, Geom g, const int* start,
+                                                                    const int* perm, int64_t ncell, int bpx,
+                                                                    const double* grid, HamParams hp,
+                                                                    const double* lam, const double* lY_prev,
+                                                                    double alpha, double beta, double* lY, double* mu)
 {
-#pragma clang fp contract(off)
     constexpr int nh = NH<NF>;
     const int64_t np = in.np;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < np; i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t p = perm[i];
-        double v[3], H[nh], Hb[nh];
-        hamiltonians<NF>(in, p, g, grid, hp, v, H, Hb);
+    const int cell = wave_cell(ncell, bpx);
+    if (cell < 0) return;
+    const int beg = __builtin_amdgcn_readfirstlane(start[cell]), end = __builtin_amdgcn_readfirstlane(start[cell + 1]);
+    const int lane = threadIdx.x & 63;
+    for (int base = beg; base < end; base += 64) {
+        const bool act = base + lane < end;
+        const int64_t p = perm[act ? base + lane : beg];
+        double H[nh], Hb[nh];
+        cell_hamiltonians<NF>(in, p, g, cell, grid, hp, H, Hb);
+        if (!act) continue;
         double gH[2][nh];
 #pragma unroll
         for (int bar = 0; bar < 2; bar++) {
             double lk[nh], L[nh], f[nh], gy[nh];
 #pragma unroll
             for (int c = 0; c < nh; c++) {
+#pragma clang fp contract(off)
                 const int64_t o = (int64_t)(bar * nh + c) * np + p;
                 const double t = alpha * lam[o];
                 lk[c] = lY_prev ? beta * lY_prev[o] + t : t;
@@ -574,31 +675,33 @@ __global__ __launch_bounds__(256) void k_vjp_stage_comm(AdjIn in, Geom g, const 
 // k_vjp_gather), then the running sum lam_out = lY_S + (lam, at the first
 // reversed stage, else lam_out).
 template <int NF>
-__global__ __launch_bounds__(256) void k_vjp_stage_gather(AdjIn in, Geom g, const int* perm, const double* gadj,
-                                                          double* lY, double* lamNbar, const double* lam,
-                                                          double* lam_out, int first)
+__global__ __launch_bounds__(64 * kCellWaves) void k_vjp_stage_gather(AdjIn in, Geom g, const int* start,
+                                                                      const int* perm, int64_t ncell, int bpx,
+                                                                      const double* gadj, double* lY,
+                                                                      double* lamNbar, const double* lam,
+                                                                      double* lam_out, int first)
 {
-#pragma clang fp contract(off)
-    constexpr int nh = NH<NF>, QG = 4 * nh;
+    constexpr int nh = NH<NF>;
     const int64_t np = in.np;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < np; i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t p = perm[i];
-        const double pos[3] = {in.pos[p], in.pos[np + p], in.pos[2 * np + p]};
-        const double pt = in.pup[3 * np + p];
-        const double v[3] = {in.pup[p] / pt, in.pup[np + p] / pt, in.pup[2 * np + p] / pt};
+    const int cell = wave_cell(ncell, bpx);
+    if (cell < 0) return;
+    const int beg = __builtin_amdgcn_readfirstlane(start[cell]), end = __builtin_amdgcn_readfirstlane(start[cell + 1]);
+    const int lane = threadIdx.x & 63;
+    for (int base = beg; base < end; base += 64) {
+        const bool act = base + lane < end;
+        const int64_t p = perm[act ? base + lane : beg];
+        double w[3][3], v[3];
+        particle_shape(in, p, g, w, v);
         double lq[nh];
 #pragma unroll
         for (int c = 0; c < nh; c++) lq[c] = 0.0;
-        for_stencil(g, pos, [&](int64_t cell, double w) {
-            const double* G = gadj + cell * QG;
-#pragma unroll
-            for (int c = 0; c < nh; c++)
-                lq[c] += w * (G[c] + v[0] * G[nh + c] + v[1] * G[2 * nh + c] + v[2] * G[3 * nh + c]);
-        });
+        gather_cell_box<NF>(g, cell, gadj, w, v, lq);
+        if (!act) continue;
         const double N = in.Nn[p], Nb = in.Nn[np + p];
         double ln = 0.0;
 #pragma unroll
         for (int c = 0; c < nh; c++) {
+#pragma clang fp contract(off)
             const double sg = is_im_comp<NF>(c) ? 1.0 : -1.0;
             const int64_t o0 = (int64_t)c * np + p, o1 = (int64_t)(nh + c) * np + p;
             const double l0 = lY[o0] + N * lq[c], l1 = lY[o1] + sg * Nb * lq[c];
@@ -898,6 +1001,13 @@ StepWork step_work(const emu_ctx* ctx, int64_t np, double* work)
     return w;
 }
 
+// blocks per XCD of a one-wave-per-cell launch
+int cell_blocks_per_xcd(int64_t ncell)
+{
+    const int64_t nb = (ncell + kCellWaves - 1) / kCellWaves;
+    return (int)((nb + 7) / 8);
+}
+
 int arm_status(emu_ctx* ctx)
 {
     if (!ctx->adj_status_armed) {
@@ -911,9 +1021,11 @@ template <int NF, int S, int STORE>
 void fwd_stage(emu_ctx* ctx, const AdjIn& in, const StepWork& w, const double* grid, const double* y0, double* ynext,
                double* acc, const double* Ll, double dt, double* pos)
 {
-    const int64_t np = in.np;
-    hipLaunchKernelGGL((k_fwd_stage<NF, S, STORE>), dim3(nblk(np)), dim3(kB), 0, ctx->stream, in, ctx->g, w.cs.perm,
-                       grid, ham_params(ctx), y0, ynext, acc, Ll, ctx->p.maxError, dt, pos, ctx->d_red + 5);
+    const int64_t nc = ncell_of(ctx);
+    const int bpx = cell_blocks_per_xcd(nc);
+    hipLaunchKernelGGL((k_fwd_stage<NF, S, STORE>), dim3(8 * bpx), dim3(64 * kCellWaves), 0, ctx->stream, in, ctx->g,
+                       w.cs.start, w.cs.perm, nc, bpx, grid, ham_params(ctx), y0, ynext, acc, Ll, ctx->p.maxError, dt,
+                       pos, ctx->d_red + 5);
 }
 
 template <int NF>
@@ -961,18 +1073,21 @@ int adj_step_vjp_nf(emu_ctx* ctx, int64_t np, const double* pos, const double* p
     // reverse: lambda_k4 = dt/6 lam; lambda_k3 = dt/3 lam + dt lY4;
     // lambda_k2 = dt/3 lam + dt/2 lY3; lambda_k1 = dt/6 lam + dt/2 lY2;
     // lam_out = lam + lY4 + lY3 + lY2 + lY1
+    const int bpx = cell_blocks_per_xcd(ncell_of(ctx));
     const double alpha[4] = {dt * (1. / 6.), dt * (1. / 3.), dt * (1. / 3.), dt * (1. / 6.)};
     const double beta[4] = {dt * 0.5, dt * 0.5, dt * 1.0, 0.0};
     for (int s = 3; s >= 0; s--) {
         in.Y = Yst[s];
         double* lY = w.lY[s & 1];
         const double* lYp = s == 3 ? nullptr : w.lY[(s + 1) & 1];
-        hipLaunchKernelGGL(k_vjp_stage_comm<NF>, dim3(nblk(np)), dim3(kB), 0, ctx->stream, in, ctx->g, w.cs.perm,
-                           w.G[s], ham_params(ctx), lam_in, lYp, alpha[s], beta[s], lY, w.mu);
+        hipLaunchKernelGGL(k_vjp_stage_comm<NF>, dim3(8 * bpx), dim3(64 * kCellWaves), 0, ctx->stream, in, ctx->g,
+                           w.cs.start, w.cs.perm, ncell_of(ctx), bpx, w.G[s], ham_params(ctx), lam_in, lYp, alpha[s],
+                           beta[s], lY, w.mu);
         hipLaunchKernelGGL(k_zero, dim3(nblk(ng)), dim3(kB), 0, ctx->stream, w.gadj, ng);
         seg_deposit<1>(ctx, in, w.cs.perm, w.mu, w.gadj);
-        hipLaunchKernelGGL(k_vjp_stage_gather<NF>, dim3(nblk(np)), dim3(kB), 0, ctx->stream, in, ctx->g, w.cs.perm,
-                           w.gadj, lY, lam_Nbar, lam_in, lam_out, s == 3 ? 1 : 0);
+        hipLaunchKernelGGL(k_vjp_stage_gather<NF>, dim3(8 * bpx), dim3(64 * kCellWaves), 0, ctx->stream, in, ctx->g,
+                           w.cs.start, w.cs.perm, ncell_of(ctx), bpx, w.gadj, lY, lam_Nbar, lam_in, lam_out,
+                           s == 3 ? 1 : 0);
     }
     return EMU_OK;
 }
