@@ -241,10 +241,10 @@ def run_adjoint(args):
     warm.gradients(warm.f0.clone(), 1.0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    J, lam, cps, Nn = prob.run(f0, 1.0)
+    J, lam, cps, Nn, order = prob.run(f0, 1.0)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    g = prob.adjoint_sweep(lam, cps, Nn)
+    g = prob.adjoint_sweep(lam, cps, Nn, order)
     dratio = float((prob.lam_nbar * prob.Nn_base[1]).sum().item())
     torch.cuda.synchronize()
     t2 = time.perf_counter()

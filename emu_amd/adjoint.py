@@ -44,13 +44,16 @@ class AdjointProblem:
     """Fixed-order particle data and scratch for forward/adjoint sweeps."""
 
     def __init__(self, emu, parts, dt: float, nsteps: int, checkpoint_every: int | None = None,
-                 segment: str = "store", memory_budget: float | None = None):
+                 segment: str = "store", memory_budget: float | None = None, resort_every: int = 4):
         self.emu = emu
         self.np = parts.np
         self.nh = emu.nh
         self.dt = float(dt)
         self.nsteps = int(nsteps)
         self.cp = int(checkpoint_every or max(1, int(math.sqrt(max(nsteps, 1)))))
+        # the particle arrays go back to cell order every `resort` steps, and at
+        # every checkpoint (so resort divides cp)
+        self.resort = math.gcd(self.cp, max(1, int(resort_every)))
         if segment not in ("store", "recompute"):
             raise ValueError("segment is 'store' or 'recompute'")
         self.segment = segment
@@ -80,14 +83,14 @@ class AdjointProblem:
         positions), the segment's step states (segment='store'), the adjoint
         and its next value, the particle constants and the work arrays."""
         nh, np_ = self.nh, self.np
-        state = 8.0 * np_ * (2 * nh + 3)
+        state = 8.0 * np_ * (2 * nh + 3) + 4.0 * np_  # flavour state, positions, cell order
         ncp = -(-self.nsteps // self.cp)
         held = ncp + (self.cp if self.segment == "store" else 1)
         ncell = 1
         for n in self.emu.parms.ncell:
             ncell *= n
         work = 8.0 * (5 * ncell * 4 * nh + 11 * nh * np_ + 2 * np_) + 8.0 * (3 * ncell * 4 * nh + nh * np_ + np_)
-        consts = 8.0 * np_ * (3 + 4 + 2 + 2 + 1 + 2 * nh)
+        consts = 8.0 * np_ * (3 + 4 + 2 + 2 + 1 + 2 * nh + 8)  # and the segment's reordered pup, Nn, Ll
         return held * state + 2 * 8.0 * 2 * nh * np_ + consts + work
 
     # ---- kernels ---------------------------------------------------------
@@ -149,11 +152,13 @@ class AdjointProblem:
         self.advance_positions(pos)
         return acc
 
-    def step(self, pos, Nn, y):
+    def step(self, pos, Nn, y, consts=None):
         """y_{n+1} from y_n; pos advanced in place; Renormalize's status is
-        checked by check_status()."""
+        checked by check_status().  consts = (pup, Ll) in the arrays' order
+        (the problem's own order when None)."""
+        pup, Ll = consts if consts is not None else (self.pup, self.Ll)
         out = torch.empty_like(y)
-        check(lib().emu_adj_step(self.emu.ctx, self.np, _p(pos), _p(self.pup), _p(Nn), _p(self.Ll), _p(y), _p(out),
+        check(lib().emu_adj_step(self.emu.ctx, self.np, _p(pos), _p(pup), _p(Nn), _p(Ll), _p(y), _p(out),
                                  self.dt, _p(self.step_work)), "emu_adj_step", self.emu.ctx)
         return out
 
@@ -162,16 +167,72 @@ class AdjointProblem:
         code = lib().emu_adj_status(self.emu.ctx, ctypes.byref(bad))
         check(code, "emu_adj_step (deferred Renormalize)", self.emu.ctx, bad.value)
 
-    def step_adjoint(self, pos, Nn, y, lam):
+    def step_adjoint(self, pos, Nn, y, lam, consts=None):
         """lam_n from lam_{n+1} for the step that started at (pos, y):
         lambda_k4 = dt/6 lam, lambda_k3 = dt/3 lam + dt lY4, lambda_k2 =
         dt/3 lam + dt/2 lY3, lambda_k1 = dt/6 lam + dt/2 lY2, lam_n = lam +
-        sum_i lY_i (in emu_adj_step_vjp)."""
+        sum_i lY_i (in emu_adj_step_vjp).  lam_nbar accumulates in the
+        arrays' order."""
+        pup = consts[0] if consts is not None else self.pup
         out = torch.empty_like(lam)
-        check(lib().emu_adj_step_vjp(self.emu.ctx, self.np, _p(pos), _p(self.pup), _p(Nn), _p(y), _p(lam), _p(out),
+        check(lib().emu_adj_step_vjp(self.emu.ctx, self.np, _p(pos), _p(pup), _p(Nn), _p(y), _p(lam), _p(out),
                                      _p(self.lam_nbar), self.dt, _p(self.step_work)), "emu_adj_step_vjp",
               self.emu.ctx)
         return out
+
+    # ---- cell order ------------------------------------------------------
+    # The step calls read every particle's arrays through the cell list; as
+    # the particles drift from the order they started in, those reads
+    # scatter (a 200-step gradient ran at 0.45x the rate of a 20-step one).
+    # So every `resort` steps the particle arrays go back to cell order.
+    # `order` maps an array order to the problem's: element i of an array in
+    # that order is particle order[i].  An adjoint carries the order it is in
+    # and is moved into the order of each step it goes back through.
+    def _permute(self, x, idx, mode, out=None):
+        rows = x.numel() // self.np if self.np else 0
+        if out is None:
+            out = torch.empty_like(x) if mode == 0 else torch.zeros_like(x)
+        check(lib().emu_adj_permute(self.emu.ctx, rows, self.np, _p(x), _p(idx), _p(out), mode), "emu_adj_permute",
+              self.emu.ctx)
+        return out
+
+    def _cell_order(self, pos):
+        o = torch.empty(self.np, dtype=torch.int32, device=pos.device)
+        check(lib().emu_adj_cell_order(self.emu.ctx, self.np, _p(pos), _p(o), _p(self.step_work)),
+              "emu_adj_cell_order", self.emu.ctx)
+        return o
+
+    def _segment_consts(self, order, Nn):
+        """(pup, Nn, Ll) in the given order."""
+        return (self._permute(self.pup, order, 0), self._permute(Nn, order, 0), self._permute(self.Ll, order, 0))
+
+    def _resort(self, pos, y, order, Nn):
+        """The arrays in cell order: (pos, y, order, consts)."""
+        o = self._cell_order(pos)
+        pos, y = self._permute(pos, o, 0), self._permute(y, o, 0)
+        order = order[o.long()]
+        return pos, y, order, self._segment_consts(order, Nn)
+
+    def _replay(self, cp, Nn, upto):
+        """The step-start states n0 .. upto of a checkpoint's segment, as the
+        forward run made them (re-sorting at the same steps): yields
+        (n, pos, y, order, consts); pos is advanced in place by the next
+        step, so a caller that keeps it clones it."""
+        n0, pos, y, order = cp
+        pos = pos.clone()
+        consts = self._segment_consts(order, Nn)
+        for n in range(n0, upto + 1):
+            if n > n0 and n % self.resort == 0:
+                pos, y, order, consts = self._resort(pos, y, order, Nn)
+            yield n, pos, y, order, consts
+            if n < upto:
+                y = self.step(pos, consts[1], y, (consts[0], consts[2]))
+
+    def _to_order(self, lam, lam_order, order):
+        """lam from lam_order into order."""
+        if order is lam_order:
+            return lam
+        return self._permute(self._permute(lam, lam_order, 1), order, 0)
 
     # ---- whole runs ------------------------------------------------------
     def Nn(self, ratio: float):
@@ -180,48 +241,62 @@ class AdjointProblem:
         return Nn
 
     def run(self, f0, ratio: float, keep_checkpoints=True):
+        """J after nsteps from f0 (problem order); with keep_checkpoints, also
+        dJ/dy_N and the checkpoints (n, pos, y, order) for the sweep, and
+        the order dJ/dy_N is in."""
         Nn = self.Nn(ratio)
         pos = self.pos0.clone()
         y = f0.clone()
+        order = torch.arange(self.np, dtype=torch.int32, device=y.device)
         cps = []
         for n in range(self.nsteps):
+            if n % self.resort == 0:
+                pos, y, order, consts = self._resort(pos, y, order, Nn)
             if keep_checkpoints and n % self.cp == 0:
-                cps.append((n, pos.clone(), y.clone()))
-            y = self.step(pos, Nn, y)
-        J, dJ = self.objective(pos, Nn, y, grad=keep_checkpoints)
+                cps.append((n, pos.clone(), y.clone(), order))
+            y = self.step(pos, consts[1], y, (consts[0], consts[2]))
+        J, dJ = self.objective(pos, consts[1], y, grad=keep_checkpoints)
         self.check_status()
-        return J, dJ, cps, Nn
+        return J, dJ, cps, Nn, order
 
-    def adjoint_sweep(self, lam, cps, Nn):
-        """Backward through the recorded run: lam = dJ/dy_N -> dJ/dy_0; the
-        nubar-weight adjoint accumulates in self.lam_nbar."""
+    def adjoint_sweep(self, lam, cps, Nn, lam_order):
+        """Backward through the recorded run: lam = dJ/dy_N (in lam_order) ->
+        dJ/dy_0 in the problem's order; the nubar-weight adjoint ends in
+        self.lam_nbar, in the problem's order too."""
+        lam_nbar = torch.zeros_like(self.lam_nbar)
         self.lam_nbar.zero_()
-        for (n0, pos0, y0) in reversed(cps):
+        nbar_order = lam_order
+
+        def back_through(n, pos, y, order, consts, lam, lam_order, nbar_order):
+            lam = self._to_order(lam, lam_order, order)
+            if order is not nbar_order:
+                self._permute(self.lam_nbar, nbar_order, 2, out=lam_nbar)
+                self.lam_nbar.zero_()
+            lam = self.step_adjoint(pos, consts[1], y, lam, (consts[0], consts[2]))
+            return lam, order, order
+
+        for cp in reversed(cps):
+            n0 = cp[0]
             n1 = min(n0 + self.cp, self.nsteps)
             if self.segment == "store":
-                states = []  # the segment's step-start states, recomputed
-                pos = pos0.clone()
-                y = y0.clone()
-                for n in range(n0, n1):
-                    states.append((pos.clone(), y.clone()))
-                    if n + 1 < n1:
-                        y = self.step(pos, Nn, y)
-                for (pos_n, y_n) in reversed(states):
-                    lam = self.step_adjoint(pos_n, Nn, y_n, lam)
+                states = [(n, pos.clone(), y, order, consts) for (n, pos, y, order, consts)
+                          in self._replay(cp, Nn, n1 - 1)]
+                for st in reversed(states):
+                    lam, lam_order, nbar_order = back_through(*st, lam, lam_order, nbar_order)
             else:
-                for n in range(n1 - 1, n0 - 1, -1):  # each step's start, from the checkpoint
-                    pos = pos0.clone()
-                    y = y0
-                    for _ in range(n0, n):
-                        y = self.step(pos, Nn, y)
-                    lam = self.step_adjoint(pos, Nn, y, lam)
+                for n in range(n1 - 1, n0 - 1, -1):  # each step's start, replayed from the checkpoint
+                    for st in self._replay(cp, Nn, n):
+                        pass
+                    lam, lam_order, nbar_order = back_through(*st, lam, lam_order, nbar_order)
+        self._permute(self.lam_nbar, nbar_order, 2, out=lam_nbar)
         self.check_status()
-        return lam
+        self.lam_nbar.copy_(lam_nbar)
+        return self._permute(lam, lam_order, 1)
 
     def gradients(self, f0, ratio: float):
         """J, dJ/df0 and dJ/dratio by the adjoint sweep."""
-        J, lam, cps, Nn = self.run(f0, ratio)
-        lam = self.adjoint_sweep(lam, cps, Nn)
+        J, lam, cps, Nn, order = self.run(f0, ratio)
+        lam = self.adjoint_sweep(lam, cps, Nn, order)
         dratio = float((self.lam_nbar * self.Nn_base[1]).sum().item())
         return J, lam, dratio
 
@@ -232,16 +307,16 @@ class FlavorObjective(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, f0, ratio, problem: AdjointProblem):
-        J, lam, cps, Nn = problem.run(f0.detach(), float(ratio))
+        J, lam, cps, Nn, order = problem.run(f0.detach(), float(ratio))
         ctx.problem = problem
-        ctx.saved = (lam, cps, Nn)
+        ctx.saved = (lam, cps, Nn, order)
         return torch.tensor(J, dtype=torch.float64, device=f0.device)
 
     @staticmethod
     def backward(ctx, gJ):
         p = ctx.problem
-        lam, cps, Nn = ctx.saved
-        grad_f0 = p.adjoint_sweep(lam, cps, Nn)
+        lam, cps, Nn, order = ctx.saved
+        grad_f0 = p.adjoint_sweep(lam, cps, Nn, order)
         grad_ratio = (p.lam_nbar * p.Nn_base[1]).sum()
         ctx.saved = None
         return gJ * grad_f0, gJ * grad_ratio, None

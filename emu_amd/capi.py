@@ -94,6 +94,8 @@ _SIGS = [
     ("emu_adj_step", ctypes.c_int, [_P, ctypes.c_int64, _P, _P, _P, _P, _P, _P, ctypes.c_double, _P]),
     ("emu_adj_step_vjp", ctypes.c_int, [_P, ctypes.c_int64, _P, _P, _P, _P, _P, _P, _P, ctypes.c_double, _P]),
     ("emu_adj_status", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64)]),
+    ("emu_adj_cell_order", ctypes.c_int, [_P, ctypes.c_int64, _P, _P, _P]),
+    ("emu_adj_permute", ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int64, _P, _P, _P, ctypes.c_int]),
     ("emu_init_count", ctypes.c_int64, [ctypes.POINTER(EmuParams), ctypes.POINTER(EmuInitConfig)]),
     ("emu_init_particles", ctypes.c_int, [_P, ctypes.POINTER(EmuInitConfig), ctypes.POINTER(EmuParticles), _P]),
 ]

@@ -1127,6 +1127,51 @@ extern "C" int emu_adj_step_vjp(emu_ctx* ctx, int64_t np, const double* pos, con
     return EMU_OK;
 }
 
+namespace {
+// out[r][i] = in[r][idx[i]] (gather) or out[r][idx[i]] = in[r][i] (scatter;
+// accumulate adds), rows of np doubles
+__global__ void k_permute_rows(int rows, int64_t np, const double* in, const int* idx, double* out, int mode)
+{
+    const int64_t n = (int64_t)rows * np;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = t / np, i = t - r * np;
+        const int64_t j = idx[i];
+        if (mode == 0) out[t] = in[r * np + j];
+        else if (mode == 1) out[r * np + j] = in[t];
+        else out[r * np + j] += in[t];
+    }
+}
+
+__global__ void k_copy_ints(int64_t n, const int* in, int* out)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+        out[t] = in[t];
+}
+} // namespace
+
+extern "C" int emu_adj_cell_order(emu_ctx* ctx, int64_t np, const double* pos, int* order, double* work)
+{
+    if (!ctx || np < 0 || !work || (np > 0 && (!pos || !order))) return EMU_ERR_ARG;
+    if (np == 0) return EMU_OK;
+    const StepWork w = step_work(ctx, np, work);
+    const AdjIn in{np, pos, nullptr, nullptr, nullptr};
+    if (int e = cell_sort(ctx, in, w.cs)) return e;
+    hipLaunchKernelGGL(k_copy_ints, dim3(nblk(np)), dim3(kB), 0, ctx->stream, np, w.cs.perm, order);
+    EMU_LAUNCH_CHECK(ctx, "adj_cell_order");
+    return EMU_OK;
+}
+
+extern "C" int emu_adj_permute(emu_ctx* ctx, int rows, int64_t np, const double* in, const int* idx, double* out,
+                               int mode)
+{
+    if (!ctx || rows < 0 || np < 0 || mode < 0 || mode > 2 || in == out) return EMU_ERR_ARG;
+    if (rows == 0 || np == 0) return EMU_OK;
+    hipLaunchKernelGGL(k_permute_rows, dim3(nblk((int64_t)rows * np)), dim3(kB), 0, ctx->stream, rows, np, in, idx,
+                       out, mode);
+    EMU_LAUNCH_CHECK(ctx, "adj_permute");
+    return EMU_OK;
+}
+
 extern "C" int emu_adj_status(emu_ctx* ctx, int64_t* bad_index)
 {
     if (!ctx) return EMU_ERR_ARG;

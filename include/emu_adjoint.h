@@ -66,6 +66,17 @@ int emu_adj_step_vjp(emu_ctx* ctx, int64_t np, const double* pos, const double* 
                      const double* y_in, const double* lam_in, double* lam_out, double* lam_Nbar, double dt,
                      double* work);
 
+/* The particles listed by cell (the cell of their stencil's centre) at
+ * positions pos: order[i] is the i-th particle in that order (within a cell,
+ * in no set order).  The step calls run fastest on arrays kept in this
+ * order: their particle streams are then read cell by cell.  Asynchronous. */
+int emu_adj_cell_order(emu_ctx* ctx, int64_t np, const double* pos, int* order, double* work);
+
+/* Rows of np doubles through a permutation: mode 0 gathers out[r][i] =
+ * in[r][idx[i]], mode 1 scatters out[r][idx[i]] = in[r][i], mode 2 adds the
+ * scatter into out.  out must not alias in.  Asynchronous. */
+int emu_adj_permute(emu_ctx* ctx, int rows, int64_t np, const double* in, const int* idx, double* out, int mode);
+
 /* The first Renormalize error of the emu_adj_step calls since the previous
  * emu_adj_status (then cleared), with the lowest particle index in
  * *bad_index.  Synchronous. */

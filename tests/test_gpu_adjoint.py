@@ -104,13 +104,13 @@ def test_adjoint_gradient_matches_finite_differences(mods, oracle, nf):
         fp[comp, p] += h
         fm = prob.f0.clone()
         fm[comp, p] -= h
-        Jp, _, _, _ = prob.run(fp, 1.0, keep_checkpoints=False)
-        Jm, _, _, _ = prob.run(fm, 1.0, keep_checkpoints=False)
+        Jp, _, _, _, _ = prob.run(fp, 1.0, keep_checkpoints=False)
+        Jm, _, _, _, _ = prob.run(fm, 1.0, keep_checkpoints=False)
         fd = (Jp - Jm) / (2 * h)
         assert float(g[comp, p]) == pytest.approx(fd, rel=1e-5, abs=1e-5 * abs(fd) + 1e-300), (comp, p)
     h = 1e-4
-    Jp, _, _, _ = prob.run(prob.f0, 1.0 + h, keep_checkpoints=False)
-    Jm, _, _, _ = prob.run(prob.f0, 1.0 - h, keep_checkpoints=False)
+    Jp, _, _, _, _ = prob.run(prob.f0, 1.0 + h, keep_checkpoints=False)
+    Jm, _, _, _, _ = prob.run(prob.f0, 1.0 - h, keep_checkpoints=False)
     fd = (Jp - Jm) / (2 * h)
     assert float(ratio.grad) == pytest.approx(fd, rel=1e-5)
 
@@ -169,8 +169,8 @@ def test_adjoint32_gradient_matches_finite_differences(mods):
     prob = adjoint.AdjointProblem(emu, c, dt=dt, nsteps=200, checkpoint_every=20)
     J0, gf, gr = prob.gradients(prob.f0.clone(), 1.0)
     h = 1e-4
-    Jp, _, _, _ = prob.run(prob.f0, 1.0 + h, keep_checkpoints=False)
-    Jm, _, _, _ = prob.run(prob.f0, 1.0 - h, keep_checkpoints=False)
+    Jp, _, _, _, _ = prob.run(prob.f0, 1.0 + h, keep_checkpoints=False)
+    Jm, _, _, _, _ = prob.run(prob.f0, 1.0 - h, keep_checkpoints=False)
     fd = (Jp - Jm) / (2 * h)
     print(f"adjoint32: J {J0:.6e}, dJ/dratio adjoint {gr:.10e} fd {fd:.10e}")
     assert gr == pytest.approx(fd, rel=1e-5)
@@ -183,8 +183,8 @@ def test_adjoint32_gradient_matches_finite_differences(mods):
         # the off-diagonal components the perturbation lives in, nu and nubar
         for comp in (i01, i01 + 1, nh + i01, nh + i01 + 1):
             v[comp] = torch.randn(c.np, generator=gen, dtype=torch.float64).to(v.device) * 1e-6
-        Jp, _, _, _ = prob.run(prob.f0 + h * v, 1.0, keep_checkpoints=False)
-        Jm, _, _, _ = prob.run(prob.f0 - h * v, 1.0, keep_checkpoints=False)
+        Jp, _, _, _, _ = prob.run(prob.f0 + h * v, 1.0, keep_checkpoints=False)
+        Jm, _, _, _, _ = prob.run(prob.f0 - h * v, 1.0, keep_checkpoints=False)
         fd = (Jp - Jm) / (2 * h)
         ad = float((gf * v).sum())
         print(f"adjoint32: direction {trial}: adjoint {ad:.10e} fd {fd:.10e}")
