@@ -9,16 +9,19 @@ libemu_amd.so (include/emu_adjoint.h): the backward of the deposit is a
 gather, the backward of the gather is a scatter, and the commutator has its
 own adjoint.  Forward states are kept at checkpoints (every
 ``checkpoint_every`` steps; ``sqrt(nsteps)`` by default) and recomputed
-segment by segment, either keeping the segment's step states
-(``segment="store"``: (nsteps/cp + cp) states in memory) or recomputing each
-step from the segment's start (``segment="recompute"``: nsteps/cp + 1 states,
-about cp/2 more forward steps per step).  The footprint is checked against
-``memory_budget`` bytes before anything runs.
+segment by segment, either keeping the segment's step states with each
+step's stage states and deposits (``segment="keep"``, the default: the
+reverse step then recomputes nothing), keeping the step states only
+(``segment="store"``: (nsteps/cp + cp) states in memory; each reverse step
+recomputes its stages) or recomputing each step from the segment's start
+(``segment="recompute"``: nsteps/cp + 1 states, about cp/2 more forward
+steps per step).  The footprint is checked against ``memory_budget`` bytes
+before anything runs.
 
 Each forward step is one library call (emu_adj_step: one cell list, the four
 stages with their deposits, Renormalize with a deferred status, the position
 update) and each reverse step another (emu_adj_step_vjp: the stage states
-recomputed, then the four stage VJPs).
+recomputed, or taken from the kept ones, then the four stage VJPs).
 
 Conventions: dt is held fixed over the run (compute_dt's maximum is not
 differentiated); Renormalize is applied in the forward and passed straight
@@ -44,7 +47,7 @@ class AdjointProblem:
     """Fixed-order particle data and scratch for forward/adjoint sweeps."""
 
     def __init__(self, emu, parts, dt: float, nsteps: int, checkpoint_every: int | None = None,
-                 segment: str = "store", memory_budget: float | None = None, resort_every: int = 4):
+                 segment: str = "keep", memory_budget: float | None = None, resort_every: int = 4):
         self.emu = emu
         self.np = parts.np
         self.nh = emu.nh
@@ -54,9 +57,11 @@ class AdjointProblem:
         # the particle arrays go back to cell order every `resort` steps, and at
         # every checkpoint (so resort divides cp)
         self.resort = math.gcd(self.cp, max(1, int(resort_every)))
-        if segment not in ("store", "recompute"):
-            raise ValueError("segment is 'store' or 'recompute'")
+        if segment not in ("keep", "store", "recompute"):
+            raise ValueError("segment is 'keep', 'store' or 'recompute'")
         self.segment = segment
+        self.keep_doubles = int(lib().emu_adj_step_keep_doubles(emu.ctx, self.np))
+        self._keeps = None
         need = self.footprint_bytes()
         if memory_budget is not None and need > memory_budget:
             from .capi import EmuError
@@ -85,13 +90,14 @@ class AdjointProblem:
         nh, np_ = self.nh, self.np
         state = 8.0 * np_ * (2 * nh + 3) + 4.0 * np_  # flavour state, positions, cell order
         ncp = -(-self.nsteps // self.cp)
-        held = ncp + (self.cp if self.segment == "store" else 1)
+        held = ncp + (1 if self.segment == "recompute" else self.cp)
+        kept = 8.0 * self.cp * self.keep_doubles if self.segment == "keep" else 0.0
         ncell = 1
         for n in self.emu.parms.ncell:
             ncell *= n
         work = 8.0 * (5 * ncell * 4 * nh + 11 * nh * np_ + 2 * np_) + 8.0 * (3 * ncell * 4 * nh + nh * np_ + np_)
         consts = 8.0 * np_ * (3 + 4 + 2 + 2 + 1 + 2 * nh + 8)  # and the segment's reordered pup, Nn, Ll
-        return held * state + 2 * 8.0 * 2 * nh * np_ + consts + work
+        return held * state + kept + 2 * 8.0 * 2 * nh * np_ + consts + work
 
     # ---- kernels ---------------------------------------------------------
     def rhs(self, pos, Nn, Y):
@@ -152,14 +158,20 @@ class AdjointProblem:
         self.advance_positions(pos)
         return acc
 
-    def step(self, pos, Nn, y, consts=None):
+    def step(self, pos, Nn, y, consts=None, keep=None):
         """y_{n+1} from y_n; pos advanced in place; Renormalize's status is
         checked by check_status().  consts = (pup, Ll) in the arrays' order
-        (the problem's own order when None)."""
+        (the problem's own order when None).  With keep (keep_doubles
+        doubles), the step's stage states and deposits stay there for
+        step_adjoint."""
         pup, Ll = consts if consts is not None else (self.pup, self.Ll)
         out = torch.empty_like(y)
-        check(lib().emu_adj_step(self.emu.ctx, self.np, _p(pos), _p(pup), _p(Nn), _p(Ll), _p(y), _p(out),
-                                 self.dt, _p(self.step_work)), "emu_adj_step", self.emu.ctx)
+        if keep is None:
+            check(lib().emu_adj_step(self.emu.ctx, self.np, _p(pos), _p(pup), _p(Nn), _p(Ll), _p(y), _p(out),
+                                     self.dt, _p(self.step_work)), "emu_adj_step", self.emu.ctx)
+        else:
+            check(lib().emu_adj_step_keep(self.emu.ctx, self.np, _p(pos), _p(pup), _p(Nn), _p(Ll), _p(y), _p(out),
+                                          self.dt, _p(self.step_work), _p(keep)), "emu_adj_step_keep", self.emu.ctx)
         return out
 
     def check_status(self):
@@ -167,17 +179,23 @@ class AdjointProblem:
         code = lib().emu_adj_status(self.emu.ctx, ctypes.byref(bad))
         check(code, "emu_adj_step (deferred Renormalize)", self.emu.ctx, bad.value)
 
-    def step_adjoint(self, pos, Nn, y, lam, consts=None):
+    def step_adjoint(self, pos, Nn, y, lam, consts=None, keep=None):
         """lam_n from lam_{n+1} for the step that started at (pos, y):
         lambda_k4 = dt/6 lam, lambda_k3 = dt/3 lam + dt lY4, lambda_k2 =
         dt/3 lam + dt/2 lY3, lambda_k1 = dt/6 lam + dt/2 lY2, lam_n = lam +
         sum_i lY_i (in emu_adj_step_vjp).  lam_nbar accumulates in the
-        arrays' order."""
+        arrays' order.  keep: the step's kept stage data (step(..., keep)),
+        which spares the recompute."""
         pup = consts[0] if consts is not None else self.pup
         out = torch.empty_like(lam)
-        check(lib().emu_adj_step_vjp(self.emu.ctx, self.np, _p(pos), _p(pup), _p(Nn), _p(y), _p(lam), _p(out),
-                                     _p(self.lam_nbar), self.dt, _p(self.step_work)), "emu_adj_step_vjp",
-              self.emu.ctx)
+        if keep is None:
+            check(lib().emu_adj_step_vjp(self.emu.ctx, self.np, _p(pos), _p(pup), _p(Nn), _p(y), _p(lam), _p(out),
+                                         _p(self.lam_nbar), self.dt, _p(self.step_work)), "emu_adj_step_vjp",
+                  self.emu.ctx)
+        else:
+            check(lib().emu_adj_step_vjp_kept(self.emu.ctx, self.np, _p(pos), _p(pup), _p(Nn), _p(y), _p(keep),
+                                              _p(lam), _p(out), _p(self.lam_nbar), self.dt, _p(self.step_work)),
+                  "emu_adj_step_vjp_kept", self.emu.ctx)
         return out
 
     # ---- cell order ------------------------------------------------------
@@ -228,6 +246,26 @@ class AdjointProblem:
             if n < upto:
                 y = self.step(pos, consts[1], y, (consts[0], consts[2]))
 
+    def _replay_keep(self, cp, Nn, n1):
+        """The steps n0 .. n1-1 of a checkpoint's segment run again (re-sorting
+        at the same steps as the forward run), each keeping its stage data:
+        a list of (n, pos, y, order, consts, keep) at the steps' starts."""
+        n0, pos, y, order = cp
+        pos = pos.clone()
+        consts = self._segment_consts(order, Nn)
+        if self._keeps is None:
+            self._keeps = [torch.empty(self.keep_doubles, dtype=torch.float64, device=pos.device)
+                           for _ in range(self.cp)]
+        out = []
+        for i, n in enumerate(range(n0, n1)):
+            if n > n0 and n % self.resort == 0:
+                pos, y, order, consts = self._resort(pos, y, order, Nn)
+            start = pos.clone()
+            y_next = self.step(pos, consts[1], y, (consts[0], consts[2]), keep=self._keeps[i])
+            out.append((n, start, y, order, consts, self._keeps[i]))
+            y = y_next
+        return out
+
     def _to_order(self, lam, lam_order, order):
         """lam from lam_order into order."""
         if order is lam_order:
@@ -267,18 +305,22 @@ class AdjointProblem:
         self.lam_nbar.zero_()
         nbar_order = lam_order
 
-        def back_through(n, pos, y, order, consts, lam, lam_order, nbar_order):
+        def back_through(n, pos, y, order, consts, lam, lam_order, nbar_order, keep=None):
             lam = self._to_order(lam, lam_order, order)
             if order is not nbar_order:
                 self._permute(self.lam_nbar, nbar_order, 2, out=lam_nbar)
                 self.lam_nbar.zero_()
-            lam = self.step_adjoint(pos, consts[1], y, lam, (consts[0], consts[2]))
+            lam = self.step_adjoint(pos, consts[1], y, lam, (consts[0], consts[2]), keep=keep)
             return lam, order, order
 
         for cp in reversed(cps):
             n0 = cp[0]
             n1 = min(n0 + self.cp, self.nsteps)
-            if self.segment == "store":
+            if self.segment == "keep":
+                for (n, pos, y, order, consts, keep) in reversed(self._replay_keep(cp, Nn, n1)):
+                    lam, lam_order, nbar_order = back_through(n, pos, y, order, consts, lam, lam_order, nbar_order,
+                                                              keep)
+            elif self.segment == "store":
                 states = [(n, pos.clone(), y, order, consts) for (n, pos, y, order, consts)
                           in self._replay(cp, Nn, n1 - 1)]
                 for st in reversed(states):

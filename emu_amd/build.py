@@ -39,7 +39,9 @@ SOURCES = {
     "emu_rk.hip": ["-ffp-contract=off"],
     "emu_state.hip": [],
     "emu_init.hip": [],
-    "emu_adjoint.hip": [],
+    # MFMA accumulators in VGPRs: in the adjoint deposit's per-key loop the
+    # default AGPR form copies them out and back at every k-step
+    "emu_adjoint.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"],
 }
 
 

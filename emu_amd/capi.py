@@ -75,6 +75,7 @@ _SIGS = [
     ("emu_state_create", ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int, ctypes.POINTER(_P)]),
     ("emu_state_destroy", ctypes.c_int, [_P]),
     ("emu_state_load", ctypes.c_int, [_P, ctypes.POINTER(EmuParticles), _P]),
+    ("emu_state_set_vvac_max", ctypes.c_int, [_P, ctypes.c_double]),
     ("emu_state_store", ctypes.c_int, [_P, ctypes.POINTER(EmuParticles), _P]),
     ("emu_state_evolve", ctypes.c_int, [_P, ctypes.c_int, ctypes.c_double]),
     ("emu_state_info", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
@@ -93,6 +94,10 @@ _SIGS = [
     ("emu_adj_step_work_doubles", ctypes.c_int64, [_P, ctypes.c_int64]),
     ("emu_adj_step", ctypes.c_int, [_P, ctypes.c_int64, _P, _P, _P, _P, _P, _P, ctypes.c_double, _P]),
     ("emu_adj_step_vjp", ctypes.c_int, [_P, ctypes.c_int64, _P, _P, _P, _P, _P, _P, _P, ctypes.c_double, _P]),
+    ("emu_adj_step_keep_doubles", ctypes.c_int64, [_P, ctypes.c_int64]),
+    ("emu_adj_step_keep", ctypes.c_int, [_P, ctypes.c_int64, _P, _P, _P, _P, _P, _P, ctypes.c_double, _P, _P]),
+    ("emu_adj_step_vjp_kept", ctypes.c_int, [_P, ctypes.c_int64, _P, _P, _P, _P, _P, _P, _P, _P, ctypes.c_double,
+                                             _P]),
     ("emu_adj_status", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64)]),
     ("emu_adj_cell_order", ctypes.c_int, [_P, ctypes.c_int64, _P, _P, _P]),
     ("emu_adj_permute", ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int64, _P, _P, _P, ctypes.c_int]),

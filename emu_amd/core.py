@@ -322,6 +322,13 @@ class FusedState:
         check(lib().emu_state_load(self.st, ctypes.byref(v), ctypes.c_void_p(c.ids.data_ptr() if c.np else None)),
               "emu_state_load", self.emu.ctx)
 
+    def set_vvac_max(self, vvac_max: float):
+        """Replace the Vvac_max the load computed and recompute the starting dt
+        (before the first evolve).  A run restarted from a checkpoint passes 0:
+        the reference sets Vvac_max only when it initialises particles
+        (FlavoredNeutrinoContainerInit.cpp:657-660)."""
+        check(lib().emu_state_set_vvac_max(self.st, float(vvac_max)), "emu_state_set_vvac_max", self.emu.ctx)
+
     def store(self, c: FlavoredNeutrinoContainer):
         v = c.view()
         check(lib().emu_state_store(self.st, ctypes.byref(v), ctypes.c_void_p(c.ids.data_ptr() if c.np else None)),

@@ -730,6 +730,9 @@ EvolveResult evolve_flavor(const TestParams* parms, const EvolveOptions& opt)
             ~Closer() { emu_state_destroy(s); }
         } closer{st};
         emu_check(emu_state_load(st, neutrinos_old.view(), neutrinos_old.ids()), "emu_state_load", geom.ctx());
+        if (parms->do_restart)
+            emu_check(emu_state_set_vvac_max(st, FlavoredNeutrinoContainer::Vvac_max), "emu_state_set_vvac_max",
+                      geom.ctx());
         if (!opt.quiet) std::cout << "Starting timestepping loop... " << std::endl;
         double time = 0, dt = 0;
         int64_t steps = 0, bad = -1;

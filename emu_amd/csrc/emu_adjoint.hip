@@ -4,7 +4,7 @@
 // Particles stay in a fixed order (gradients are per particle).  The gathers
 // take one particle per thread with its own stencil; the deposits sum over
 // the particles that share a stencil on the matrix cores first
-// (k_seg_deposit), then add to the grid with FP64 atomics.  All arrays are
+// (k_cell_deposit), then add to the grid with FP64 atomics.  All arrays are
 // SoA: [component][np].
 //
 // Forward RHS for stage state Y (positions frozen within the step):
@@ -188,10 +188,29 @@ __global__ __launch_bounds__(256) void k_vjp_comm(AdjIn in, Geom g, const int* p
 }
 
 // The centre cell of each particle's stencil, and the particle count per cell.
+// The lanes of a wave holding one key add to its counter once (particles in
+// cell order give one or two keys per wave: per-lane atomics on one counter
+// would serialise).
+template <typename F> __device__ __forceinline__ void for_each_wave_key(bool act, int key, F&& f)
+{
+    unsigned long long left = __ballot(act);
+    while (left) {
+        const int lead = __ffsll((long long)left) - 1;
+        const int k = __shfl(key, lead);
+        const unsigned long long m = __ballot(act && key == k);
+        f(k, m, lead);
+        left &= ~m;
+    }
+}
+
 __global__ void k_cell_keys(AdjIn in, Geom g, int* key, int* counts)
 {
     const int64_t np = in.np;
-    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const bool act = p < np;
+    const int lane = threadIdx.x & 63;
+    int k = 0;
+    if (act) {
         int c[3];
 #pragma unroll
         for (int d = 0; d < 3; d++) {
@@ -200,152 +219,29 @@ __global__ void k_cell_keys(AdjIn in, Geom g, int* key, int* counts)
             shape3((in.pos[d * np + p] - g.plo[d]) * g.dxi[d], g.order[d], idx, w);
             c[d] = pmod(idx, g.n[d]);
         }
-        const int k = (c[2] * g.n[1] + c[1]) * g.n[0] + c[0];
+        k = (c[2] * g.n[1] + c[1]) * g.n[0] + c[0];
         key[p] = k;
-        atomicAdd(&counts[k], 1);
     }
+    for_each_wave_key(act, k, [&](int kk, unsigned long long m, int lead) {
+        if (lane == lead) atomicAdd(&counts[kk], __popcll(m));
+    });
 }
 
-// perm[start[key] + i] = p: the particles listed cell by cell (the order within
-// a cell follows the atomics and does not matter to the deposit's sums beyond
-// rounding).
+// perm[start[key] + i] = p: the particles listed cell by cell; within a cell in
+// lane order within a wave, waves in the order their atomics land (which does
+// not matter to the deposit's sums beyond rounding).
 __global__ void k_cell_perm(int64_t np, const int* key, const int* start, int* cursor, int* perm)
 {
-    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
-        const int k = key[p];
-        perm[start[k] + atomicAdd(&cursor[k], 1)] = (int)p;
-    }
-}
-
-// The deposit of both directions with the sum over particles on the matrix
-// cores.  KIND 0: the forward q_c = N f_c - conj(Nbar fbar_c) with (1, v);
-// KIND 1: the gather's adjoint mu_c with (1, -v).  Per-particle atomics put
-// every particle of a cell on the same 27 x 4 NH grid entries, so the sum
-// over a cell's particles is done first, as in the fused stage kernel.
-//
-// The particles are taken through `perm`, which lists them by the centre cell
-// of their stencil (the key; k_cell_keys and k_cell_perm), 64 per wave.
-// Particles with one key share their 27 targets; for each distinct key in the
-// wave, D[t][q] = sum_p W[p][t] R[p][q] runs on f64 16x16x4 MFMAs over the
-// k-steps that span the key's lanes, with the other lanes' sources zeroed,
-// and is then added to the grid once.  With the cell order a wave holds one
-// or two keys; any order gives the same sums.
-template <int NF, int WX, int WY, int WZ, int KIND>
-__global__ __launch_bounds__(128) void k_seg_deposit(AdjIn in, Geom g, const int* perm, const double* mu, double* grid)
-{
-    constexpr int nh = NH<NF>, QG = 4 * nh;
-    constexpr int NT = WX * WY * WZ, MT = (NT + 15) / 16, NTL = (QG + 15) / 16;
-    constexpr int NSR = nh + 4;            // q (or mu), v (or -v), 1
-    constexpr int REG = 64 * 9 + 64 * NSR + 32; // doubles per wave: weights, sources, keys
-    __shared__ double lds[2 * REG];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    double* Wf = lds + wave * REG;
-    double* Sr = Wf + 64 * 9;
-    int* Kc = (int*)(Sr + 64 * NSR);
-    const int64_t np = in.np;
-    const int64_t p0 = ((int64_t)blockIdx.x * 2 + wave) * 64;
-    if (p0 >= np) return; // uniform over the wave
-    const int64_t p = p0 + lane;
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     const bool act = p < np;
-    const int64_t pl = perm[act ? p : p0]; // the particles in cell order
-    const double pos[3] = {in.pos[pl], in.pos[np + pl], in.pos[2 * np + pl]};
-    const double pt = in.pup[3 * np + pl];
-    const double v[3] = {in.pup[pl] / pt, in.pup[np + pl] / pt, in.pup[2 * np + pl] / pt};
-    int idx[3];
-    double wd[3][3];
-#pragma unroll
-    for (int d = 0; d < 3; d++) shape3((pos[d] - g.plo[d]) * g.dxi[d], g.order[d], idx[d], wd[d]);
-#pragma unroll
-    for (int d = 0; d < 3; d++)
-#pragma unroll
-        for (int e = 0; e < 3; e++) Wf[lane * 9 + 3 * d + e] = wd[d][e];
-    if constexpr (KIND == 0) {
-        const double N = in.Nn[pl], Nb = in.Nn[np + pl];
-#pragma unroll
-        for (int c = 0; c < nh; c++) {
-            const double f = in.Y[(int64_t)c * np + pl], fb = in.Y[(int64_t)(nh + c) * np + pl];
-            Sr[lane * NSR + c] = act ? (is_im_comp<NF>(c) ? N * f + Nb * fb : N * f - Nb * fb) : 0.0;
-        }
-    } else {
-#pragma unroll
-        for (int c = 0; c < nh; c++) Sr[lane * NSR + c] = act ? mu[(int64_t)c * np + pl] : 0.0;
-    }
-#pragma unroll
-    for (int d = 0; d < 3; d++) Sr[lane * NSR + nh + d] = KIND == 0 ? v[d] : -v[d];
-    Sr[lane * NSR + nh + 3] = 1.0;
-    const int cx = pmod(idx[0], g.n[0]), cy = pmod(idx[1], g.n[1]), cz = pmod(idx[2], g.n[2]);
-    const int key = (cz * g.n[1] + cy) * g.n[0] + cx;
-    Kc[lane] = key;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    // operand lane constants, as in the stage kernel: row t = 16m + l%16 reads the
-    // shape factors of target t; column q = f nh + c is source c times (1, v)_f
-    int tdx[MT][3];
-#pragma unroll
-    for (int m = 0; m < MT; m++) {
-        const int t = 16 * m + (lane & 15);
-        const int tc = t < NT ? t : NT - 1;
-        const int ii = tc % WX, jj = (tc / WX) % WY, kk = tc / (WX * WY);
-        tdx[m][0] = WX == 1 ? 1 : ii;
-        tdx[m][1] = 3 + (WY == 1 ? 1 : jj);
-        tdx[m][2] = 6 + (WZ == 1 ? 1 : kk);
-    }
-    int bi1[NTL], bi2[NTL];
-#pragma unroll
-    for (int n = 0; n < NTL; n++) {
-        const int q = 16 * n + (lane & 15);
-        const int f = q / nh, c = q % nh;
-        bi1[n] = q < QG ? c : nh + 3;
-        bi2[n] = q < QG ? (f == 0 ? nh + 3 : nh + f - 1) : nh + 3;
-    }
-    unsigned long long rem = __ballot(act);
-    while (rem) { // uniform: one pass per distinct key in the wave
-        const int lead = __builtin_ctzll(rem);
-        const int key0 = __builtin_amdgcn_readlane(key, lead);
-        const unsigned long long seg = __ballot(act && key == key0);
-        rem &= ~seg;
-        const int lo = __builtin_ctzll(seg), hi = 63 - __builtin_clzll(seg);
-        mf_d4 acc[MT][NTL];
-#pragma unroll
-        for (int m = 0; m < MT; m++)
-#pragma unroll
-            for (int n = 0; n < NTL; n++) acc[m][n] = mf_d4{0.0, 0.0, 0.0, 0.0};
-        for (int ks = lo >> 2; ks <= (hi >> 2); ks++) {
-            const int pp = 4 * ks + (lane >> 4);
-            const double in_seg = ((seg >> pp) & 1ull) ? 1.0 : 0.0;
-            const double* wf = Wf + pp * 9;
-            const double* sr = Sr + pp * NSR;
-            double A[MT];
-#pragma unroll
-            for (int m = 0; m < MT; m++) A[m] = wf[tdx[m][0]] * wf[tdx[m][1]] * wf[tdx[m][2]];
-#pragma unroll
-            for (int n = 0; n < NTL; n++) {
-                const double B = in_seg * (sr[bi1[n]] * sr[bi2[n]]);
-#pragma unroll
-                for (int m = 0; m < MT; m++) acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[m], B, acc[m][n], 0, 0, 0);
-            }
-        }
-        // element r of lane l holds D[16m + 4r + l/16][16n + l%16]
-        const int kx = key0 % g.n[0], ky = (key0 / g.n[0]) % g.n[1], kz = key0 / (g.n[0] * g.n[1]);
-#pragma unroll
-        for (int m = 0; m < MT; m++)
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int t = 16 * m + 4 * r + (lane >> 4);
-                if (t >= NT) continue;
-                const int ii = t % WX, jj = (t / WX) % WY, kk = t / (WX * WY);
-                const int tx = WX == 1 ? kx : pmod(kx + ii - 1, g.n[0]);
-                const int ty = WY == 1 ? ky : pmod(ky + jj - 1, g.n[1]);
-                const int tz = WZ == 1 ? kz : pmod(kz + kk - 1, g.n[2]);
-                double* G = grid + (((int64_t)tz * g.n[1] + ty) * g.n[0] + tx) * QG;
-#pragma unroll
-                for (int n = 0; n < NTL; n++) {
-                    const int q = 16 * n + (lane & 15);
-                    if (q < QG) unsafeAtomicAdd(&G[q], acc[m][n][r]);
-                }
-            }
-    }
+    const int lane = threadIdx.x & 63;
+    const int k = act ? key[p] : 0;
+    for_each_wave_key(act, k, [&](int kk, unsigned long long m, int lead) {
+        int base = 0;
+        if (lane == lead) base = atomicAdd(&cursor[kk], __popcll(m));
+        base = __shfl(base, lead);
+        if ((m >> lane) & 1ull) perm[start[kk] + base + __popcll(m & ((1ull << lane) - 1ull))] = (int)p;
+    });
 }
 
 // VJP part 3: gather of the grid adjoint (adjoint of the deposit).
@@ -381,17 +277,67 @@ __global__ __launch_bounds__(256) void k_vjp_gather(AdjIn in, Geom g, const int*
 }
 
 // Objective J = sum over cells of |N_01|^2 with N_01 = sum_p w N f01 (nu).
+// One thread per particle; the lanes of a wave whose stencils share a centre
+// cell (one or two such groups per wave in cell order) sum each target's
+// contribution across the wave before one atomic per target and component:
+// per-particle atomics would put every particle of a cell on the same 27
+// targets.
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
 template <int NF>
-__global__ void k_obj_deposit(AdjIn in, Geom g, double* g2)
+__global__ __launch_bounds__(256) void k_obj_deposit(AdjIn in, Geom g, double* g2)
 {
     const int64_t np = in.np;
-    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
-        const double pos[3] = {in.pos[p], in.pos[np + p], in.pos[2 * np + p]};
-        const double N = in.Nn[p];
-        const double re = N * in.Y[(int64_t)herm_re<NF>(0, 1) * np + p], im = N * in.Y[(int64_t)herm_im<NF>(0, 1) * np + p];
-        for_stencil(g, pos, [&](int64_t cell, double w) {
-            unsafeAtomicAdd(&g2[2 * cell], w * re);
-            unsafeAtomicAdd(&g2[2 * cell + 1], w * im);
+    const int lane = threadIdx.x & 63;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    // p0 is the wave's first particle, so the loop bound is the same for all its lanes
+    for (int64_t p0 = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~63); p0 < np; p0 += stride) {
+        const int64_t p = p0 + lane;
+        const bool act = p < np;
+        int c[3] = {0, 0, 0};
+        double wd[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+        double re = 0.0, im = 0.0;
+        if (act) {
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                int idx;
+                shape3((in.pos[d * np + p] - g.plo[d]) * g.dxi[d], g.order[d], idx, wd[d]);
+                c[d] = pmod(idx, g.n[d]);
+            }
+            const double N = in.Nn[p];
+            re = N * in.Y[(int64_t)herm_re<NF>(0, 1) * np + p];
+            im = N * in.Y[(int64_t)herm_im<NF>(0, 1) * np + p];
+        }
+        const int key = (c[2] * g.n[1] + c[1]) * g.n[0] + c[0];
+        for_each_wave_key(act, key, [&](int k, unsigned long long m, int lead) {
+            const bool mine = (m >> lane) & 1ull;
+            const int kx = k % g.n[0], ky = (k / g.n[0]) % g.n[1], kz = k / (g.n[0] * g.n[1]);
+#pragma unroll
+            for (int kk = 0; kk < 3; kk++) {
+                if (g.order[2] == 0 && kk != 1) continue;
+                const int tz = pmod(kz + kk - 1, g.n[2]);
+#pragma unroll
+                for (int jj = 0; jj < 3; jj++) {
+                    if (g.order[1] == 0 && jj != 1) continue;
+                    const int ty = pmod(ky + jj - 1, g.n[1]);
+#pragma unroll
+                    for (int ii = 0; ii < 3; ii++) {
+                        if (g.order[0] == 0 && ii != 1) continue;
+                        const int64_t t = ((int64_t)tz * g.n[1] + ty) * g.n[0] + pmod(kx + ii - 1, g.n[0]);
+                        const double w = wd[0][ii] * wd[1][jj] * wd[2][kk];
+                        const double a = wave_sum(mine ? w * re : 0.0), b = wave_sum(mine ? w * im : 0.0);
+                        if (lane == lead) {
+                            unsafeAtomicAdd(&g2[2 * t], a);
+                            unsafeAtomicAdd(&g2[2 * t + 1], b);
+                        }
+                    }
+                }
+            }
         });
     }
 }
@@ -476,6 +422,139 @@ __device__ __forceinline__ int wave_cell(int64_t ncell, int blocks_per_xcd)
     return c < ncell ? __builtin_amdgcn_readfirstlane((int)c) : -1;
 }
 
+// The deposit of both directions with the sum over particles on the matrix
+// cores.  KIND 0: the forward q_c = N f_c - conj(Nbar fbar_c) with (1, v);
+// KIND 1: the gather's adjoint mu_c with (1, -v).  Per-particle atomics would
+// put every particle of a cell on the same 27 x 4 NH grid entries, so the sum
+// over a cell's particles is done first, as in the fused stage kernel.
+//
+// One wave per cell of the cell list (the centre cell of the particles'
+// stencils): its particles share the 27 targets, so D[t][q] = sum_p W[p][t]
+// R[p][q] runs on f64 16x16x4 MFMAs over all of them, 64 at a time, and is
+// added to the grid once per cell, one atomic per value.
+template <int NF, int WX, int WY, int WZ, int KIND>
+__global__ __launch_bounds__(64 * kCellWaves) void k_cell_deposit(AdjIn in, Geom g, const int* start, const int* perm,
+                                                                  int64_t ncell, int bpx, const double* mu,
+                                                                  double* grid)
+{
+    constexpr int nh = NH<NF>, QG = 4 * nh;
+    constexpr int NT = WX * WY * WZ, MT = (NT + 15) / 16, NTL = (QG + 15) / 16;
+    constexpr int NSR = nh + 4;            // q (or mu), v (or -v), 1
+    constexpr int REG = 64 * 9 + 64 * NSR; // doubles per wave: weights, sources
+    __shared__ double lds[kCellWaves * REG];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double* Wf = lds + wave * REG;
+    double* Sr = Wf + 64 * 9;
+    const int cell = wave_cell(ncell, bpx);
+    if (cell < 0) return;
+    const int beg = __builtin_amdgcn_readfirstlane(start[cell]), end = __builtin_amdgcn_readfirstlane(start[cell + 1]);
+    if (beg == end) return;
+    const int64_t np = in.np;
+    // operand lane constants, as in the stage kernel: row t = 16m + l%16 reads the
+    // shape factors of target t; column q = f nh + c is source c times (1, v)_f
+    int tdx[MT][3];
+#pragma unroll
+    for (int m = 0; m < MT; m++) {
+        const int t = 16 * m + (lane & 15);
+        const int tc = t < NT ? t : NT - 1;
+        const int ii = tc % WX, jj = (tc / WX) % WY, kk = tc / (WX * WY);
+        tdx[m][0] = WX == 1 ? 1 : ii;
+        tdx[m][1] = 3 + (WY == 1 ? 1 : jj);
+        tdx[m][2] = 6 + (WZ == 1 ? 1 : kk);
+    }
+    int bi1[NTL], bi2[NTL];
+#pragma unroll
+    for (int n = 0; n < NTL; n++) {
+        const int q = 16 * n + (lane & 15);
+        const int f = q / nh, c = q % nh;
+        bi1[n] = q < QG ? c : nh + 3;
+        bi2[n] = q < QG ? (f == 0 ? nh + 3 : nh + f - 1) : nh + 3;
+    }
+    mf_d4 acc[MT][NTL];
+#pragma unroll
+    for (int m = 0; m < MT; m++)
+#pragma unroll
+        for (int n = 0; n < NTL; n++) acc[m][n] = mf_d4{0.0, 0.0, 0.0, 0.0};
+    for (int base = beg; base < end; base += 64) {
+        // stage the chunk: every lane a valid particle, the inactive ones with zero sources
+        const bool act = base + lane < end;
+        const int64_t pl = perm[act ? base + lane : beg];
+        const double pos[3] = {in.pos[pl], in.pos[np + pl], in.pos[2 * np + pl]};
+        const double pt = in.pup[3 * np + pl];
+        const double v[3] = {in.pup[pl] / pt, in.pup[np + pl] / pt, in.pup[2 * np + pl] / pt};
+        double wd[3][3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            int idx;
+            shape3((pos[d] - g.plo[d]) * g.dxi[d], g.order[d], idx, wd[d]);
+        }
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+#pragma unroll
+            for (int e = 0; e < 3; e++) Wf[lane * 9 + 3 * d + e] = wd[d][e];
+        if constexpr (KIND == 0) {
+            const double N = in.Nn[pl], Nb = in.Nn[np + pl];
+#pragma unroll
+            for (int c = 0; c < nh; c++) {
+                const double f = in.Y[(int64_t)c * np + pl], fb = in.Y[(int64_t)(nh + c) * np + pl];
+                Sr[lane * NSR + c] = act ? (is_im_comp<NF>(c) ? N * f + Nb * fb : N * f - Nb * fb) : 0.0;
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < nh; c++) Sr[lane * NSR + c] = act ? mu[(int64_t)c * np + pl] : 0.0;
+        }
+#pragma unroll
+        for (int d = 0; d < 3; d++) Sr[lane * NSR + nh + d] = KIND == 0 ? v[d] : -v[d];
+        Sr[lane * NSR + nh + 3] = 1.0;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int nks = (min(64, end - base) + 3) >> 2; // k-steps of 4 particles
+        for (int ks = 0; ks < nks; ks++) {
+            const int pp = 4 * ks + (lane >> 4);
+            const double* wf = Wf + pp * 9;
+            const double* sr = Sr + pp * NSR;
+            double A[MT];
+#pragma unroll
+            for (int m = 0; m < MT; m++) A[m] = wf[tdx[m][0]] * wf[tdx[m][1]] * wf[tdx[m][2]];
+#pragma unroll
+            for (int n = 0; n < NTL; n++) {
+                const double B = sr[bi1[n]] * sr[bi2[n]];
+#pragma unroll
+                for (int m = 0; m < MT; m++) acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[m], B, acc[m][n], 0, 0, 0);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    // element r of lane l holds D[16m + 4r + l/16][16n + l%16]; the neighbours
+    // of a cell are at most one period away, so they wrap with one add or
+    // subtract
+    const int kx = __builtin_amdgcn_readfirstlane(cell % g.n[0]);
+    const int ky = __builtin_amdgcn_readfirstlane((cell / g.n[0]) % g.n[1]);
+    const int kz = __builtin_amdgcn_readfirstlane(cell / (g.n[0] * g.n[1]));
+    auto wrap = [](int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); };
+#pragma unroll
+    for (int m = 0; m < MT; m++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int t = 16 * m + 4 * r + (lane >> 4);
+            if (t >= NT) continue;
+            const int ii = t % WX, jj = (t / WX) % WY, kk = t / (WX * WY);
+            const int tx = WX == 1 ? kx : wrap(kx + ii - 1, g.n[0]);
+            const int ty = WY == 1 ? ky : wrap(ky + jj - 1, g.n[1]);
+            const int tz = WZ == 1 ? kz : wrap(kz + kk - 1, g.n[2]);
+            double* G = grid + (((int64_t)tz * g.n[1] + ty) * g.n[0] + tx) * QG;
+#pragma unroll
+            for (int n = 0; n < NTL; n++) {
+                const int q = 16 * n + (lane & 15);
+                if (q < QG) unsafeAtomicAdd(&G[q], acc[m][n][r]);
+            }
+        }
+}
+
+
 // out[c] += sum over the neighbourhood of `cell` (wave-uniform) of
 // w_t (G_t[0][c] + s0 G_t[1][c] + s1 G_t[2][c] + s2 G_t[3][c]), with the
 // particle's shape factors w (shape3's windows) and multipliers s.
@@ -486,7 +565,13 @@ __device__ __forceinline__ void gather_cell_box(const Geom& g, int cell, const d
     constexpr int nh = NH<NF>, QG = 4 * nh;
     typedef __attribute__((address_space(4))) const double* cdp;
     const cdp grid = (cdp)grid_;
-    const int cx = cell % g.n[0], cy = (cell / g.n[0]) % g.n[1], cz = cell / (g.n[0] * g.n[1]);
+    // the neighbourhood's indices are wave-uniform: kept in scalar registers so
+    // that the grid rows below are scalar loads (vector loads of one address
+    // would return 64 copies of each row through the vector memory path)
+    const int cx = __builtin_amdgcn_readfirstlane(cell % g.n[0]);
+    const int cy = __builtin_amdgcn_readfirstlane((cell / g.n[0]) % g.n[1]);
+    const int cz = __builtin_amdgcn_readfirstlane(cell / (g.n[0] * g.n[1]));
+    auto wrap = [](int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); };
     int lo[3], hi[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
@@ -496,14 +581,14 @@ __device__ __forceinline__ void gather_cell_box(const Geom& g, int cell, const d
     auto pick = [](const double* x, int i) { return i == 0 ? x[0] : (i == 1 ? x[1] : x[2]); };
 #pragma unroll 1
     for (int kk = lo[2]; kk < hi[2]; kk++) {
-        const int nz = pmod(cz + kk - 1, g.n[2]);
+        const int nz = wrap(cz + kk - 1, g.n[2]);
 #pragma unroll 1
         for (int jj = lo[1]; jj < hi[1]; jj++) {
-            const int ny = pmod(cy + jj - 1, g.n[1]);
+            const int ny = wrap(cy + jj - 1, g.n[1]);
             const double wyz = pick(w[1], jj) * pick(w[2], kk);
 #pragma unroll 1
             for (int ii = lo[0]; ii < hi[0]; ii++) {
-                const int64_t nb = ((int64_t)nz * g.n[1] + ny) * g.n[0] + pmod(cx + ii - 1, g.n[0]);
+                const int64_t nb = __builtin_amdgcn_readfirstlane((nz * g.n[1] + ny) * g.n[0] + wrap(cx + ii - 1, g.n[0]));
                 const double wt = pick(w[0], ii) * wyz;
                 const double a[4] = {wt, wt * s[0], wt * s[1], wt * s[2]};
                 const cdp G = grid + nb * QG;
@@ -830,34 +915,44 @@ static double m00_of(const emu_ctx* ctx)
     return ctx->p.rho * ctx->p.Ye / phys::Mp / ctx->g.inv_cell_volume;
 }
 
-// k_seg_deposit for the stencil widths of the grid (3 where the shape order is
-// above 0, else 1), 2 waves per workgroup of 128 threads
+// blocks per XCD of a one-wave-per-cell launch
+static int cell_blocks_per_xcd(int64_t ncell)
+{
+    const int64_t nb = (ncell + kCellWaves - 1) / kCellWaves;
+    return (int)((nb + 7) / 8);
+}
+
+// k_cell_deposit for the stencil widths of the grid (3 where the shape order is
+// above 0, else 1), one wave per cell of the cell list
 template <int NF, int KIND>
-static void seg_deposit_nf(emu_ctx* ctx, const AdjIn& in, const int* perm, const double* mu, double* grid)
+static void cell_deposit_nf(emu_ctx* ctx, const AdjIn& in, const CellSort& cs, const double* mu, double* grid)
 {
     const int wx = ctx->g.order[0] > 0 ? 3 : 1, wy = ctx->g.order[1] > 0 ? 3 : 1, wz = ctx->g.order[2] > 0 ? 3 : 1;
-    const unsigned nb = (unsigned)((in.np + 127) / 128);
+    const int64_t nc = ncell_of(ctx);
+    const int bpx = cell_blocks_per_xcd(nc);
     const int code = (wx == 3) * 4 + (wy == 3) * 2 + (wz == 3);
-#define SEG_LAUNCH(X, Y, Z) hipLaunchKernelGGL((k_seg_deposit<NF, X, Y, Z, KIND>), dim3(nb), dim3(128), 0, ctx->stream, in, ctx->g, perm, mu, grid)
+#define DEP_LAUNCH(X, Y, Z)                                                                                            \
+    hipLaunchKernelGGL((k_cell_deposit<NF, X, Y, Z, KIND>), dim3(8 * bpx), dim3(64 * kCellWaves), 0, ctx->stream, in, \
+                       ctx->g, cs.start, cs.perm, nc, bpx, mu, grid)
     switch (code) {
-    case 0: SEG_LAUNCH(1, 1, 1); break;
-    case 1: SEG_LAUNCH(1, 1, 3); break;
-    case 2: SEG_LAUNCH(1, 3, 1); break;
-    case 3: SEG_LAUNCH(1, 3, 3); break;
-    case 4: SEG_LAUNCH(3, 1, 1); break;
-    case 5: SEG_LAUNCH(3, 1, 3); break;
-    case 6: SEG_LAUNCH(3, 3, 1); break;
-    default: SEG_LAUNCH(3, 3, 3); break;
+    case 0: DEP_LAUNCH(1, 1, 1); break;
+    case 1: DEP_LAUNCH(1, 1, 3); break;
+    case 2: DEP_LAUNCH(1, 3, 1); break;
+    case 3: DEP_LAUNCH(1, 3, 3); break;
+    case 4: DEP_LAUNCH(3, 1, 1); break;
+    case 5: DEP_LAUNCH(3, 1, 3); break;
+    case 6: DEP_LAUNCH(3, 3, 1); break;
+    default: DEP_LAUNCH(3, 3, 3); break;
     }
-#undef SEG_LAUNCH
+#undef DEP_LAUNCH
 }
 
 template <int KIND>
-static void seg_deposit(emu_ctx* ctx, const AdjIn& in, const int* perm, const double* mu, double* grid)
+static void cell_deposit(emu_ctx* ctx, const AdjIn& in, const CellSort& cs, const double* mu, double* grid)
 {
     if (in.np == 0) return;
-    if (ctx->p.nf == 2) seg_deposit_nf<2, KIND>(ctx, in, perm, mu, grid);
-    else seg_deposit_nf<3, KIND>(ctx, in, perm, mu, grid);
+    if (ctx->p.nf == 2) cell_deposit_nf<2, KIND>(ctx, in, cs, mu, grid);
+    else cell_deposit_nf<3, KIND>(ctx, in, cs, mu, grid);
 }
 
 // the particles listed by the centre cell of their stencil (positions are
@@ -867,10 +962,11 @@ static int cell_sort(emu_ctx* ctx, const AdjIn& in, const CellSort& cs)
     const int64_t np = in.np, nc = ncell_of(ctx);
     const int64_t zero_doubles = (2 * nc + 1 + 1) / 2; // start and cursor are adjacent ints
     hipLaunchKernelGGL(k_zero, dim3(nblk(zero_doubles)), dim3(kB), 0, ctx->stream, (double*)cs.start, zero_doubles);
-    hipLaunchKernelGGL(k_cell_keys, dim3(nblk(np)), dim3(kB), 0, ctx->stream, in, ctx->g, cs.key, cs.start);
+    const unsigned nb = (unsigned)((np + kB - 1) / kB); // one thread per particle
+    hipLaunchKernelGGL(k_cell_keys, dim3(nb), dim3(kB), 0, ctx->stream, in, ctx->g, cs.key, cs.start);
     size_t bytes = cs.tmp_bytes;
     EMU_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(cs.tmp, bytes, cs.start, cs.start, (int)(nc + 1), ctx->stream));
-    hipLaunchKernelGGL(k_cell_perm, dim3(nblk(np)), dim3(kB), 0, ctx->stream, np, cs.key, cs.start, cs.cursor, cs.perm);
+    hipLaunchKernelGGL(k_cell_perm, dim3(nb), dim3(kB), 0, ctx->stream, np, cs.key, cs.start, cs.cursor, cs.perm);
     return EMU_OK;
 }
 
@@ -879,7 +975,7 @@ static void deposit(emu_ctx* ctx, const AdjIn& in, const CellSort& cs, double* g
     const int64_t ng = ncell_of(ctx) * 4 * ctx->p.nf * ctx->p.nf;
     if (ctx->p.nf == 2) hipLaunchKernelGGL(k_fill_base<2>, dim3(nblk(ng)), dim3(kB), 0, ctx->stream, grid, ncell_of(ctx), m00_of(ctx));
     else hipLaunchKernelGGL(k_fill_base<3>, dim3(nblk(ng)), dim3(kB), 0, ctx->stream, grid, ncell_of(ctx), m00_of(ctx));
-    seg_deposit<0>(ctx, in, cs.perm, nullptr, grid);
+    cell_deposit<0>(ctx, in, cs, nullptr, grid);
 }
 
 extern "C" int emu_adj_rhs(emu_ctx* ctx, int64_t np, const double* pos, const double* pup, const double* Nn,
@@ -913,7 +1009,7 @@ extern "C" int emu_adj_rhs_vjp(emu_ctx* ctx, int64_t np, const double* pos, cons
     deposit(ctx, in, cs, grid);
     ADJ_DISPATCH(k_vjp_comm, in, ctx->g, cs.perm, grid, ham_params(ctx), lam_k, lam_Y, mu);
     hipLaunchKernelGGL(k_zero, dim3(nblk(ng)), dim3(kB), 0, ctx->stream, gadj, ng);
-    seg_deposit<1>(ctx, in, cs.perm, mu, gadj);
+    cell_deposit<1>(ctx, in, cs, mu, gadj);
     ADJ_DISPATCH(k_vjp_gather, in, ctx->g, cs.perm, gadj, lam_Y, lam_Nbar);
     EMU_LAUNCH_CHECK(ctx, "adj_rhs_vjp");
     return EMU_OK;
@@ -1001,12 +1097,6 @@ StepWork step_work(const emu_ctx* ctx, int64_t np, double* work)
     return w;
 }
 
-// blocks per XCD of a one-wave-per-cell launch
-int cell_blocks_per_xcd(int64_t ncell)
-{
-    const int64_t nb = (ncell + kCellWaves - 1) / kCellWaves;
-    return (int)((nb + 7) / 8);
-}
 
 int arm_status(emu_ctx* ctx)
 {
@@ -1028,48 +1118,88 @@ void fwd_stage(emu_ctx* ctx, const AdjIn& in, const StepWork& w, const double* g
                        pos, ctx->d_red + 5);
 }
 
+// A step's kept data (emu_adj_step_keep): the stage states Y2, Y3, Y4
+// ([3][2 NH][np]) and the four stage deposits ([4][ng]).
+struct StepKeep {
+    double* Ys[3];
+    double* G[4];
+};
+
+StepKeep step_keep(const emu_ctx* ctx, int64_t np, double* keep)
+{
+    const int nh = ctx->p.nf * ctx->p.nf;
+    const int64_t ng = ncell_of(ctx) * 4 * nh, ns = 2 * nh * np;
+    StepKeep k;
+    double* b = keep;
+    for (int s = 0; s < 3; s++, b += ns) k.Ys[s] = b;
+    for (int s = 0; s < 4; s++, b += ng) k.G[s] = b;
+    return k;
+}
+
+int64_t step_keep_doubles(const emu_ctx* ctx, int64_t np)
+{
+    const int nh = ctx->p.nf * ctx->p.nf;
+    return 3 * 2 * nh * np + 4 * ncell_of(ctx) * 4 * nh;
+}
+
+// Stage s reads Y_s and its deposit G[s - 1] and writes Y_{s+1} into Ys[s - 1]:
+// every stage state and deposit has its own array, so that a step run with
+// `keep` leaves them all for its VJP.
 template <int NF>
 int adj_step_nf(emu_ctx* ctx, int64_t np, double* pos, const double* pup, const double* Nn, const double* Ll,
-                const double* y_in, double* y_out, double dt, double* work)
+                const double* y_in, double* y_out, double dt, double* work, double* keep)
 {
-    const StepWork w = step_work(ctx, np, work);
+    StepWork w = step_work(ctx, np, work);
+    if (keep) {
+        const StepKeep k = step_keep(ctx, np, keep);
+        for (int s = 0; s < 3; s++) w.Ys[s] = k.Ys[s];
+        for (int s = 0; s < 4; s++) w.G[s] = k.G[s];
+    }
     AdjIn in{np, pos, pup, Nn, y_in};
     if (int e = cell_sort(ctx, in, w.cs)) return e;
-    double* Ys = w.Ys[0];
     deposit(ctx, in, w.cs, w.G[0]);
-    fwd_stage<NF, 1, 0>(ctx, in, w, w.G[0], y_in, Ys, y_out, Ll, dt, pos);
-    in.Y = Ys;
-    deposit(ctx, in, w.cs, w.G[0]);
-    fwd_stage<NF, 2, 0>(ctx, in, w, w.G[0], y_in, Ys, y_out, Ll, dt, pos);
-    deposit(ctx, in, w.cs, w.G[0]);
-    fwd_stage<NF, 3, 0>(ctx, in, w, w.G[0], y_in, Ys, y_out, Ll, dt, pos);
-    deposit(ctx, in, w.cs, w.G[0]);
-    fwd_stage<NF, 4, 0>(ctx, in, w, w.G[0], y_in, Ys, y_out, Ll, dt, pos);
+    fwd_stage<NF, 1, 0>(ctx, in, w, w.G[0], y_in, w.Ys[0], y_out, Ll, dt, pos);
+    in.Y = w.Ys[0];
+    deposit(ctx, in, w.cs, w.G[1]);
+    fwd_stage<NF, 2, 0>(ctx, in, w, w.G[1], y_in, w.Ys[1], y_out, Ll, dt, pos);
+    in.Y = w.Ys[1];
+    deposit(ctx, in, w.cs, w.G[2]);
+    fwd_stage<NF, 3, 0>(ctx, in, w, w.G[2], y_in, w.Ys[2], y_out, Ll, dt, pos);
+    in.Y = w.Ys[2];
+    deposit(ctx, in, w.cs, w.G[3]);
+    fwd_stage<NF, 4, 0>(ctx, in, w, w.G[3], y_in, nullptr, y_out, Ll, dt, pos);
     return EMU_OK;
 }
 
 template <int NF>
 int adj_step_vjp_nf(emu_ctx* ctx, int64_t np, const double* pos, const double* pup, const double* Nn,
                     const double* y_in, const double* lam_in, double* lam_out, double* lam_Nbar, double dt,
-                    double* work)
+                    double* work, const double* keep)
 {
     const int nh = NF * NF;
     const int64_t ng = ncell_of(ctx) * 4 * nh;
-    const StepWork w = step_work(ctx, np, work);
+    StepWork w = step_work(ctx, np, work);
     AdjIn in{np, pos, pup, Nn, y_in};
     if (int e = cell_sort(ctx, in, w.cs)) return e;
-    // recompute the stage states and their deposits
+    if (keep) {
+        // the forward step's own stage states and deposits
+        const StepKeep k = step_keep(ctx, np, (double*)keep);
+        for (int s = 0; s < 3; s++) w.Ys[s] = k.Ys[s];
+        for (int s = 0; s < 4; s++) w.G[s] = k.G[s];
+    } else {
+        // recompute the stage states and their deposits
+        deposit(ctx, in, w.cs, w.G[0]);
+        fwd_stage<NF, 1, 1>(ctx, in, w, w.G[0], y_in, w.Ys[0], nullptr, nullptr, dt, nullptr);
+        in.Y = w.Ys[0];
+        deposit(ctx, in, w.cs, w.G[1]);
+        fwd_stage<NF, 2, 1>(ctx, in, w, w.G[1], y_in, w.Ys[1], nullptr, nullptr, dt, nullptr);
+        in.Y = w.Ys[1];
+        deposit(ctx, in, w.cs, w.G[2]);
+        fwd_stage<NF, 3, 1>(ctx, in, w, w.G[2], y_in, w.Ys[2], nullptr, nullptr, dt, nullptr);
+        in.Y = w.Ys[2];
+        deposit(ctx, in, w.cs, w.G[3]);
+    }
     const double* Yst[4] = {y_in, w.Ys[0], w.Ys[1], w.Ys[2]};
-    deposit(ctx, in, w.cs, w.G[0]);
-    fwd_stage<NF, 1, 1>(ctx, in, w, w.G[0], y_in, w.Ys[0], nullptr, nullptr, dt, nullptr);
-    in.Y = w.Ys[0];
-    deposit(ctx, in, w.cs, w.G[1]);
-    fwd_stage<NF, 2, 1>(ctx, in, w, w.G[1], y_in, w.Ys[1], nullptr, nullptr, dt, nullptr);
-    in.Y = w.Ys[1];
-    deposit(ctx, in, w.cs, w.G[2]);
-    fwd_stage<NF, 3, 1>(ctx, in, w, w.G[2], y_in, w.Ys[2], nullptr, nullptr, dt, nullptr);
-    in.Y = w.Ys[2];
-    deposit(ctx, in, w.cs, w.G[3]);
     // reverse: lambda_k4 = dt/6 lam; lambda_k3 = dt/3 lam + dt lY4;
     // lambda_k2 = dt/3 lam + dt/2 lY3; lambda_k1 = dt/6 lam + dt/2 lY2;
     // lam_out = lam + lY4 + lY3 + lY2 + lY1
@@ -1084,7 +1214,7 @@ int adj_step_vjp_nf(emu_ctx* ctx, int64_t np, const double* pos, const double* p
                            w.cs.start, w.cs.perm, ncell_of(ctx), bpx, w.G[s], ham_params(ctx), lam_in, lYp, alpha[s],
                            beta[s], lY, w.mu);
         hipLaunchKernelGGL(k_zero, dim3(nblk(ng)), dim3(kB), 0, ctx->stream, w.gadj, ng);
-        seg_deposit<1>(ctx, in, w.cs.perm, w.mu, w.gadj);
+        cell_deposit<1>(ctx, in, w.cs, w.mu, w.gadj);
         hipLaunchKernelGGL(k_vjp_stage_gather<NF>, dim3(8 * bpx), dim3(64 * kCellWaves), 0, ctx->stream, in, ctx->g,
                            w.cs.start, w.cs.perm, ncell_of(ctx), bpx, w.gadj, lY, lam_Nbar, lam_in, lam_out,
                            s == 3 ? 1 : 0);
@@ -1100,31 +1230,66 @@ extern "C" int64_t emu_adj_step_work_doubles(const emu_ctx* ctx, int64_t np)
     return 5 * ncell_of(ctx) * 4 * nh + 5 * 2 * nh * np + (int64_t)nh * np + cell_sort_doubles(ctx, np);
 }
 
-extern "C" int emu_adj_step(emu_ctx* ctx, int64_t np, double* pos, const double* pup, const double* Nn,
-                            const double* Ll, const double* y_in, double* y_out, double dt, double* work)
+static int adj_step_any(emu_ctx* ctx, int64_t np, double* pos, const double* pup, const double* Nn,
+                        const double* Ll, const double* y_in, double* y_out, double dt, double* work, double* keep)
 {
     if (!ctx || np < 0 || !work || y_in == y_out) return EMU_ERR_ARG;
     arm_status(ctx);
     if (np == 0) return EMU_OK;
-    const int e = ctx->p.nf == 2 ? adj_step_nf<2>(ctx, np, pos, pup, Nn, Ll, y_in, y_out, dt, work)
-                                 : adj_step_nf<3>(ctx, np, pos, pup, Nn, Ll, y_in, y_out, dt, work);
+    const int e = ctx->p.nf == 2 ? adj_step_nf<2>(ctx, np, pos, pup, Nn, Ll, y_in, y_out, dt, work, keep)
+                                 : adj_step_nf<3>(ctx, np, pos, pup, Nn, Ll, y_in, y_out, dt, work, keep);
     if (e) return e;
     EMU_LAUNCH_CHECK(ctx, "adj_step");
     return EMU_OK;
+}
+
+static int adj_step_vjp_any(emu_ctx* ctx, int64_t np, const double* pos, const double* pup, const double* Nn,
+                            const double* y_in, const double* lam_in, double* lam_out, double* lam_Nbar, double dt,
+                            double* work, const double* keep)
+{
+    if (!ctx || np < 0 || !work || lam_in == lam_out) return EMU_ERR_ARG;
+    if (np == 0) return EMU_OK;
+    const int e = ctx->p.nf == 2
+                      ? adj_step_vjp_nf<2>(ctx, np, pos, pup, Nn, y_in, lam_in, lam_out, lam_Nbar, dt, work, keep)
+                      : adj_step_vjp_nf<3>(ctx, np, pos, pup, Nn, y_in, lam_in, lam_out, lam_Nbar, dt, work, keep);
+    if (e) return e;
+    EMU_LAUNCH_CHECK(ctx, "adj_step_vjp");
+    return EMU_OK;
+}
+
+extern "C" int emu_adj_step(emu_ctx* ctx, int64_t np, double* pos, const double* pup, const double* Nn,
+                            const double* Ll, const double* y_in, double* y_out, double dt, double* work)
+{
+    return adj_step_any(ctx, np, pos, pup, Nn, Ll, y_in, y_out, dt, work, nullptr);
 }
 
 extern "C" int emu_adj_step_vjp(emu_ctx* ctx, int64_t np, const double* pos, const double* pup, const double* Nn,
                                 const double* y_in, const double* lam_in, double* lam_out, double* lam_Nbar, double dt,
                                 double* work)
 {
-    if (!ctx || np < 0 || !work || lam_in == lam_out) return EMU_ERR_ARG;
-    if (np == 0) return EMU_OK;
-    const int e = ctx->p.nf == 2
-                      ? adj_step_vjp_nf<2>(ctx, np, pos, pup, Nn, y_in, lam_in, lam_out, lam_Nbar, dt, work)
-                      : adj_step_vjp_nf<3>(ctx, np, pos, pup, Nn, y_in, lam_in, lam_out, lam_Nbar, dt, work);
-    if (e) return e;
-    EMU_LAUNCH_CHECK(ctx, "adj_step_vjp");
-    return EMU_OK;
+    return adj_step_vjp_any(ctx, np, pos, pup, Nn, y_in, lam_in, lam_out, lam_Nbar, dt, work, nullptr);
+}
+
+extern "C" int64_t emu_adj_step_keep_doubles(const emu_ctx* ctx, int64_t np)
+{
+    if (!ctx || np < 0) return -1;
+    return step_keep_doubles(ctx, np);
+}
+
+extern "C" int emu_adj_step_keep(emu_ctx* ctx, int64_t np, double* pos, const double* pup, const double* Nn,
+                                 const double* Ll, const double* y_in, double* y_out, double dt, double* work,
+                                 double* keep)
+{
+    if (!keep) return EMU_ERR_ARG;
+    return adj_step_any(ctx, np, pos, pup, Nn, Ll, y_in, y_out, dt, work, keep);
+}
+
+extern "C" int emu_adj_step_vjp_kept(emu_ctx* ctx, int64_t np, const double* pos, const double* pup,
+                                     const double* Nn, const double* y_in, const double* keep, const double* lam_in,
+                                     double* lam_out, double* lam_Nbar, double dt, double* work)
+{
+    if (!keep) return EMU_ERR_ARG;
+    return adj_step_vjp_any(ctx, np, pos, pup, Nn, y_in, lam_in, lam_out, lam_Nbar, dt, work, keep);
 }
 
 namespace {

@@ -499,6 +499,7 @@ struct emu_state {
     size_t cub_bytes = 0;
     int cur = 0;      // which persistent copy is current
     int gcur = 0;     // grid holding the deposit of the current state
+    bool evolved = false; // a step was taken since the last load
     double Vvac_max = 0.0;
     double m00 = 0.0;
     // The particle weights are held as N * nscale, nscale = 2^-k chosen at load
@@ -899,6 +900,23 @@ extern "C" int emu_state_load(emu_state* st, const emu_particles* parts, const i
     if (e) return e;
     EMU_HIP(ctx, hipMemcpyAsync(st->scal + 1, st->scal + 2, sizeof(double), hipMemcpyDeviceToDevice, s));
     EMU_HIP(ctx, hipStreamSynchronize(s));
+    st->evolved = false;
+    return EMU_OK;
+}
+
+extern "C" int emu_state_set_vvac_max(emu_state* st, double vvac_max)
+{
+    if (!st || !(vvac_max >= 0.0) || !std::isfinite(vvac_max)) return EMU_ERR_ARG;
+    emu_ctx* ctx = st->ctx;
+    if (st->evolved) return set_error(ctx, EMU_ERR_ARG, "Vvac_max is set between emu_state_load and the first step");
+    st->Vvac_max = vvac_max;
+    // grid[0] (and aux) still hold the load's deposit: the starting dt again
+    // (main.cpp:206) with the new value
+    const int e = dt_from_grid(st, st->grid[0], 0.0);
+    if (e) return e;
+    hipStream_t s = ctx->stream;
+    EMU_HIP(ctx, hipMemcpyAsync(st->scal + 1, st->scal + 2, sizeof(double), hipMemcpyDeviceToDevice, s));
+    EMU_HIP(ctx, hipStreamSynchronize(s));
     return EMU_OK;
 }
 
@@ -955,6 +973,7 @@ extern "C" int emu_state_evolve(emu_state* st, int nsteps, double fixed_dt)
         if (fixed_dt <= 0 || fixed_dt * phys::c >= std::min(std::min(g.dx[0], g.dx[1]), g.dx[2]))
             return set_error(ctx, EMU_ERR_MOVE, "fused step needs c*dt < dx (0 < cfl_factor < 1)");
     if (fixed_dt > 0) EMU_HIP(ctx, hipMemcpyAsync(st->scal + 1, &fixed_dt, sizeof(double), hipMemcpyHostToDevice, s));
+    if (nsteps > 0) st->evolved = true;
     for (int step = 0; step < nsteps; step++) {
         double* A = st->grid[st->gcur];     // holds the deposit of y_n
         double* B = st->grid[1 - st->gcur];

@@ -66,6 +66,24 @@ int emu_adj_step_vjp(emu_ctx* ctx, int64_t np, const double* pos, const double* 
                      const double* y_in, const double* lam_in, double* lam_out, double* lam_Nbar, double dt,
                      double* work);
 
+/* Doubles of one step's keep buffer: its stage states Y2, Y3, Y4 and its four
+ * stage deposits.  No counterpart in the reference (the forward keeps no
+ * stage data; this buffer trades memory for the VJP's recompute). */
+int64_t emu_adj_step_keep_doubles(const emu_ctx* ctx, int64_t np);
+
+/* emu_adj_step that leaves its stage states and deposits in keep
+ * (emu_adj_step_keep_doubles doubles) for emu_adj_step_vjp_kept.  Same
+ * y_out, positions and deferred status as emu_adj_step.  Asynchronous. */
+int emu_adj_step_keep(emu_ctx* ctx, int64_t np, double* pos, const double* pup, const double* Nn, const double* Ll,
+                      const double* y_in, double* y_out, double dt, double* work, double* keep);
+
+/* emu_adj_step_vjp of a step run by emu_adj_step_keep (pos, y_in: that step's
+ * start): the stage states and deposits come from keep instead of being
+ * recomputed.  Asynchronous. */
+int emu_adj_step_vjp_kept(emu_ctx* ctx, int64_t np, const double* pos, const double* pup, const double* Nn,
+                          const double* y_in, const double* keep, const double* lam_in, double* lam_out,
+                          double* lam_Nbar, double dt, double* work);
+
 /* The particles listed by cell (the cell of their stencil's centre) at
  * positions pos: order[i] is the i-th particle in that order (within a cell,
  * in no set order).  The step calls run fastest on arrays kept in this

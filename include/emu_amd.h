@@ -171,6 +171,15 @@ int emu_state_destroy(emu_state* st);
 /* Load particles (device SoA view, any order); computes Vvac_max, deposits and
  * sets the starting dt exactly as main.cpp:115-206.  Synchronous. */
 int emu_state_load(emu_state* st, const emu_particles* parts, const int64_t* ids);
+/* Replace the Vvac_max that emu_state_load computed (the vacuum term added to
+ * compute_dt's flavor speeds, Evolve.cpp:60-61) and recompute the starting dt
+ * with it.  The reference sets FlavoredNeutrinoContainer::Vvac_max
+ * (FlavoredNeutrinoContainer.H:71) only in InitParticles
+ * (FlavoredNeutrinoContainerInit.cpp:657-660), so a run restarted from a
+ * checkpoint (main.cpp:103-105) keeps its zero initial value: a restart driver
+ * passes 0.  Only between emu_state_load and the first emu_state_evolve
+ * (EMU_ERR_ARG afterwards).  Synchronous. */
+int emu_state_set_vvac_max(emu_state* st, double vvac_max);
 /* Write the state back (sorted order) with each particle's id.  Synchronous. */
 int emu_state_store(emu_state* st, emu_particles* parts, int64_t* ids);
 /* Advance nsteps RK4 steps (asynchronous; dt stays on the device between

@@ -150,6 +150,47 @@ def test_fused_step_matches_per_rhs_step(mods, oracle, nf):
     assert float((g * dy).sum()) == pytest.approx(fd, rel=1e-8)
 
 
+@pytest.mark.parametrize("nf", [2, 3])
+def test_kept_stages_match_recompute(mods, oracle, nf):
+    """emu_adj_step_keep gives emu_adj_step's y_{n+1} and positions, and
+    emu_adj_step_vjp_kept (the stage states and deposits taken from the kept
+    buffer) the recomputing VJP's lam_n and lam_Nbar; the three segment modes
+    give one gradient.  The deposits' atomics add in no set order, so the bar
+    is 1e-12 of each array's largest entry, not equality."""
+    torch, core, adjoint = mods
+    emu, c, deck, parts = setup(mods, oracle, nf, amp=1e-3)
+    dt = 0.4 * 0.5 / 2.99792458e10 * 0.5
+    prob = adjoint.AdjointProblem(emu, c, dt=dt, nsteps=3)
+    keep = torch.empty(prob.keep_doubles, dtype=torch.float64, device=prob.f0.device)
+
+    def close(a, b):
+        return float((a - b).abs().max() / b.abs().max().clamp_min(1e-300)) < 1e-12
+
+    pa, pb = prob.pos0.clone(), prob.pos0.clone()
+    ya = prob.step(pa, prob.Nn_base, prob.f0)
+    yb = prob.step(pb, prob.Nn_base, prob.f0, keep=keep)
+    prob.check_status()
+    assert torch.equal(pa, pb)
+    assert close(yb, ya)
+    gen = torch.Generator(device="cpu").manual_seed(7)
+    lam = torch.randn(prob.f0.shape, generator=gen, dtype=torch.float64).to(prob.f0.device)
+    prob.lam_nbar.zero_()
+    la = prob.step_adjoint(prob.pos0, prob.Nn_base, prob.f0, lam)
+    na = prob.lam_nbar.clone()
+    prob.lam_nbar.zero_()
+    lb = prob.step_adjoint(prob.pos0, prob.Nn_base, prob.f0, lam, keep=keep)
+    assert close(lb, la)
+    assert close(prob.lam_nbar, na)
+    grads = {}
+    for seg in ("keep", "store", "recompute"):
+        p = adjoint.AdjointProblem(emu, c, dt=dt, nsteps=8, checkpoint_every=3, segment=seg)
+        grads[seg] = p.gradients(p.f0.clone(), 1.0)
+    for seg in ("store", "recompute"):
+        assert grads[seg][0] == pytest.approx(grads["keep"][0], rel=1e-12)
+        assert close(grads[seg][1], grads["keep"][1])
+        assert grads[seg][2] == pytest.approx(grads["keep"][2], rel=1e-10)
+
+
 def test_adjoint32_gradient_matches_finite_differences(mods):
     """The two-flavour 32^3 x 92 workload (3,014,656 particles), 200 RK4
     steps with checkpoints every 20: dJ/dratio and directional derivatives of

@@ -142,7 +142,8 @@ def test_cpp_plotfiles_and_restart(cpp, oracle, tmp_path, monkeypatch):
     """evolve_flavor writes plt00000 (initial state) and plt<step+1> after each
     step (main.cpp:118-121,181-191); their particles and the last grid match
     the oracle.  A restart from plt00002 (IO.cpp:38-63) then takes one step,
-    with Vvac_max = 0 as in the reference's restarted process."""
+    with Vvac_max = 0 as in the reference's restarted process, on the plain
+    and the fused FP64 loops."""
     from emu_amd import plotfile
     nf, nsteps = 2, 3
     d = cases.fast_flavor_dict(nf=nf, ncell=(4, 4, 4), nphi=4, nsteps=nsteps, end_time=1.0, write_plot_every=1,
@@ -187,9 +188,6 @@ def test_cpp_plotfiles_and_restart(cpp, oracle, tmp_path, monkeypatch):
     rd = dict(d, do_restart=1, restart_dir=str(tmp_path / "plt00002"), nsteps=1)
     rdeck = tmp_path / "inputs_restart"
     rdeck.write_text(cases.deck_text(rd))
-    dump = tmp_path / "restart.bin"
-    code, out, err = run(cpp, rdeck, "plain", dump)
-    assert code == 0, err
     start = plot_particles(oracle, nf, tmp_path / "plt00002")
     t0 = plotfile.read_header(str(tmp_path / "plt00002"))["time"]
     g2 = oracle.Grid(nf, prm)
@@ -197,12 +195,21 @@ def test_cpp_plotfiles_and_restart(cpp, oracle, tmp_path, monkeypatch):
     e, dt0 = oracle.compute_dt(nf, prm, g2, 0.0)
     e, _, bad = oracle.rk4_step(nf, prm, start, g2, dt0, 0.0)
     assert e == 0
-    ids, pos, rdata, time = read_dump(dump, nf)
     want = start[np.argsort(start["id"])]
-    np.testing.assert_array_equal(ids, want["id"])
-    assert time == pytest.approx(t0 + dt0, rel=1e-13)
-    np.testing.assert_allclose(pos, want["pos"], rtol=1e-13, atol=0)
-    assert cases.max_rel_err(rdata, want["rdata"]) < 1e-12
+    # the Vvac_max the first run used moves dt by ~3e-9 relative here, far
+    # above the 1e-13 bar on the time, so the time pins Vvac_max = 0
+    e, dt_vv = oracle.compute_dt(nf, prm, g2, vv)
+    assert abs(dt_vv - dt0) > 1e-10 * dt0
+    # the plain loop, then the fused FP64 loop (emu_state_set_vvac_max)
+    for path in ("plain", "fused-fp64"):
+        dump = tmp_path / ("restart_%s.bin" % path)
+        code, out, err = run(cpp, rdeck, path, dump)
+        assert code == 0, (path, err)
+        ids, pos, rdata, time = read_dump(dump, nf)
+        np.testing.assert_array_equal(ids, want["id"])
+        assert time == pytest.approx(t0 + dt0, rel=1e-13), path
+        np.testing.assert_allclose(pos, want["pos"], rtol=1e-13, atol=0)
+        assert cases.max_rel_err(rdata, want["rdata"]) < 1e-12, path
 
 
 def test_cpp_fused_plotfile_schedule(cpp, oracle, tmp_path, monkeypatch):
