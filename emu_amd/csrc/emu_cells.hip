@@ -245,15 +245,16 @@ __global__ __launch_bounds__(64 * kWavesGather) void k_gather_boxes(PView P, Fab
     }
 }
 
-// The deposit takes boxes a patch at a time: PX x PY x PZ boxes (2 x 2 x 2 for
-// 3-D stencils, 1 x 1 x 8 along z for 1-D ones), one wave each.  Their
+// The deposit takes boxes a patch at a time: PX x PY x PZ boxes (2 x 2 x 1 for
+// 3-D stencils, 1 x 1 x 4 along z for 1-D ones), one wave each.  Their
 // stencil regions overlap, so the waves add their sums into the patch's
-// region in LDS first (4 x 4 x 4 cells instead of 8 x 27) and the workgroup
-// adds that to the fab: 3.4x fewer device-scope atomics, which are what
-// bounds this kernel.
+// region in LDS first (4 x 4 x 3 cells instead of 4 x 27) and the workgroup
+// adds that to the fab: 2.25x fewer device-scope atomics.  Four waves per
+// workgroup: the kernel's ~160 VGPRs allow three waves per SIMD, which an
+// 8-wave workgroup (two per SIMD) cannot use.
 template <int WX, int WY, int WZ> struct DepPatch {
     static constexpr int PX = WX == 3 ? 2 : 1, PY = WY == 3 ? 2 : 1;
-    static constexpr int PZ = WZ == 3 ? 8 / (PX * PY) : 1;
+    static constexpr int PZ = WZ == 3 ? 4 / (PX * PY) : 1;
     static constexpr int WAVES = PX * PY * PZ;
     static constexpr int RX = PX + WX - 1, RY = PY + WY - 1, RZ = PZ + WZ - 1, NR = RX * RY * RZ;
 };
@@ -280,7 +281,7 @@ constexpr int dep_lds_doubles()
 }
 
 template <int NF, int WX, int WY, int WZ>
-__global__ __launch_bounds__((64 * DepPatch<WX, WY, WZ>::WAVES)) void k_deposit_boxes(
+__global__ __launch_bounds__((64 * DepPatch<WX, WY, WZ>::WAVES)) __attribute__((amdgpu_waves_per_eu(3))) void k_deposit_boxes(
     PView P, Fab fab, Geom g, const int* __restrict__ start, const int* __restrict__ perm, int npx, int npy,
     int64_t npatch, int blocks_per_xcd)
 {

@@ -143,10 +143,13 @@ __global__ void k_fill_u64(unsigned long long* p, int n, unsigned long long v)
     if ((int)threadIdx.x < n) p[threadIdx.x] = v;
 }
 
+// grid-stride over at most 1024 workgroups, so that one word takes a few
+// thousand atomics rather than one per wave of particles
 __global__ void k_min_pupt(const double* __restrict__ pupt, int64_t np, unsigned long long* red)
 {
-    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    double v = t < np ? pupt[t] : INFINITY;
+    double v = INFINITY;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < np; t += (int64_t)gridDim.x * blockDim.x)
+        v = fmin(v, pupt[t]);
     for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off));
     if ((threadIdx.x & 63) == 0) atomicMin(red, (unsigned long long)__double_as_longlong(v));
 }
@@ -551,7 +554,8 @@ extern "C" int emu_vvac_max(emu_ctx* ctx, const emu_particles* parts, double* ou
     std::memcpy(&inf_bits, &inf, 8);
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, ctx->d_red, 1, inf_bits);
     if (P.np > 0)
-        hipLaunchKernelGGL(k_min_pupt, dim3(nblocks(P.np)), dim3(kBlock), 0, ctx->stream, P.r[ipupt], P.np, ctx->d_red);
+        hipLaunchKernelGGL(k_min_pupt, dim3(std::min(nblocks(P.np), 1024u)), dim3(kBlock), 0, ctx->stream, P.r[ipupt],
+                           P.np, ctx->d_red);
     EMU_LAUNCH_CHECK(ctx, "vvac_max");
     unsigned long long h;
     EMU_HIP(ctx, hipMemcpyAsync(&h, ctx->d_red, 8, hipMemcpyDeviceToHost, ctx->stream));
