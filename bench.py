@@ -234,33 +234,39 @@ def run_adjoint(args):
     st.load(parts)
     dt = st.info()["dt"]  # compute_dt of the initial state, held fixed over the run
     st.close()
-    cp = max(1, min(20, nsteps))
-    prob = adjoint.AdjointProblem(emu, parts, dt=dt, nsteps=nsteps, checkpoint_every=cp)
+    cp = max(1, min(args.checkpoint_every, nsteps))
+    prec = "fp32" if args.precision == "fp32" else "fp64"
+    prob = adjoint.AdjointProblem(emu, parts, dt=dt, nsteps=nsteps, checkpoint_every=cp, segment=args.segment,
+                                  precision=prec)
+    footprint = prob.footprint_bytes()
     f0 = prob.f0.clone()
-    warm = adjoint.AdjointProblem(emu, parts, dt=dt, nsteps=min(2, nsteps), checkpoint_every=1)
+    warm = adjoint.AdjointProblem(emu, parts, dt=dt, nsteps=min(2, nsteps), checkpoint_every=1, precision=prec)
     warm.gradients(warm.f0.clone(), 1.0)
+    del warm
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     J, lam, cps, Nn, order = prob.run(f0, 1.0)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     g = prob.adjoint_sweep(lam, cps, Nn, order)
-    dratio = float((prob.lam_nbar * prob.Nn_base[1]).sum().item())
+    dratio = float(prob.ratio_gradient().item())
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     ncell = parms.ncell[0] * parms.ncell[1] * parms.ncell[2]
     ppc = parts.np / ncell
-    bps = 4 * algorithmic_bytes(parms.nf, ppc, "fp64")
+    bps = 4 * algorithmic_bytes(parms.nf, ppc, prec)
     total = t2 - t0
     achieved = parts.np * nsteps * bps / total / 1e9
     out = {
         "metric": "adjoint gradient (forward with checkpoints + reverse sweep), particle-steps/s",
         "value": parts.np * nsteps / total, "unit": "particle-steps/s", "n_gpus": 1, "steps": nsteps,
-        "ms_per_step": total / nsteps * 1e3, "higher_is_better": True, "dtype": "f64",
+        "ms_per_step": total / nsteps * 1e3, "higher_is_better": True, "dtype": "f64" if prec == "fp64" else "f32",
         "data": "synthetic (seeded initial conditions of the workload, generated on the GPU)",
         "config": {"workload": f"{workload}: {parms.nf} flavours, {parms.ncell[0]}x{parms.ncell[1]}x{parms.ncell[2]} "
                                f"periodic cells, {round(ppc)} directions/cell, {parts.np} particles",
-                   "particles": parts.np, "checkpoint_every": cp, "dt_s": dt},
+                   "particles": parts.np, "checkpoint_every": cp, "segment": args.segment, "precision": prec,
+                   "dt_s": dt},
+        "footprint_gib": footprint / 2**30, "peak_allocated_gib": torch.cuda.max_memory_allocated() / 2**30,
         "forward_ms_per_step": (t1 - t0) / nsteps * 1e3, "reverse_ms_per_step": (t2 - t1) / nsteps * 1e3,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes_per_particle_step": bps},
@@ -326,6 +332,9 @@ def main():
                     help="time one gradient through the hand-written adjoint instead (forward with checkpoints and "
                          "the reverse sweep; the 2-flavour 32^3 x 92 workload, 200 steps, checkpoints every 20, "
                          "unless --workload / --steps say otherwise)")
+    ap.add_argument("--segment", default="keep", choices=["keep", "store", "recompute"],
+                    help="--adjoint: what the reverse sweep holds per segment")
+    ap.add_argument("--checkpoint-every", type=int, default=20, help="--adjoint: steps per checkpoint")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device (5 times): the known-size kernel that scales the "
                          "FETCH_SIZE / WRITE_SIZE counters")

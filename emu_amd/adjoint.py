@@ -47,8 +47,16 @@ class AdjointProblem:
     """Fixed-order particle data and scratch for forward/adjoint sweeps."""
 
     def __init__(self, emu, parts, dt: float, nsteps: int, checkpoint_every: int | None = None,
-                 segment: str = "keep", memory_budget: float | None = None, resort_every: int = 4):
+                 segment: str = "keep", memory_budget: float | None = None, resort_every: int = 4,
+                 precision: str = "fp64"):
         self.emu = emu
+        if precision not in ("fp64", "fp32"):
+            raise ValueError("precision is 'fp64' or 'fp32'")
+        # fp32: the FP32-bulk form (float flavour arrays, particle constants and
+        # adjoints; FP64 positions, deposit sums, Renormalize and lam_nbar)
+        self.precision = precision
+        self.dtype = torch.float64 if precision == "fp64" else torch.float32
+        self._sfx = "" if precision == "fp64" else "_f32"
         self.np = parts.np
         self.nh = emu.nh
         self.dt = float(dt)
@@ -60,7 +68,9 @@ class AdjointProblem:
         if segment not in ("keep", "store", "recompute"):
             raise ValueError("segment is 'keep', 'store' or 'recompute'")
         self.segment = segment
-        self.keep_doubles = int(lib().emu_adj_step_keep_doubles(emu.ctx, self.np))
+        self.keep_doubles = int(self._fn("emu_adj_step_keep_doubles")(emu.ctx, self.np))
+        self.step_work_doubles = int(self._fn("emu_adj_step_work_doubles")(emu.ctx, self.np))
+        self.work_doubles = int(lib().emu_adj_work_doubles(emu.ctx, self.np)) if precision == "fp64" else 0
         self._keeps = None
         need = self.footprint_bytes()
         if memory_budget is not None and need > memory_budget:
@@ -70,49 +80,74 @@ class AdjointProblem:
                            f"{memory_budget / 2**30:.1f} GiB (fewer checkpoints or segment='recompute')")
         K = emu.K
         r = parts.rdata
+        dt_ = self.dtype
         self.pos0 = parts.pos.clone()
-        self.pup = r[K["pupx"]:K["pupt"] + 1].clone()
-        self.Nn_base = torch.stack([r[K["N"]], r[K["Nbar"]]]).clone()
-        self.Ll = torch.stack([r[K["L"]], r[K["Lbar"]]]).clone()
+        self.pup = r[K["pupx"]:K["pupt"] + 1].to(dt_, copy=True)
+        Nn = torch.stack([r[K["N"]], r[K["Nbar"]]])
+        # fp32: the weights times a power of two that puts the largest in [1, 2)
+        # (FP32 cannot hold N = n dV of large cells, nor the objective's
+        # adjoints, which grow as N^2); the library takes it back out of the
+        # evolution, and J and its gradients come back divided by its square
+        self.wscale = 1.0
+        if precision == "fp32" and self.np > 0:
+            nmax = float(Nn.abs().max())
+            if nmax > 0 and math.isfinite(nmax):
+                self.wscale = 2.0 ** -math.floor(math.log2(nmax))
+        self.Nn_base = (Nn * self.wscale).to(dt_)
+        self.Ll = torch.stack([r[K["L"]], r[K["Lbar"]]]).to(dt_)
         f = K["f00_Re"]
         fb = K["f00_Rebar"]
-        self.f0 = torch.cat([r[f:f + self.nh], r[fb:fb + self.nh]]).clone()
-        nwork = lib().emu_adj_work_doubles(emu.ctx, self.np)
-        self.work = torch.empty(nwork, dtype=torch.float64, device=emu.device)
-        self.step_work = torch.empty(lib().emu_adj_step_work_doubles(emu.ctx, self.np), dtype=torch.float64,
-                                     device=emu.device)
+        self.f0 = torch.cat([r[f:f + self.nh], r[fb:fb + self.nh]]).to(dt_)
+        # the per-RHS calls' work (FP64 only); the objective and the step calls use step_work
+        self.work = (torch.empty(self.work_doubles, dtype=torch.float64, device=emu.device)
+                     if precision == "fp64" else None)
+        self.step_work = torch.empty(self.step_work_doubles, dtype=torch.float64, device=emu.device)
         self.lam_nbar = torch.zeros(self.np, dtype=torch.float64, device=emu.device)
 
+    def _fn(self, name):
+        return getattr(lib(), name + self._sfx)
+
     def footprint_bytes(self) -> float:
-        """Device bytes of a gradient: the run's checkpoints (flavour state and
-        positions), the segment's step states (segment='store'), the adjoint
-        and its next value, the particle constants and the work arrays."""
-        nh, np_ = self.nh, self.np
-        state = 8.0 * np_ * (2 * nh + 3) + 4.0 * np_  # flavour state, positions, cell order
+        """Device bytes of a gradient: the run's checkpoints (flavour state,
+        FP64 positions, cell order), the segment's held step states and kept
+        stage data, the live flavour states and adjoints, the particle
+        constants and the work arrays."""
+        nh2, np_ = 2 * self.nh, self.np
+        e = 8.0 if self.precision == "fp64" else 4.0
+        state = np_ * (e * nh2 + 8.0 * 3 + 4.0)
         ncp = -(-self.nsteps // self.cp)
         held = ncp + (1 if self.segment == "recompute" else self.cp)
         kept = 8.0 * self.cp * self.keep_doubles if self.segment == "keep" else 0.0
-        ncell = 1
-        for n in self.emu.parms.ncell:
-            ncell *= n
-        work = 8.0 * (5 * ncell * 4 * nh + 11 * nh * np_ + 2 * np_) + 8.0 * (3 * ncell * 4 * nh + nh * np_ + np_)
-        consts = 8.0 * np_ * (3 + 4 + 2 + 2 + 1 + 2 * nh + 8)  # and the segment's reordered pup, Nn, Ll
-        return held * state + kept + 2 * 8.0 * 2 * nh * np_ + consts + work
+        work = 8.0 * (self.step_work_doubles + self.work_doubles)
+        # pos0; pup, Nn, Ll; f0; Nn(ratio); the segment's reordered pup, Nn, Ll; lam_nbar twice
+        consts = np_ * (8.0 * 3 + e * 8 + e * nh2 + e * 2 + e * 8 + 8.0 * 2)
+        live = 4 * e * nh2 * np_  # y and the next y, the adjoint and its next value
+        # a re-sort's new copies of the positions and the flavour state, and the
+        # adjoint's move between orders, live beside the old ones for a moment
+        transient = state + e * nh2 * np_
+        return held * state + kept + work + consts + live + transient
 
     # ---- kernels ---------------------------------------------------------
+    def _fp64_only(self, what):
+        if self.precision != "fp64":
+            raise NotImplementedError(what + " runs in FP64 only (precision='fp64')")
+
     def rhs(self, pos, Nn, Y):
+        self._fp64_only("rhs")
         k = torch.empty_like(Y)
         check(lib().emu_adj_rhs(self.emu.ctx, self.np, _p(pos), _p(self.pup), _p(Nn), _p(Y), _p(k), _p(self.work)),
               "emu_adj_rhs", self.emu.ctx)
         return k
 
     def vjp(self, pos, Nn, Y, lam_k):
+        self._fp64_only("vjp")
         lam_Y = torch.empty_like(Y)
         check(lib().emu_adj_rhs_vjp(self.emu.ctx, self.np, _p(pos), _p(self.pup), _p(Nn), _p(Y), _p(lam_k),
                                     _p(lam_Y), _p(self.lam_nbar), _p(self.work)), "emu_adj_rhs_vjp", self.emu.ctx)
         return lam_Y
 
     def axpy(self, a, x, y, out=None):
+        self._fp64_only("axpy")
         out = torch.empty_like(y) if out is None else out
         check(lib().emu_adj_axpy(self.emu.ctx, x.numel(), float(a), _p(x), _p(y), _p(out)), "emu_adj_axpy",
               self.emu.ctx)
@@ -121,15 +156,18 @@ class AdjointProblem:
     def objective(self, pos, Nn, Y, grad=True):
         J = ctypes.c_double()
         dJ = torch.empty_like(Y) if grad else None
-        check(lib().emu_adj_objective(self.emu.ctx, self.np, _p(pos), _p(Nn), _p(Y), ctypes.byref(J),
-                                      _p(dJ) if grad else None, _p(self.work)), "emu_adj_objective", self.emu.ctx)
+        check(self._fn("emu_adj_objective")(self.emu.ctx, self.np, _p(pos), _p(Nn), _p(Y), ctypes.byref(J),
+                                            _p(dJ) if grad else None, _p(self.step_work)), "emu_adj_objective",
+              self.emu.ctx)
         return J.value, dJ
 
     def advance_positions(self, pos):
+        self._fp64_only("advance_positions")
         check(lib().emu_adj_advance_positions(self.emu.ctx, self.np, _p(pos), _p(self.pup), self.dt),
               "emu_adj_advance_positions", self.emu.ctx)
 
     def renormalize(self, Y):
+        self._fp64_only("renormalize")
         bad = ctypes.c_int64(-1)
         code = lib().emu_adj_renormalize(self.emu.ctx, self.np, _p(Y), _p(self.Ll), ctypes.byref(bad))
         check(code, "emu_adj_renormalize", self.emu.ctx, bad.value)
@@ -166,12 +204,14 @@ class AdjointProblem:
         step_adjoint."""
         pup, Ll = consts if consts is not None else (self.pup, self.Ll)
         out = torch.empty_like(y)
+        ws = (self.wscale,) if self.precision == "fp32" else ()
         if keep is None:
-            check(lib().emu_adj_step(self.emu.ctx, self.np, _p(pos), _p(pup), _p(Nn), _p(Ll), _p(y), _p(out),
-                                     self.dt, _p(self.step_work)), "emu_adj_step", self.emu.ctx)
+            check(self._fn("emu_adj_step")(self.emu.ctx, self.np, _p(pos), _p(pup), _p(Nn), _p(Ll), _p(y), _p(out),
+                                           self.dt, *ws, _p(self.step_work)), "emu_adj_step", self.emu.ctx)
         else:
-            check(lib().emu_adj_step_keep(self.emu.ctx, self.np, _p(pos), _p(pup), _p(Nn), _p(Ll), _p(y), _p(out),
-                                          self.dt, _p(self.step_work), _p(keep)), "emu_adj_step_keep", self.emu.ctx)
+            check(self._fn("emu_adj_step_keep")(self.emu.ctx, self.np, _p(pos), _p(pup), _p(Nn), _p(Ll), _p(y),
+                                                _p(out), self.dt, *ws, _p(self.step_work), _p(keep)),
+                  "emu_adj_step_keep", self.emu.ctx)
         return out
 
     def check_status(self):
@@ -188,14 +228,15 @@ class AdjointProblem:
         which spares the recompute."""
         pup = consts[0] if consts is not None else self.pup
         out = torch.empty_like(lam)
+        ws = (self.wscale,) if self.precision == "fp32" else ()
         if keep is None:
-            check(lib().emu_adj_step_vjp(self.emu.ctx, self.np, _p(pos), _p(pup), _p(Nn), _p(y), _p(lam), _p(out),
-                                         _p(self.lam_nbar), self.dt, _p(self.step_work)), "emu_adj_step_vjp",
-                  self.emu.ctx)
+            check(self._fn("emu_adj_step_vjp")(self.emu.ctx, self.np, _p(pos), _p(pup), _p(Nn), _p(y), _p(lam),
+                                               _p(out), _p(self.lam_nbar), self.dt, *ws, _p(self.step_work)),
+                  "emu_adj_step_vjp", self.emu.ctx)
         else:
-            check(lib().emu_adj_step_vjp_kept(self.emu.ctx, self.np, _p(pos), _p(pup), _p(Nn), _p(y), _p(keep),
-                                              _p(lam), _p(out), _p(self.lam_nbar), self.dt, _p(self.step_work)),
-                  "emu_adj_step_vjp_kept", self.emu.ctx)
+            check(self._fn("emu_adj_step_vjp_kept")(self.emu.ctx, self.np, _p(pos), _p(pup), _p(Nn), _p(y), _p(keep),
+                                                    _p(lam), _p(out), _p(self.lam_nbar), self.dt, *ws,
+                                                    _p(self.step_work)), "emu_adj_step_vjp_kept", self.emu.ctx)
         return out
 
     # ---- cell order ------------------------------------------------------
@@ -210,8 +251,8 @@ class AdjointProblem:
         rows = x.numel() // self.np if self.np else 0
         if out is None:
             out = torch.empty_like(x) if mode == 0 else torch.zeros_like(x)
-        check(lib().emu_adj_permute(self.emu.ctx, rows, self.np, _p(x), _p(idx), _p(out), mode), "emu_adj_permute",
-              self.emu.ctx)
+        fn = lib().emu_adj_permute_f32 if x.dtype == torch.float32 else lib().emu_adj_permute
+        check(fn(self.emu.ctx, rows, self.np, _p(x), _p(idx), _p(out), mode), "emu_adj_permute", self.emu.ctx)
         return out
 
     def _cell_order(self, pos):
@@ -295,7 +336,7 @@ class AdjointProblem:
             y = self.step(pos, consts[1], y, (consts[0], consts[2]))
         J, dJ = self.objective(pos, consts[1], y, grad=keep_checkpoints)
         self.check_status()
-        return J, dJ, cps, Nn, order
+        return J / self.wscale**2, dJ, cps, Nn, order
 
     def adjoint_sweep(self, lam, cps, Nn, lam_order):
         """Backward through the recorded run: lam = dJ/dy_N (in lam_order) ->
@@ -333,14 +374,19 @@ class AdjointProblem:
         self._permute(self.lam_nbar, nbar_order, 2, out=lam_nbar)
         self.check_status()
         self.lam_nbar.copy_(lam_nbar)
-        return self._permute(lam, lam_order, 1)
+        lam = self._permute(lam, lam_order, 1)
+        # the unscaled gradient outgrows FP32's range: it comes back in FP64
+        return lam if self.wscale == 1.0 else lam.double() / self.wscale**2
+
+    def ratio_gradient(self):
+        """dJ/dratio from the nubar-weight adjoint of the last sweep."""
+        return (self.lam_nbar * self.Nn_base[1].double()).sum() / self.wscale**2
 
     def gradients(self, f0, ratio: float):
         """J, dJ/df0 and dJ/dratio by the adjoint sweep."""
         J, lam, cps, Nn, order = self.run(f0, ratio)
         lam = self.adjoint_sweep(lam, cps, Nn, order)
-        dratio = float((self.lam_nbar * self.Nn_base[1]).sum().item())
-        return J, lam, dratio
+        return J, lam, float(self.ratio_gradient().item())
 
 
 class FlavorObjective(torch.autograd.Function):
@@ -349,7 +395,8 @@ class FlavorObjective(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, f0, ratio, problem: AdjointProblem):
-        J, lam, cps, Nn, order = problem.run(f0.detach(), float(ratio))
+        J, lam, cps, Nn, order = problem.run(f0.detach().to(problem.dtype), float(ratio))
+        ctx.f0_dtype = f0.dtype
         ctx.problem = problem
         ctx.saved = (lam, cps, Nn, order)
         return torch.tensor(J, dtype=torch.float64, device=f0.device)
@@ -359,6 +406,6 @@ class FlavorObjective(torch.autograd.Function):
         p = ctx.problem
         lam, cps, Nn, order = ctx.saved
         grad_f0 = p.adjoint_sweep(lam, cps, Nn, order)
-        grad_ratio = (p.lam_nbar * p.Nn_base[1]).sum()
+        grad_ratio = p.ratio_gradient()
         ctx.saved = None
-        return gJ * grad_f0, gJ * grad_ratio, None
+        return (gJ * grad_f0).to(ctx.f0_dtype), gJ * grad_ratio, None

@@ -94,6 +94,19 @@ _SIGS = [
     ("emu_adj_step_work_doubles", ctypes.c_int64, [_P, ctypes.c_int64]),
     ("emu_adj_step", ctypes.c_int, [_P, ctypes.c_int64, _P, _P, _P, _P, _P, _P, ctypes.c_double, _P]),
     ("emu_adj_step_vjp", ctypes.c_int, [_P, ctypes.c_int64, _P, _P, _P, _P, _P, _P, _P, ctypes.c_double, _P]),
+    ("emu_adj_step_work_doubles_f32", ctypes.c_int64, [_P, ctypes.c_int64]),
+    ("emu_adj_step_f32", ctypes.c_int, [_P, ctypes.c_int64, _P, _P, _P, _P, _P, _P, ctypes.c_double,
+                                        ctypes.c_double, _P]),
+    ("emu_adj_step_vjp_f32", ctypes.c_int, [_P, ctypes.c_int64, _P, _P, _P, _P, _P, _P, _P, ctypes.c_double,
+                                            ctypes.c_double, _P]),
+    ("emu_adj_step_keep_doubles_f32", ctypes.c_int64, [_P, ctypes.c_int64]),
+    ("emu_adj_step_keep_f32", ctypes.c_int, [_P, ctypes.c_int64, _P, _P, _P, _P, _P, _P, ctypes.c_double,
+                                             ctypes.c_double, _P, _P]),
+    ("emu_adj_step_vjp_kept_f32", ctypes.c_int, [_P, ctypes.c_int64, _P, _P, _P, _P, _P, _P, _P, _P,
+                                                 ctypes.c_double, ctypes.c_double, _P]),
+    ("emu_adj_objective_f32", ctypes.c_int, [_P, ctypes.c_int64, _P, _P, _P, ctypes.POINTER(ctypes.c_double), _P,
+                                             _P]),
+    ("emu_adj_permute_f32", ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int64, _P, _P, _P, ctypes.c_int]),
     ("emu_adj_step_keep_doubles", ctypes.c_int64, [_P, ctypes.c_int64]),
     ("emu_adj_step_keep", ctypes.c_int, [_P, ctypes.c_int64, _P, _P, _P, _P, _P, _P, ctypes.c_double, _P, _P]),
     ("emu_adj_step_vjp_kept", ctypes.c_int, [_P, ctypes.c_int64, _P, _P, _P, _P, _P, _P, _P, _P, ctypes.c_double,

@@ -67,13 +67,17 @@ template <typename F> __device__ __forceinline__ void for_stencil(const Geom& g,
     }
 }
 
-struct AdjIn {
+// Particle inputs; T is the storage of the flavour arrays and particle
+// constants (double, or float for the FP32-bulk step calls), positions stay
+// double.
+template <typename T> struct AdjInT {
     int64_t np;
     const double* pos; // [3][np]
-    const double* pup; // [4][np]
-    const double* Nn;  // [2][np]
-    const double* Y;   // [2 NH][np]
+    const T* pup;      // [4][np]
+    const T* Nn;       // [2][np]
+    const T* Y;        // [2 NH][np]
 };
+using AdjIn = AdjInT<double>;
 
 template <int NF>
 __global__ void k_fill_base(double* grid, int64_t ncell, double m00)
@@ -137,22 +141,22 @@ __global__ __launch_bounds__(256) void k_rhs(AdjIn in, Geom g, const int* perm, 
 }
 
 // Hermitian component vector of Lam: off-diagonals halved.
-template <int NF> __device__ __forceinline__ void lam_matrix(const double* lam, double* L)
+template <int NF, typename T> __device__ __forceinline__ void lam_matrix(const T* lam, T* L)
 {
     for (int c = 0; c < NH<NF>; c++) L[c] = lam[c];
     for (int a = 0; a < NF; a++)
         for (int b = a + 1; b < NF; b++) {
-            L[herm_re<NF>(a, b)] *= 0.5;
-            L[herm_im<NF>(a, b)] *= 0.5;
+            L[herm_re<NF>(a, b)] *= T(0.5);
+            L[herm_im<NF>(a, b)] *= T(0.5);
         }
 }
 
-template <int NF> __device__ __forceinline__ void double_offdiag(double* x)
+template <int NF, typename T> __device__ __forceinline__ void double_offdiag(T* x)
 {
     for (int a = 0; a < NF; a++)
         for (int b = a + 1; b < NF; b++) {
-            x[herm_re<NF>(a, b)] *= 2.0;
-            x[herm_im<NF>(a, b)] *= 2.0;
+            x[herm_re<NF>(a, b)] *= T(2);
+            x[herm_im<NF>(a, b)] *= T(2);
         }
 }
 
@@ -203,9 +207,8 @@ template <typename F> __device__ __forceinline__ void for_each_wave_key(bool act
     }
 }
 
-__global__ void k_cell_keys(AdjIn in, Geom g, int* key, int* counts)
+__global__ void k_cell_keys(int64_t np, const double* pos, Geom g, int* key, int* counts)
 {
-    const int64_t np = in.np;
     const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     const bool act = p < np;
     const int lane = threadIdx.x & 63;
@@ -216,7 +219,7 @@ __global__ void k_cell_keys(AdjIn in, Geom g, int* key, int* counts)
         for (int d = 0; d < 3; d++) {
             int idx;
             double w[3];
-            shape3((in.pos[d * np + p] - g.plo[d]) * g.dxi[d], g.order[d], idx, w);
+            shape3((pos[d * np + p] - g.plo[d]) * g.dxi[d], g.order[d], idx, w);
             c[d] = pmod(idx, g.n[d]);
         }
         k = (c[2] * g.n[1] + c[1]) * g.n[0] + c[0];
@@ -289,8 +292,8 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-template <int NF>
-__global__ __launch_bounds__(256) void k_obj_deposit(AdjIn in, Geom g, double* g2)
+template <int NF, typename T>
+__global__ __launch_bounds__(256) void k_obj_deposit(AdjInT<T> in, Geom g, double* g2)
 {
     const int64_t np = in.np;
     const int lane = threadIdx.x & 63;
@@ -310,8 +313,8 @@ __global__ __launch_bounds__(256) void k_obj_deposit(AdjIn in, Geom g, double* g
                 c[d] = pmod(idx, g.n[d]);
             }
             const double N = in.Nn[p];
-            re = N * in.Y[(int64_t)herm_re<NF>(0, 1) * np + p];
-            im = N * in.Y[(int64_t)herm_im<NF>(0, 1) * np + p];
+            re = N * double(in.Y[(int64_t)herm_re<NF>(0, 1) * np + p]);
+            im = N * double(in.Y[(int64_t)herm_im<NF>(0, 1) * np + p]);
         }
         const int key = (c[2] * g.n[1] + c[1]) * g.n[0] + c[0];
         for_each_wave_key(act, key, [&](int k, unsigned long long m, int lead) {
@@ -351,8 +354,8 @@ __global__ void k_obj_sum(const double* g2, int64_t ncell, double* J)
     if ((threadIdx.x & 63) == 0) unsafeAtomicAdd(J, s);
 }
 
-template <int NF>
-__global__ void k_obj_grad(AdjIn in, Geom g, const double* g2, double* dJdY)
+template <int NF, typename T>
+__global__ void k_obj_grad(AdjInT<T> in, Geom g, const double* g2, T* dJdY)
 {
     const int64_t np = in.np;
     for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
@@ -363,9 +366,9 @@ __global__ void k_obj_grad(AdjIn in, Geom g, const double* g2, double* dJdY)
             im += w * g2[2 * cell + 1];
         });
         const double N = in.Nn[p];
-        for (int c = 0; c < 2 * NH<NF>; c++) dJdY[(int64_t)c * np + p] = 0.0;
-        dJdY[(int64_t)herm_re<NF>(0, 1) * np + p] = 2.0 * N * re;
-        dJdY[(int64_t)herm_im<NF>(0, 1) * np + p] = 2.0 * N * im;
+        for (int c = 0; c < 2 * NH<NF>; c++) dJdY[(int64_t)c * np + p] = T(0);
+        dJdY[(int64_t)herm_re<NF>(0, 1) * np + p] = T(2.0 * N * re);
+        dJdY[(int64_t)herm_im<NF>(0, 1) * np + p] = T(2.0 * N * im);
     }
 }
 
@@ -432,10 +435,10 @@ __device__ __forceinline__ int wave_cell(int64_t ncell, int blocks_per_xcd)
 // stencils): its particles share the 27 targets, so D[t][q] = sum_p W[p][t]
 // R[p][q] runs on f64 16x16x4 MFMAs over all of them, 64 at a time, and is
 // added to the grid once per cell, one atomic per value.
-template <int NF, int WX, int WY, int WZ, int KIND>
-__global__ __launch_bounds__(64 * kCellWaves) void k_cell_deposit(AdjIn in, Geom g, const int* start, const int* perm,
-                                                                  int64_t ncell, int bpx, const double* mu,
-                                                                  double* grid)
+template <int NF, int WX, int WY, int WZ, int KIND, typename T>
+__global__ __launch_bounds__(64 * kCellWaves) void k_cell_deposit(AdjInT<T> in, Geom g, const int* start,
+                                                                  const int* perm, int64_t ncell, int bpx,
+                                                                  const T* mu, double* grid)
 {
     constexpr int nh = NH<NF>, QG = 4 * nh;
     constexpr int NT = WX * WY * WZ, MT = (NT + 15) / 16, NTL = (QG + 15) / 16;
@@ -480,8 +483,8 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_cell_deposit(AdjIn in, Geom
         const bool act = base + lane < end;
         const int64_t pl = perm[act ? base + lane : beg];
         const double pos[3] = {in.pos[pl], in.pos[np + pl], in.pos[2 * np + pl]};
-        const double pt = in.pup[3 * np + pl];
-        const double v[3] = {in.pup[pl] / pt, in.pup[np + pl] / pt, in.pup[2 * np + pl] / pt};
+        const T pt = in.pup[3 * np + pl];
+        const T v[3] = {in.pup[pl] / pt, in.pup[np + pl] / pt, in.pup[2 * np + pl] / pt};
         double wd[3][3];
 #pragma unroll
         for (int d = 0; d < 3; d++) {
@@ -492,19 +495,21 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_cell_deposit(AdjIn in, Geom
         for (int d = 0; d < 3; d++)
 #pragma unroll
             for (int e = 0; e < 3; e++) Wf[lane * 9 + 3 * d + e] = wd[d][e];
+        // the sources in T (FP32-rounded in the FP32 form, as its fused
+        // deposit's), summed in FP64 on the matrix cores
         if constexpr (KIND == 0) {
-            const double N = in.Nn[pl], Nb = in.Nn[np + pl];
+            const T N = in.Nn[pl], Nb = in.Nn[np + pl];
 #pragma unroll
             for (int c = 0; c < nh; c++) {
-                const double f = in.Y[(int64_t)c * np + pl], fb = in.Y[(int64_t)(nh + c) * np + pl];
-                Sr[lane * NSR + c] = act ? (is_im_comp<NF>(c) ? N * f + Nb * fb : N * f - Nb * fb) : 0.0;
+                const T f = in.Y[(int64_t)c * np + pl], fb = in.Y[(int64_t)(nh + c) * np + pl];
+                Sr[lane * NSR + c] = act ? double(is_im_comp<NF>(c) ? N * f + Nb * fb : N * f - Nb * fb) : 0.0;
             }
         } else {
 #pragma unroll
-            for (int c = 0; c < nh; c++) Sr[lane * NSR + c] = act ? mu[(int64_t)c * np + pl] : 0.0;
+            for (int c = 0; c < nh; c++) Sr[lane * NSR + c] = act ? double(mu[(int64_t)c * np + pl]) : 0.0;
         }
 #pragma unroll
-        for (int d = 0; d < 3; d++) Sr[lane * NSR + nh + d] = KIND == 0 ? v[d] : -v[d];
+        for (int d = 0; d < 3; d++) Sr[lane * NSR + nh + d] = double(KIND == 0 ? v[d] : -v[d]);
         Sr[lane * NSR + nh + 3] = 1.0;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -558,12 +563,12 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_cell_deposit(AdjIn in, Geom
 // out[c] += sum over the neighbourhood of `cell` (wave-uniform) of
 // w_t (G_t[0][c] + s0 G_t[1][c] + s1 G_t[2][c] + s2 G_t[3][c]), with the
 // particle's shape factors w (shape3's windows) and multipliers s.
-template <int NF>
-__device__ __forceinline__ void gather_cell_box(const Geom& g, int cell, const double* grid_, const double w[3][3],
-                                                const double s[3], double* out)
+template <int NF, typename T>
+__device__ __forceinline__ void gather_cell_box(const Geom& g, int cell, const T* grid_, const T w[3][3], const T s[3],
+                                                T* out)
 {
     constexpr int nh = NH<NF>, QG = 4 * nh;
-    typedef __attribute__((address_space(4))) const double* cdp;
+    typedef __attribute__((address_space(4))) const T* cdp;
     const cdp grid = (cdp)grid_;
     // the neighbourhood's indices are wave-uniform: kept in scalar registers so
     // that the grid rows below are scalar loads (vector loads of one address
@@ -578,19 +583,19 @@ __device__ __forceinline__ void gather_cell_box(const Geom& g, int cell, const d
         lo[d] = g.order[d] > 0 ? 0 : 1;
         hi[d] = g.order[d] > 0 ? 3 : 2;
     }
-    auto pick = [](const double* x, int i) { return i == 0 ? x[0] : (i == 1 ? x[1] : x[2]); };
+    auto pick = [](const T* x, int i) { return i == 0 ? x[0] : (i == 1 ? x[1] : x[2]); };
 #pragma unroll 1
     for (int kk = lo[2]; kk < hi[2]; kk++) {
         const int nz = wrap(cz + kk - 1, g.n[2]);
 #pragma unroll 1
         for (int jj = lo[1]; jj < hi[1]; jj++) {
             const int ny = wrap(cy + jj - 1, g.n[1]);
-            const double wyz = pick(w[1], jj) * pick(w[2], kk);
+            const T wyz = pick(w[1], jj) * pick(w[2], kk);
 #pragma unroll 1
             for (int ii = lo[0]; ii < hi[0]; ii++) {
                 const int64_t nb = __builtin_amdgcn_readfirstlane((nz * g.n[1] + ny) * g.n[0] + wrap(cx + ii - 1, g.n[0]));
-                const double wt = pick(w[0], ii) * wyz;
-                const double a[4] = {wt, wt * s[0], wt * s[1], wt * s[2]};
+                const T wt = pick(w[0], ii) * wyz;
+                const T a[4] = {wt, wt * s[0], wt * s[1], wt * s[2]};
                 const cdp G = grid + nb * QG;
 #pragma unroll
                 for (int v = 0; v < 4; v++)
@@ -601,39 +606,44 @@ __device__ __forceinline__ void gather_cell_box(const Geom& g, int cell, const d
     }
 }
 
-// The particle's shape factors and direction (positions frozen within a step).
-__device__ __forceinline__ void particle_shape(const AdjIn& in, int64_t p, const Geom& g, double w[3][3], double v[3])
+// The particle's shape factors (from the FP64 positions, rounded to T) and
+// direction (positions frozen within a step).
+template <typename T>
+__device__ __forceinline__ void particle_shape(const AdjInT<T>& in, int64_t p, const Geom& g, T w[3][3], T v[3])
 {
     const int64_t np = in.np;
     const double x[3] = {in.pos[p], in.pos[np + p], in.pos[2 * np + p]};
-    const double pt = in.pup[3 * np + p];
+    const T pt = in.pup[3 * np + p];
     v[0] = in.pup[p] / pt;
     v[1] = in.pup[np + p] / pt;
     v[2] = in.pup[2 * np + p] / pt;
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         int idx;
-        shape3((x[d] - g.plo[d]) * g.dxi[d], g.order[d], idx, w[d]);
+        double wd[3];
+        shape3((x[d] - g.plo[d]) * g.dxi[d], g.order[d], idx, wd);
+#pragma unroll
+        for (int e = 0; e < 3; e++) w[d][e] = T(wd[e]);
     }
 }
 
 // H/hbar and Hbar/hbar of a particle of the wave's cell (as hamiltonians()).
-template <int NF>
-__device__ __forceinline__ void cell_hamiltonians(const AdjIn& in, int64_t p, const Geom& g, int cell,
-                                                  const double* grid, const HamParams& hp, double* H, double* Hb)
+template <int NF, typename T>
+__device__ __forceinline__ void cell_hamiltonians(const AdjInT<T>& in, int64_t p, const Geom& g, int cell,
+                                                  const T* grid, const HamParams& hp, T* H, T* Hb)
 {
     constexpr int nh = NH<NF>;
-    double w[3][3], v[3];
+    T w[3][3], v[3];
     particle_shape(in, p, g, w, v);
-    const double s[3] = {-v[0], -v[1], -v[2]};
-    double inside[nh];
+    const T s[3] = {-v[0], -v[1], -v[2]};
+    T inside[nh];
 #pragma unroll
-    for (int c = 0; c < nh; c++) inside[c] = 0.0;
-    gather_cell_box<NF>(g, cell, grid, w, s, inside);
-    const double pt = in.pup[3 * in.np + p];
+    for (int c = 0; c < nh; c++) inside[c] = T(0);
+    gather_cell_box<NF, T>(g, cell, grid, w, s, inside);
+    const T pt = in.pup[3 * in.np + p];
 #pragma unroll
     for (int c = 0; c < nh; c++) {
-        const double hv = hp.vac[c] / pt, hs = hp.coef * inside[c];
+        const T hv = T(hp.vac[c]) / pt, hs = T(hp.coef) * inside[c];
         H[c] = hv + hs;
         Hb[c] = is_im_comp<NF>(c) ? hs - hv : hv - hs;
     }
@@ -645,12 +655,12 @@ __device__ __forceinline__ void cell_hamiltonians(const AdjIn& in, int64_t p, co
 //  S = 4 finishes acc = y_{n+1}, renormalizes it (status deferred into err)
 //  and advances the particle's position.
 //  STORE 1 (the VJP's recompute): S = 1..3 write Y_{S+1} into ynext only.
-template <int NF, int S, int STORE>
-__global__ __launch_bounds__(64 * kCellWaves) void k_fwd_stage(AdjIn in, Geom g, const int* start, const int* perm,
-                                                               int64_t ncell, int bpx, const double* grid,
-                                                               HamParams hp, const double* y0, double* ynext,
-                                                               double* acc, const double* Ll, double maxError,
-                                                               double dt, double* pos_out, unsigned long long* err)
+template <int NF, int S, int STORE, typename T>
+__global__ __launch_bounds__(64 * kCellWaves) void k_fwd_stage(AdjInT<T> in, Geom g, const int* start, const int* perm,
+                                                               int64_t ncell, int bpx, const T* grid, HamParams hp,
+                                                               const T* y0, T* ynext, T* acc, const T* Ll,
+                                                               double maxError, double dt, double* pos_out,
+                                                               unsigned long long* err)
 {
     constexpr int nh = NH<NF>, S2 = 2 * nh;
     const int64_t np = in.np;
@@ -658,46 +668,46 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_fwd_stage(AdjIn in, Geom g,
     if (cell < 0) return;
     const int beg = __builtin_amdgcn_readfirstlane(start[cell]), end = __builtin_amdgcn_readfirstlane(start[cell + 1]);
     const int lane = threadIdx.x & 63;
-    const double a_next = S == 3 ? dt * 1.0 : dt * 0.5;
-    const double a_acc = S == 1 || S == 4 ? dt * (1. / 6.) : dt * (1. / 3.);
+    const T a_next = T(S == 3 ? dt * 1.0 : dt * 0.5);
+    const T a_acc = T(S == 1 || S == 4 ? dt * (1. / 6.) : dt * (1. / 3.));
     for (int base = beg; base < end; base += 64) {
         const bool act = base + lane < end;
         const int64_t p = perm[act ? base + lane : beg];
-        double H[nh], Hb[nh], f[nh], k[S2];
-        cell_hamiltonians<NF>(in, p, g, cell, grid, hp, H, Hb);
+        T H[nh], Hb[nh], f[nh], k[S2];
+        cell_hamiltonians<NF, T>(in, p, g, cell, grid, hp, H, Hb);
         if (!act) continue;
 #pragma unroll
         for (int c = 0; c < nh; c++) f[c] = in.Y[(int64_t)c * np + p];
-        commutator_rhs<NF, double>(H, f, k, 1.0);
+        commutator_rhs<NF, T>(H, f, k, T(1));
 #pragma unroll
         for (int c = 0; c < nh; c++) f[c] = in.Y[(int64_t)(nh + c) * np + p];
-        commutator_rhs<NF, double>(Hb, f, k + nh, 1.0);
+        commutator_rhs<NF, T>(Hb, f, k + nh, T(1));
         {
 #pragma clang fp contract(off)
             if constexpr (S < 4) {
 #pragma unroll
                 for (int c = 0; c < S2; c++) {
-                    const double y = y0[(int64_t)c * np + p];
+                    const T y = y0[(int64_t)c * np + p];
                     ynext[(int64_t)c * np + p] = a_next * k[c] + y;
                     if constexpr (STORE == 0)
                         acc[(int64_t)c * np + p] = a_acc * k[c] + (S == 1 ? y : acc[(int64_t)c * np + p]);
                 }
             } else {
+                // y_{n+1} in T, then Renormalize in FP64 (FlavoredNeutrinoContainer.cpp:89-113):
+                // nu, then nubar unless nu failed; a failing half is left as it is
                 double y[S2];
 #pragma unroll
-                for (int c = 0; c < S2; c++) y[c] = a_acc * k[c] + acc[(int64_t)c * np + p];
-                // Renormalize (FlavoredNeutrinoContainer.cpp:89-113): nu, then nubar
-                // unless nu failed; a failing half is left as it is
-                int e = renorm_fp64<NF>(y, Ll[p], maxError);
-                if (e == EMU_OK) e = renorm_fp64<NF>(y + nh, Ll[np + p], maxError);
+                for (int c = 0; c < S2; c++) y[c] = double(a_acc * k[c] + acc[(int64_t)c * np + p]);
+                int e = renorm_fp64<NF>(y, double(Ll[p]), maxError);
+                if (e == EMU_OK) e = renorm_fp64<NF>(y + nh, double(Ll[np + p]), maxError);
                 if (e != EMU_OK) atomicMin(err, status_word(p, e));
 #pragma unroll
-                for (int c = 0; c < S2; c++) acc[(int64_t)c * np + p] = y[c];
+                for (int c = 0; c < S2; c++) acc[(int64_t)c * np + p] = T(y[c]);
                 double x[3], v[3], nx[3];
 #pragma unroll
                 for (int d = 0; d < 3; d++) {
                     x[d] = in.pos[(int64_t)d * np + p];
-                    v[d] = in.pup[(int64_t)d * np + p] / in.pup[3 * np + p] * phys::c;
+                    v[d] = double(in.pup[(int64_t)d * np + p]) / double(in.pup[3 * np + p]) * phys::c;
                 }
                 int nc;
                 end_position(x, v, dt, g, nx, nc);
@@ -708,95 +718,204 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_fwd_stage(AdjIn in, Geom g,
     }
 }
 
-// VJP of stage S, part 1: lambda_k = alpha lambda + beta lY_{S+1} (the
-// reverse of the stage algebra; beta = 0 at stage 4), then the commutator
-// adjoints: lY_S (commutator part) and mu, as k_vjp_comm.
-template <int NF>
-__global__ __launch_bounds__(64 * kCellWaves) void k_vjp_stage_comm(AdjIn in, Geom g, const int* start,
-                                                                    const int* perm, int64_t ncell, int bpx,
-                                                                    const double* grid, HamParams hp,
-                                                                    const double* lam, const double* lY_prev,
-                                                                    double alpha, double beta, double* lY, double* mu)
+// One reverse stage boundary in one pass, one cell per wave (s = 3 .. -1
+// over the launches; G and C say which halves run):
+//  G (stage s+1's gather back): lY_{s+1} = its commutator part (in lY) plus
+//    the gather of gadj_{s+1} (N lq for nu, sigma Nbar lq for nubar), then
+//    lam_out = lY_{s+1} + (lam at the first G, else lam_out) and
+//    lam_Nbar += sigma fbar_{s+1} . lq;
+//  C (stage s's commutator): lambda_k = alpha lam + beta lY_{s+1}, the
+//    commutator adjoints (lY_s's commutator part, into lY in place) and mu_s,
+//    which the wave deposits onto gadj_s on the f64 matrix cores (as
+//    k_cell_deposit with KIND 1) without writing mu out.
+// Against separate commutator, deposit and gather launches per stage, this
+// form reads each particle once per boundary and holds one stage adjoint
+// array instead of two and no mu array.  Every
+// stencil width runs the 3 x 3 x 3 target set (the weights of an order-0
+// dimension are (0, 1, 0), so its other targets receive zeros).
+template <int NF, typename T, bool G, bool C>
+__global__ __launch_bounds__(64 * kCellWaves) void k_vjp_fused(AdjInT<T> in, Geom g, const int* start,
+                                                               const int* perm, int64_t ncell, int bpx,
+                                                               const T* Ynext, const T* Ycur, const T* gadj_in,
+                                                               const T* grid, double* gadj_out, HamParams hp,
+                                                               const T* lam, T* lY, T* lam_out, double* lamNbar,
+                                                               int first, double alpha, double beta)
 {
-    constexpr int nh = NH<NF>;
-    const int64_t np = in.np;
+    constexpr int nh = NH<NF>, QG = 4 * nh;
+    constexpr int NT = 27, MT = 2, NTL = (QG + 15) / 16;
+    constexpr int NSR = nh + 4;            // mu, -v, 1
+    constexpr int REG = 64 * 9 + 64 * NSR; // doubles per wave: weights, sources
+    __shared__ double lds[C ? kCellWaves * REG : 1];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double* Wf = lds + wave * REG;
+    double* Sr = Wf + 64 * 9;
     const int cell = wave_cell(ncell, bpx);
     if (cell < 0) return;
     const int beg = __builtin_amdgcn_readfirstlane(start[cell]), end = __builtin_amdgcn_readfirstlane(start[cell + 1]);
-    const int lane = threadIdx.x & 63;
-    for (int base = beg; base < end; base += 64) {
-        const bool act = base + lane < end;
-        const int64_t p = perm[act ? base + lane : beg];
-        double H[nh], Hb[nh];
-        cell_hamiltonians<NF>(in, p, g, cell, grid, hp, H, Hb);
-        if (!act) continue;
-        double gH[2][nh];
+    if (beg == end) return;
+    const int64_t np = in.np;
+    const T al = T(alpha), be = T(beta);
+    int tdx[MT][3];
 #pragma unroll
-        for (int bar = 0; bar < 2; bar++) {
-            double lk[nh], L[nh], f[nh], gy[nh];
-#pragma unroll
-            for (int c = 0; c < nh; c++) {
-#pragma clang fp contract(off)
-                const int64_t o = (int64_t)(bar * nh + c) * np + p;
-                const double t = alpha * lam[o];
-                lk[c] = lY_prev ? beta * lY_prev[o] + t : t;
-                f[c] = in.Y[o];
-            }
-            lam_matrix<NF>(lk, L);
-            commutator_rhs<NF, double>(L, bar ? Hb : H, gy, 1.0); // -i[Lam, A]
-            double_offdiag<NF>(gy);
-#pragma unroll
-            for (int c = 0; c < nh; c++) lY[(int64_t)(bar * nh + c) * np + p] = gy[c];
-            commutator_rhs<NF, double>(f, L, gH[bar], 1.0);       // -i[B, Lam]
-            double_offdiag<NF>(gH[bar]);
-        }
-#pragma unroll
-        for (int c = 0; c < nh; c++)
-            mu[(int64_t)c * np + p] = hp.coef * (gH[0][c] + (is_im_comp<NF>(c) ? gH[1][c] : -gH[1][c]));
+    for (int m = 0; m < MT; m++) {
+        const int t = 16 * m + (lane & 15);
+        const int tc = t < NT ? t : NT - 1;
+        tdx[m][0] = tc % 3;
+        tdx[m][1] = 3 + (tc / 3) % 3;
+        tdx[m][2] = 6 + tc / 9;
     }
-}
-
-// VJP of stage S, part 3: the gather of the grid adjoint into lY_S (as
-// k_vjp_gather), then the running sum lam_out = lY_S + (lam, at the first
-// reversed stage, else lam_out).
-template <int NF>
-__global__ __launch_bounds__(64 * kCellWaves) void k_vjp_stage_gather(AdjIn in, Geom g, const int* start,
-                                                                      const int* perm, int64_t ncell, int bpx,
-                                                                      const double* gadj, double* lY,
-                                                                      double* lamNbar, const double* lam,
-                                                                      double* lam_out, int first)
-{
-    constexpr int nh = NH<NF>;
-    const int64_t np = in.np;
-    const int cell = wave_cell(ncell, bpx);
-    if (cell < 0) return;
-    const int beg = __builtin_amdgcn_readfirstlane(start[cell]), end = __builtin_amdgcn_readfirstlane(start[cell + 1]);
-    const int lane = threadIdx.x & 63;
+    int bi1[NTL], bi2[NTL];
+#pragma unroll
+    for (int n = 0; n < NTL; n++) {
+        const int q = 16 * n + (lane & 15);
+        const int f = q / nh, c = q % nh;
+        bi1[n] = q < QG ? c : nh + 3;
+        bi2[n] = q < QG ? (f == 0 ? nh + 3 : nh + f - 1) : nh + 3;
+    }
+    mf_d4 acc[MT][NTL];
+#pragma unroll
+    for (int m = 0; m < MT; m++)
+#pragma unroll
+        for (int n = 0; n < NTL; n++) acc[m][n] = mf_d4{0.0, 0.0, 0.0, 0.0};
     for (int base = beg; base < end; base += 64) {
         const bool act = base + lane < end;
         const int64_t p = perm[act ? base + lane : beg];
-        double w[3][3], v[3];
-        particle_shape(in, p, g, w, v);
-        double lq[nh];
+        // shape factors (FP64 for the deposit, T for the gathers) and direction
+        const double x[3] = {in.pos[p], in.pos[np + p], in.pos[2 * np + p]};
+        double wd[3][3];
+        T w[3][3];
 #pragma unroll
-        for (int c = 0; c < nh; c++) lq[c] = 0.0;
-        gather_cell_box<NF>(g, cell, gadj, w, v, lq);
-        if (!act) continue;
-        const double N = in.Nn[p], Nb = in.Nn[np + p];
-        double ln = 0.0;
+        for (int d = 0; d < 3; d++) {
+            int idx;
+            shape3((x[d] - g.plo[d]) * g.dxi[d], g.order[d], idx, wd[d]);
 #pragma unroll
-        for (int c = 0; c < nh; c++) {
-#pragma clang fp contract(off)
-            const double sg = is_im_comp<NF>(c) ? 1.0 : -1.0;
-            const int64_t o0 = (int64_t)c * np + p, o1 = (int64_t)(nh + c) * np + p;
-            const double l0 = lY[o0] + N * lq[c], l1 = lY[o1] + sg * Nb * lq[c];
-            lY[o0] = l0;
-            lY[o1] = l1;
-            lam_out[o0] = 1.0 * l0 + (first ? lam[o0] : lam_out[o0]);
-            lam_out[o1] = 1.0 * l1 + (first ? lam[o1] : lam_out[o1]);
-            ln += sg * in.Y[o1] * lq[c];
+            for (int e = 0; e < 3; e++) w[d][e] = T(wd[d][e]);
         }
-        if (lamNbar) lamNbar[p] += ln;
+        const T pt = in.pup[3 * np + p];
+        const T v[3] = {in.pup[p] / pt, in.pup[np + p] / pt, in.pup[2 * np + p] / pt};
+        T lYn[2 * nh];
+        if constexpr (G) {
+            T lq[nh];
+#pragma unroll
+            for (int c = 0; c < nh; c++) lq[c] = T(0);
+            gather_cell_box<NF, T>(g, cell, gadj_in, w, v, lq);
+            if (act) {
+                const T N = in.Nn[p], Nb = in.Nn[np + p];
+                T ln = T(0);
+#pragma unroll
+                for (int c = 0; c < nh; c++) {
+#pragma clang fp contract(off)
+                    const T sg = is_im_comp<NF>(c) ? T(1) : T(-1);
+                    const int64_t o0 = (int64_t)c * np + p, o1 = (int64_t)(nh + c) * np + p;
+                    lYn[c] = lY[o0] + N * lq[c];
+                    lYn[nh + c] = lY[o1] + sg * Nb * lq[c];
+                    lam_out[o0] = T(1) * lYn[c] + (first ? lam[o0] : lam_out[o0]);
+                    lam_out[o1] = T(1) * lYn[nh + c] + (first ? lam[o1] : lam_out[o1]);
+                    ln += sg * Ynext[o1] * lq[c];
+                }
+                if (lamNbar) lamNbar[p] += double(ln);
+            }
+        }
+        if constexpr (C) {
+            T H[nh], Hb[nh];
+            {
+                const T sv[3] = {-v[0], -v[1], -v[2]};
+                T inside[nh];
+#pragma unroll
+                for (int c = 0; c < nh; c++) inside[c] = T(0);
+                gather_cell_box<NF, T>(g, cell, grid, w, sv, inside);
+#pragma unroll
+                for (int c = 0; c < nh; c++) {
+                    const T hv = T(hp.vac[c]) / pt, hs = T(hp.coef) * inside[c];
+                    H[c] = hv + hs;
+                    Hb[c] = is_im_comp<NF>(c) ? hs - hv : hv - hs;
+                }
+            }
+            T mu[nh];
+#pragma unroll
+            for (int c = 0; c < nh; c++) mu[c] = T(0);
+            if (act) {
+                T gH[2][nh];
+#pragma unroll
+                for (int bar = 0; bar < 2; bar++) {
+                    T lk[nh], L[nh], f[nh], gy[nh];
+#pragma unroll
+                    for (int c = 0; c < nh; c++) {
+#pragma clang fp contract(off)
+                        const int64_t o = (int64_t)(bar * nh + c) * np + p;
+                        const T t = al * lam[o];
+                        if constexpr (G) lk[c] = be * lYn[bar * nh + c] + t;
+                        else lk[c] = t;
+                        f[c] = Ycur[o];
+                    }
+                    lam_matrix<NF, T>(lk, L);
+                    commutator_rhs<NF, T>(L, bar ? Hb : H, gy, T(1)); // -i[Lam, A]
+                    double_offdiag<NF, T>(gy);
+#pragma unroll
+                    for (int c = 0; c < nh; c++) lY[(int64_t)(bar * nh + c) * np + p] = gy[c];
+                    commutator_rhs<NF, T>(f, L, gH[bar], T(1));       // -i[B, Lam]
+                    double_offdiag<NF, T>(gH[bar]);
+                }
+#pragma unroll
+                for (int c = 0; c < nh; c++)
+                    mu[c] = T(hp.coef) * (gH[0][c] + (is_im_comp<NF>(c) ? gH[1][c] : -gH[1][c]));
+            }
+            // stage mu with its weights and (-v, 1), then D[t][q] += W^T R on the matrix cores
+#pragma unroll
+            for (int d = 0; d < 3; d++)
+#pragma unroll
+                for (int e = 0; e < 3; e++) Wf[lane * 9 + 3 * d + e] = wd[d][e];
+#pragma unroll
+            for (int c = 0; c < nh; c++) Sr[lane * NSR + c] = double(mu[c]);
+#pragma unroll
+            for (int d = 0; d < 3; d++) Sr[lane * NSR + nh + d] = double(-v[d]);
+            Sr[lane * NSR + nh + 3] = 1.0;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const int nks = (min(64, end - base) + 3) >> 2;
+            for (int ks = 0; ks < nks; ks++) {
+                const int pp = 4 * ks + (lane >> 4);
+                const double* wf = Wf + pp * 9;
+                const double* sr = Sr + pp * NSR;
+                double A[MT];
+#pragma unroll
+                for (int m = 0; m < MT; m++) A[m] = wf[tdx[m][0]] * wf[tdx[m][1]] * wf[tdx[m][2]];
+#pragma unroll
+                for (int n = 0; n < NTL; n++) {
+                    const double B = sr[bi1[n]] * sr[bi2[n]];
+#pragma unroll
+                    for (int m = 0; m < MT; m++)
+                        acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[m], B, acc[m][n], 0, 0, 0);
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+    }
+    if constexpr (C) {
+        // element r of lane l holds D[16m + 4r + l/16][16n + l%16]
+        const int kx = __builtin_amdgcn_readfirstlane(cell % g.n[0]);
+        const int ky = __builtin_amdgcn_readfirstlane((cell / g.n[0]) % g.n[1]);
+        const int kz = __builtin_amdgcn_readfirstlane(cell / (g.n[0] * g.n[1]));
+        auto wrap = [](int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); };
+#pragma unroll
+        for (int m = 0; m < MT; m++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int t = 16 * m + 4 * r + (lane >> 4);
+                if (t >= NT) continue;
+                const int tx = wrap(kx + t % 3 - 1, g.n[0]);
+                const int ty = wrap(ky + (t / 3) % 3 - 1, g.n[1]);
+                const int tz = wrap(kz + t / 9 - 1, g.n[2]);
+                double* Gd = gadj_out + (((int64_t)tz * g.n[1] + ty) * g.n[0] + tx) * QG;
+#pragma unroll
+                for (int n = 0; n < NTL; n++) {
+                    const int q = 16 * n + (lane & 15);
+                    if (q < QG) unsafeAtomicAdd(&Gd[q], acc[m][n][r]);
+                }
+            }
     }
 }
 
@@ -817,12 +936,14 @@ __global__ void k_zero(double* p, int64_t n)
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) p[t] = 0.0;
 }
 
-HamParams ham_params(const emu_ctx* ctx)
+// wscale: the factor the particle weights N, Nbar carry (1, or a power of two
+// in the FP32 form), which the potential's coefficient takes back out
+HamParams ham_params(const emu_ctx* ctx, double wscale = 1.0)
 {
     HamParams hp;
     const int nh = ctx->p.nf * ctx->p.nf;
     for (int c = 0; c < 9; c++) hp.vac[c] = c < nh ? ctx->M2[c] * phys::c4 / 2.0 / phys::hbar : 0.0;
-    hp.coef = M_SQRT2 * phys::GF * ctx->g.inv_cell_volume / phys::hbar;
+    hp.coef = M_SQRT2 * phys::GF * ctx->g.inv_cell_volume / phys::hbar / wscale;
     return hp;
 }
 
@@ -924,15 +1045,15 @@ static int cell_blocks_per_xcd(int64_t ncell)
 
 // k_cell_deposit for the stencil widths of the grid (3 where the shape order is
 // above 0, else 1), one wave per cell of the cell list
-template <int NF, int KIND>
-static void cell_deposit_nf(emu_ctx* ctx, const AdjIn& in, const CellSort& cs, const double* mu, double* grid)
+template <int NF, int KIND, typename T>
+static void cell_deposit_nf(emu_ctx* ctx, const AdjInT<T>& in, const CellSort& cs, const T* mu, double* grid)
 {
     const int wx = ctx->g.order[0] > 0 ? 3 : 1, wy = ctx->g.order[1] > 0 ? 3 : 1, wz = ctx->g.order[2] > 0 ? 3 : 1;
     const int64_t nc = ncell_of(ctx);
     const int bpx = cell_blocks_per_xcd(nc);
     const int code = (wx == 3) * 4 + (wy == 3) * 2 + (wz == 3);
 #define DEP_LAUNCH(X, Y, Z)                                                                                            \
-    hipLaunchKernelGGL((k_cell_deposit<NF, X, Y, Z, KIND>), dim3(8 * bpx), dim3(64 * kCellWaves), 0, ctx->stream, in, \
+    hipLaunchKernelGGL((k_cell_deposit<NF, X, Y, Z, KIND, T>), dim3(8 * bpx), dim3(64 * kCellWaves), 0, ctx->stream, in, \
                        ctx->g, cs.start, cs.perm, nc, bpx, mu, grid)
     switch (code) {
     case 0: DEP_LAUNCH(1, 1, 1); break;
@@ -947,35 +1068,39 @@ static void cell_deposit_nf(emu_ctx* ctx, const AdjIn& in, const CellSort& cs, c
 #undef DEP_LAUNCH
 }
 
-template <int KIND>
-static void cell_deposit(emu_ctx* ctx, const AdjIn& in, const CellSort& cs, const double* mu, double* grid)
+template <int KIND, typename T>
+static void cell_deposit(emu_ctx* ctx, const AdjInT<T>& in, const CellSort& cs, const T* mu, double* grid)
 {
     if (in.np == 0) return;
-    if (ctx->p.nf == 2) cell_deposit_nf<2, KIND>(ctx, in, cs, mu, grid);
-    else cell_deposit_nf<3, KIND>(ctx, in, cs, mu, grid);
+    if (ctx->p.nf == 2) cell_deposit_nf<2, KIND, T>(ctx, in, cs, mu, grid);
+    else cell_deposit_nf<3, KIND, T>(ctx, in, cs, mu, grid);
 }
 
 // the particles listed by the centre cell of their stencil (positions are
 // frozen within a step, so every deposit of a call shares one list)
-static int cell_sort(emu_ctx* ctx, const AdjIn& in, const CellSort& cs)
+template <typename T> static int cell_sort(emu_ctx* ctx, const AdjInT<T>& in, const CellSort& cs)
 {
     const int64_t np = in.np, nc = ncell_of(ctx);
     const int64_t zero_doubles = (2 * nc + 1 + 1) / 2; // start and cursor are adjacent ints
     hipLaunchKernelGGL(k_zero, dim3(nblk(zero_doubles)), dim3(kB), 0, ctx->stream, (double*)cs.start, zero_doubles);
     const unsigned nb = (unsigned)((np + kB - 1) / kB); // one thread per particle
-    hipLaunchKernelGGL(k_cell_keys, dim3(nb), dim3(kB), 0, ctx->stream, in, ctx->g, cs.key, cs.start);
+    hipLaunchKernelGGL(k_cell_keys, dim3(nb), dim3(kB), 0, ctx->stream, np, in.pos, ctx->g, cs.key, cs.start);
     size_t bytes = cs.tmp_bytes;
     EMU_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(cs.tmp, bytes, cs.start, cs.start, (int)(nc + 1), ctx->stream));
     hipLaunchKernelGGL(k_cell_perm, dim3(nb), dim3(kB), 0, ctx->stream, np, cs.key, cs.start, cs.cursor, cs.perm);
     return EMU_OK;
 }
 
-static void deposit(emu_ctx* ctx, const AdjIn& in, const CellSort& cs, double* grid)
+// the deposit of Y onto the base grid (matter in component 0, carrying the
+// weights' factor wscale like the deposited fields)
+template <typename T>
+static void deposit(emu_ctx* ctx, const AdjInT<T>& in, const CellSort& cs, double* grid, double wscale = 1.0)
 {
     const int64_t ng = ncell_of(ctx) * 4 * ctx->p.nf * ctx->p.nf;
-    if (ctx->p.nf == 2) hipLaunchKernelGGL(k_fill_base<2>, dim3(nblk(ng)), dim3(kB), 0, ctx->stream, grid, ncell_of(ctx), m00_of(ctx));
-    else hipLaunchKernelGGL(k_fill_base<3>, dim3(nblk(ng)), dim3(kB), 0, ctx->stream, grid, ncell_of(ctx), m00_of(ctx));
-    cell_deposit<0>(ctx, in, cs, nullptr, grid);
+    const double m00 = m00_of(ctx) * wscale;
+    if (ctx->p.nf == 2) hipLaunchKernelGGL(k_fill_base<2>, dim3(nblk(ng)), dim3(kB), 0, ctx->stream, grid, ncell_of(ctx), m00);
+    else hipLaunchKernelGGL(k_fill_base<3>, dim3(nblk(ng)), dim3(kB), 0, ctx->stream, grid, ncell_of(ctx), m00);
+    cell_deposit<0, T>(ctx, in, cs, (const T*)nullptr, grid);
 }
 
 extern "C" int emu_adj_rhs(emu_ctx* ctx, int64_t np, const double* pos, const double* pup, const double* Nn,
@@ -1009,29 +1134,48 @@ extern "C" int emu_adj_rhs_vjp(emu_ctx* ctx, int64_t np, const double* pos, cons
     deposit(ctx, in, cs, grid);
     ADJ_DISPATCH(k_vjp_comm, in, ctx->g, cs.perm, grid, ham_params(ctx), lam_k, lam_Y, mu);
     hipLaunchKernelGGL(k_zero, dim3(nblk(ng)), dim3(kB), 0, ctx->stream, gadj, ng);
-    cell_deposit<1>(ctx, in, cs, mu, gadj);
+    cell_deposit<1, double>(ctx, in, cs, mu, gadj);
     ADJ_DISPATCH(k_vjp_gather, in, ctx->g, cs.perm, gadj, lam_Y, lam_Nbar);
     EMU_LAUNCH_CHECK(ctx, "adj_rhs_vjp");
+    return EMU_OK;
+}
+
+template <typename T>
+static int adj_objective(emu_ctx* ctx, int64_t np, const double* pos, const T* Nn, const T* Y, double* J, T* dJdY,
+                         double* work)
+{
+    if (!ctx || np < 0 || !work || !J) return EMU_ERR_ARG;
+    const AdjInT<T> in{np, pos, nullptr, Nn, Y};
+    const int64_t nc = ncell_of(ctx);
+    double* g2 = work; // [ncell][2] N_01, then J
+    double* dJ = g2 + 2 * nc;
+    hipLaunchKernelGGL(k_zero, dim3(nblk(2 * nc + 1)), dim3(kB), 0, ctx->stream, g2, 2 * nc + 1);
+    if (np > 0) {
+        if (ctx->p.nf == 2) hipLaunchKernelGGL((k_obj_deposit<2, T>), dim3(nblk(np)), dim3(kB), 0, ctx->stream, in, ctx->g, g2);
+        else hipLaunchKernelGGL((k_obj_deposit<3, T>), dim3(nblk(np)), dim3(kB), 0, ctx->stream, in, ctx->g, g2);
+    }
+    hipLaunchKernelGGL(k_obj_sum, dim3(nblk(nc)), dim3(kB), 0, ctx->stream, g2, nc, dJ);
+    if (dJdY && np > 0) {
+        if (ctx->p.nf == 2)
+            hipLaunchKernelGGL((k_obj_grad<2, T>), dim3(nblk(np)), dim3(kB), 0, ctx->stream, in, ctx->g, g2, dJdY);
+        else hipLaunchKernelGGL((k_obj_grad<3, T>), dim3(nblk(np)), dim3(kB), 0, ctx->stream, in, ctx->g, g2, dJdY);
+    }
+    EMU_LAUNCH_CHECK(ctx, "adj_objective");
+    EMU_HIP(ctx, hipMemcpyAsync(J, dJ, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    EMU_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return EMU_OK;
 }
 
 extern "C" int emu_adj_objective(emu_ctx* ctx, int64_t np, const double* pos, const double* Nn, const double* Y,
                                  double* J, double* dJdY, double* work)
 {
-    if (!ctx || np < 0 || !work || !J) return EMU_ERR_ARG;
-    const AdjIn in{np, pos, nullptr, Nn, Y};
-    const int nh = ctx->p.nf * ctx->p.nf;
-    const int64_t nc = ncell_of(ctx);
-    double* g2 = work + 2 * nc * 4 * nh + (int64_t)nh * np;
-    double* dJ = g2 + 2 * nc;
-    hipLaunchKernelGGL(k_zero, dim3(nblk(2 * nc + 1)), dim3(kB), 0, ctx->stream, g2, 2 * nc + 1);
-    if (np > 0) ADJ_DISPATCH(k_obj_deposit, in, ctx->g, g2);
-    hipLaunchKernelGGL(k_obj_sum, dim3(nblk(nc)), dim3(kB), 0, ctx->stream, g2, nc, dJ);
-    if (dJdY && np > 0) ADJ_DISPATCH(k_obj_grad, in, ctx->g, g2, dJdY);
-    EMU_LAUNCH_CHECK(ctx, "adj_objective");
-    EMU_HIP(ctx, hipMemcpyAsync(J, dJ, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    EMU_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    return EMU_OK;
+    return adj_objective<double>(ctx, np, pos, Nn, Y, J, dJdY, work);
+}
+
+extern "C" int emu_adj_objective_f32(emu_ctx* ctx, int64_t np, const double* pos, const float* Nn, const float* Y,
+                                     double* J, float* dJdY, double* work)
+{
+    return adj_objective<float>(ctx, np, pos, Nn, Y, J, dJdY, work);
 }
 
 extern "C" int emu_adj_advance_positions(emu_ctx* ctx, int64_t np, double* pos, const double* pup, double dt)
@@ -1065,38 +1209,73 @@ extern "C" int emu_adj_axpy(emu_ctx* ctx, int64_t n, double a, const double* x, 
 }
 
 // ---------------------------------------------------------------------------
-// Step-level calls.  Work layout (doubles): the four stage grids
-// [4][ncell][4 NH], the grid adjoint [ncell][4 NH], the stage states
-// [3][2 NH][np] (Y2, Y3, Y4; the forward step uses the first as its stage
-// buffer), the stage adjoints [2][2 NH][np], mu [NH][np], then the cell sort.
+// Step-level calls, in FP64 (T = double) or in the FP32-bulk form (T = float:
+// the flavour arrays, the particle constants and the adjoints in FP32, with
+// the gathers, commutators and stage algebra in FP32; the deposits sum in
+// FP64 and Renormalize runs in FP64, as in the fused FP32-bulk step).
+// Work layout, in doubles: the four stage grids [4][ncell][4 NH] and two
+// grid adjoints (FP64); for T = float two T copies of grids, which the
+// gathers read; then, in T, the stage states [3][2 NH][np] (Y2, Y3, Y4) and
+// one stage adjoint [2 NH][np]; then the cell sort.
 // ---------------------------------------------------------------------------
 namespace {
-struct StepWork {
+template <typename T> int64_t t_doubles(int64_t n) { return (n * (int64_t)sizeof(T) + 7) / 8; }
+
+template <typename T> struct StepWork {
     double* G[4];
-    double* gadj;
-    double* Ys[3];
-    double* lY[2];
-    double* mu;
+    double* gadj[2];
+    T* gT[2]; // the grids a float gather reads (the double gathers read the grids themselves)
+    T* Ys[3];
+    T* lY;
     CellSort cs;
 };
 
-StepWork step_work(const emu_ctx* ctx, int64_t np, double* work)
+template <typename T> StepWork<T> step_work(const emu_ctx* ctx, int64_t np, double* work)
 {
     const int nh = ctx->p.nf * ctx->p.nf;
     const int64_t ng = ncell_of(ctx) * 4 * nh, ns = 2 * nh * np;
-    StepWork w;
+    StepWork<T> w;
     double* b = work;
     for (int s = 0; s < 4; s++, b += ng) w.G[s] = b;
-    w.gadj = b;
-    b += ng;
-    for (int s = 0; s < 3; s++, b += ns) w.Ys[s] = b;
-    for (int s = 0; s < 2; s++, b += ns) w.lY[s] = b;
-    w.mu = b;
-    b += (int64_t)nh * np;
+    for (int s = 0; s < 2; s++, b += ng) w.gadj[s] = b;
+    w.gT[0] = w.gT[1] = nullptr;
+    if constexpr (sizeof(T) != sizeof(double)) {
+        for (int s = 0; s < 2; s++, b += t_doubles<T>(ng)) w.gT[s] = (T*)b;
+    }
+    for (int s = 0; s < 3; s++, b += t_doubles<T>(ns)) w.Ys[s] = (T*)b;
+    w.lY = (T*)b;
+    b += t_doubles<T>(ns);
     w.cs = cell_sort_at(ctx, np, b);
     return w;
 }
 
+template <typename T> int64_t step_work_doubles(const emu_ctx* ctx, int64_t np)
+{
+    const int nh = ctx->p.nf * ctx->p.nf;
+    const int64_t ng = ncell_of(ctx) * 4 * nh, ns = 2 * nh * np;
+    return 6 * ng + (sizeof(T) != sizeof(double) ? 2 * t_doubles<T>(ng) : 0) + 4 * t_doubles<T>(ns) +
+           cell_sort_doubles(ctx, np);
+}
+
+__global__ void k_to_float(const double* in, float* out, int64_t n)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+        out[t] = (float)in[t];
+}
+
+// The grid a gather reads: the FP64 grid, or its FP32 rounding in buffer
+// `slot` (the FP32-bulk gather's input, as the fused step rounds its grid
+// when staging it).
+template <typename T> const T* gather_grid(emu_ctx* ctx, const StepWork<T>& w, const double* g, int slot = 0)
+{
+    if constexpr (sizeof(T) == sizeof(double)) {
+        return g;
+    } else {
+        const int64_t ng = ncell_of(ctx) * 4 * ctx->p.nf * ctx->p.nf;
+        hipLaunchKernelGGL(k_to_float, dim3(nblk(ng)), dim3(kB), 0, ctx->stream, g, w.gT[slot], ng);
+        return w.gT[slot];
+    }
+}
 
 int arm_status(emu_ctx* ctx)
 {
@@ -1107,118 +1286,163 @@ int arm_status(emu_ctx* ctx)
     return EMU_OK;
 }
 
-template <int NF, int S, int STORE>
-void fwd_stage(emu_ctx* ctx, const AdjIn& in, const StepWork& w, const double* grid, const double* y0, double* ynext,
-               double* acc, const double* Ll, double dt, double* pos)
+template <int NF, int S, int STORE, typename T>
+void fwd_stage(emu_ctx* ctx, const AdjInT<T>& in, const StepWork<T>& w, const double* grid, const T* y0, T* ynext,
+               T* acc, const T* Ll, double dt, double* pos, double wscale)
 {
     const int64_t nc = ncell_of(ctx);
     const int bpx = cell_blocks_per_xcd(nc);
-    hipLaunchKernelGGL((k_fwd_stage<NF, S, STORE>), dim3(8 * bpx), dim3(64 * kCellWaves), 0, ctx->stream, in, ctx->g,
-                       w.cs.start, w.cs.perm, nc, bpx, grid, ham_params(ctx), y0, ynext, acc, Ll, ctx->p.maxError, dt,
-                       pos, ctx->d_red + 5);
+    const T* gg = gather_grid<T>(ctx, w, grid);
+    hipLaunchKernelGGL((k_fwd_stage<NF, S, STORE, T>), dim3(8 * bpx), dim3(64 * kCellWaves), 0, ctx->stream, in,
+                       ctx->g, w.cs.start, w.cs.perm, nc, bpx, gg, ham_params(ctx, wscale), y0, ynext, acc, Ll,
+                       ctx->p.maxError, dt, pos, ctx->d_red + 5);
 }
 
 // A step's kept data (emu_adj_step_keep): the stage states Y2, Y3, Y4
-// ([3][2 NH][np]) and the four stage deposits ([4][ng]).
-struct StepKeep {
-    double* Ys[3];
+// ([3][2 NH][np], in T) and the four stage deposits ([4][ng], FP64).
+template <typename T> struct StepKeep {
+    T* Ys[3];
     double* G[4];
 };
 
-StepKeep step_keep(const emu_ctx* ctx, int64_t np, double* keep)
+template <typename T> StepKeep<T> step_keep(const emu_ctx* ctx, int64_t np, double* keep)
 {
     const int nh = ctx->p.nf * ctx->p.nf;
     const int64_t ng = ncell_of(ctx) * 4 * nh, ns = 2 * nh * np;
-    StepKeep k;
+    StepKeep<T> k;
     double* b = keep;
-    for (int s = 0; s < 3; s++, b += ns) k.Ys[s] = b;
+    for (int s = 0; s < 3; s++, b += t_doubles<T>(ns)) k.Ys[s] = (T*)b;
     for (int s = 0; s < 4; s++, b += ng) k.G[s] = b;
     return k;
 }
 
-int64_t step_keep_doubles(const emu_ctx* ctx, int64_t np)
+template <typename T> int64_t step_keep_doubles(const emu_ctx* ctx, int64_t np)
 {
     const int nh = ctx->p.nf * ctx->p.nf;
-    return 3 * 2 * nh * np + 4 * ncell_of(ctx) * 4 * nh;
+    return 3 * t_doubles<T>(2 * nh * np) + 4 * ncell_of(ctx) * 4 * nh;
 }
 
 // Stage s reads Y_s and its deposit G[s - 1] and writes Y_{s+1} into Ys[s - 1]:
 // every stage state and deposit has its own array, so that a step run with
 // `keep` leaves them all for its VJP.
-template <int NF>
-int adj_step_nf(emu_ctx* ctx, int64_t np, double* pos, const double* pup, const double* Nn, const double* Ll,
-                const double* y_in, double* y_out, double dt, double* work, double* keep)
+template <int NF, typename T>
+int adj_step_nf(emu_ctx* ctx, int64_t np, double* pos, const T* pup, const T* Nn, const T* Ll, const T* y_in,
+                T* y_out, double dt, double wscale, double* work, double* keep)
 {
-    StepWork w = step_work(ctx, np, work);
+    StepWork<T> w = step_work<T>(ctx, np, work);
     if (keep) {
-        const StepKeep k = step_keep(ctx, np, keep);
+        const StepKeep<T> k = step_keep<T>(ctx, np, keep);
         for (int s = 0; s < 3; s++) w.Ys[s] = k.Ys[s];
         for (int s = 0; s < 4; s++) w.G[s] = k.G[s];
     }
-    AdjIn in{np, pos, pup, Nn, y_in};
+    AdjInT<T> in{np, pos, pup, Nn, y_in};
     if (int e = cell_sort(ctx, in, w.cs)) return e;
-    deposit(ctx, in, w.cs, w.G[0]);
-    fwd_stage<NF, 1, 0>(ctx, in, w, w.G[0], y_in, w.Ys[0], y_out, Ll, dt, pos);
+    deposit(ctx, in, w.cs, w.G[0], wscale);
+    fwd_stage<NF, 1, 0, T>(ctx, in, w, w.G[0], y_in, w.Ys[0], y_out, Ll, dt, pos, wscale);
     in.Y = w.Ys[0];
-    deposit(ctx, in, w.cs, w.G[1]);
-    fwd_stage<NF, 2, 0>(ctx, in, w, w.G[1], y_in, w.Ys[1], y_out, Ll, dt, pos);
+    deposit(ctx, in, w.cs, w.G[1], wscale);
+    fwd_stage<NF, 2, 0, T>(ctx, in, w, w.G[1], y_in, w.Ys[1], y_out, Ll, dt, pos, wscale);
     in.Y = w.Ys[1];
-    deposit(ctx, in, w.cs, w.G[2]);
-    fwd_stage<NF, 3, 0>(ctx, in, w, w.G[2], y_in, w.Ys[2], y_out, Ll, dt, pos);
+    deposit(ctx, in, w.cs, w.G[2], wscale);
+    fwd_stage<NF, 3, 0, T>(ctx, in, w, w.G[2], y_in, w.Ys[2], y_out, Ll, dt, pos, wscale);
     in.Y = w.Ys[2];
-    deposit(ctx, in, w.cs, w.G[3]);
-    fwd_stage<NF, 4, 0>(ctx, in, w, w.G[3], y_in, nullptr, y_out, Ll, dt, pos);
+    deposit(ctx, in, w.cs, w.G[3], wscale);
+    fwd_stage<NF, 4, 0, T>(ctx, in, w, w.G[3], y_in, (T*)nullptr, y_out, Ll, dt, pos, wscale);
     return EMU_OK;
 }
 
-template <int NF>
-int adj_step_vjp_nf(emu_ctx* ctx, int64_t np, const double* pos, const double* pup, const double* Nn,
-                    const double* y_in, const double* lam_in, double* lam_out, double* lam_Nbar, double dt,
-                    double* work, const double* keep)
+template <int NF, typename T>
+int adj_step_vjp_nf(emu_ctx* ctx, int64_t np, const double* pos, const T* pup, const T* Nn, const T* y_in,
+                    const T* lam_in, T* lam_out, double* lam_Nbar, double dt, double wscale, double* work,
+                    const double* keep)
 {
     const int nh = NF * NF;
     const int64_t ng = ncell_of(ctx) * 4 * nh;
-    StepWork w = step_work(ctx, np, work);
-    AdjIn in{np, pos, pup, Nn, y_in};
+    StepWork<T> w = step_work<T>(ctx, np, work);
+    AdjInT<T> in{np, pos, pup, Nn, y_in};
     if (int e = cell_sort(ctx, in, w.cs)) return e;
     if (keep) {
         // the forward step's own stage states and deposits
-        const StepKeep k = step_keep(ctx, np, (double*)keep);
+        const StepKeep<T> k = step_keep<T>(ctx, np, (double*)keep);
         for (int s = 0; s < 3; s++) w.Ys[s] = k.Ys[s];
         for (int s = 0; s < 4; s++) w.G[s] = k.G[s];
     } else {
         // recompute the stage states and their deposits
-        deposit(ctx, in, w.cs, w.G[0]);
-        fwd_stage<NF, 1, 1>(ctx, in, w, w.G[0], y_in, w.Ys[0], nullptr, nullptr, dt, nullptr);
+        deposit(ctx, in, w.cs, w.G[0], wscale);
+        fwd_stage<NF, 1, 1, T>(ctx, in, w, w.G[0], y_in, w.Ys[0], nullptr, nullptr, dt, nullptr, wscale);
         in.Y = w.Ys[0];
-        deposit(ctx, in, w.cs, w.G[1]);
-        fwd_stage<NF, 2, 1>(ctx, in, w, w.G[1], y_in, w.Ys[1], nullptr, nullptr, dt, nullptr);
+        deposit(ctx, in, w.cs, w.G[1], wscale);
+        fwd_stage<NF, 2, 1, T>(ctx, in, w, w.G[1], y_in, w.Ys[1], nullptr, nullptr, dt, nullptr, wscale);
         in.Y = w.Ys[1];
-        deposit(ctx, in, w.cs, w.G[2]);
-        fwd_stage<NF, 3, 1>(ctx, in, w, w.G[2], y_in, w.Ys[2], nullptr, nullptr, dt, nullptr);
+        deposit(ctx, in, w.cs, w.G[2], wscale);
+        fwd_stage<NF, 3, 1, T>(ctx, in, w, w.G[2], y_in, w.Ys[2], nullptr, nullptr, dt, nullptr, wscale);
         in.Y = w.Ys[2];
-        deposit(ctx, in, w.cs, w.G[3]);
+        deposit(ctx, in, w.cs, w.G[3], wscale);
     }
-    const double* Yst[4] = {y_in, w.Ys[0], w.Ys[1], w.Ys[2]};
+    const T* Yst[4] = {y_in, w.Ys[0], w.Ys[1], w.Ys[2]};
     // reverse: lambda_k4 = dt/6 lam; lambda_k3 = dt/3 lam + dt lY4;
     // lambda_k2 = dt/3 lam + dt/2 lY3; lambda_k1 = dt/6 lam + dt/2 lY2;
-    // lam_out = lam + lY4 + lY3 + lY2 + lY1
+    // lam_out = lam + lY4 + lY3 + lY2 + lY1.  Launch s (3 .. -1) finishes
+    // stage s+1's lY (the gather back of its grid adjoint) and runs stage s's
+    // commutator adjoints, depositing mu_s onto the other grid adjoint.
     const int bpx = cell_blocks_per_xcd(ncell_of(ctx));
     const double alpha[4] = {dt * (1. / 6.), dt * (1. / 3.), dt * (1. / 3.), dt * (1. / 6.)};
     const double beta[4] = {dt * 0.5, dt * 0.5, dt * 1.0, 0.0};
-    for (int s = 3; s >= 0; s--) {
-        in.Y = Yst[s];
-        double* lY = w.lY[s & 1];
-        const double* lYp = s == 3 ? nullptr : w.lY[(s + 1) & 1];
-        hipLaunchKernelGGL(k_vjp_stage_comm<NF>, dim3(8 * bpx), dim3(64 * kCellWaves), 0, ctx->stream, in, ctx->g,
-                           w.cs.start, w.cs.perm, ncell_of(ctx), bpx, w.G[s], ham_params(ctx), lam_in, lYp, alpha[s],
-                           beta[s], lY, w.mu);
-        hipLaunchKernelGGL(k_zero, dim3(nblk(ng)), dim3(kB), 0, ctx->stream, w.gadj, ng);
-        cell_deposit<1>(ctx, in, w.cs, w.mu, w.gadj);
-        hipLaunchKernelGGL(k_vjp_stage_gather<NF>, dim3(8 * bpx), dim3(64 * kCellWaves), 0, ctx->stream, in, ctx->g,
-                           w.cs.start, w.cs.perm, ncell_of(ctx), bpx, w.gadj, lY, lam_Nbar, lam_in, lam_out,
-                           s == 3 ? 1 : 0);
+    const HamParams hp = ham_params(ctx, wscale);
+    const dim3 grid_dim(8 * bpx), block_dim(64 * kCellWaves);
+    for (int s = 3; s >= -1; s--) {
+        const bool g_half = s < 3, c_half = s >= 0;
+        double* gout = c_half ? w.gadj[s & 1] : nullptr;
+        if (c_half) hipLaunchKernelGGL(k_zero, dim3(nblk(ng)), dim3(kB), 0, ctx->stream, gout, ng);
+        const T* gin = g_half ? gather_grid<T>(ctx, w, w.gadj[(s + 1) & 1], 1) : nullptr;
+        const T* gs = c_half ? gather_grid<T>(ctx, w, w.G[s], 0) : nullptr;
+        const T* ynext = g_half ? Yst[s + 1] : nullptr;
+        const T* ycur = c_half ? Yst[s] : nullptr;
+        const double a = c_half ? alpha[s] : 0.0, b = c_half ? beta[s] : 0.0;
+        const int first = s == 2 ? 1 : 0;
+        if (!g_half)
+            hipLaunchKernelGGL((k_vjp_fused<NF, T, false, true>), grid_dim, block_dim, 0, ctx->stream, in, ctx->g,
+                               w.cs.start, w.cs.perm, ncell_of(ctx), bpx, ynext, ycur, gin, gs, gout, hp, lam_in, w.lY,
+                               lam_out, lam_Nbar, first, a, b);
+        else if (c_half)
+            hipLaunchKernelGGL((k_vjp_fused<NF, T, true, true>), grid_dim, block_dim, 0, ctx->stream, in, ctx->g,
+                               w.cs.start, w.cs.perm, ncell_of(ctx), bpx, ynext, ycur, gin, gs, gout, hp, lam_in, w.lY,
+                               lam_out, lam_Nbar, first, a, b);
+        else
+            hipLaunchKernelGGL((k_vjp_fused<NF, T, true, false>), grid_dim, block_dim, 0, ctx->stream, in, ctx->g,
+                               w.cs.start, w.cs.perm, ncell_of(ctx), bpx, ynext, ycur, gin, gs, gout, hp, lam_in, w.lY,
+                               lam_out, lam_Nbar, first, a, b);
     }
+    return EMU_OK;
+}
+
+template <typename T>
+int adj_step_any(emu_ctx* ctx, int64_t np, double* pos, const T* pup, const T* Nn, const T* Ll, const T* y_in,
+                 T* y_out, double dt, double wscale, double* work, double* keep)
+{
+    if (!ctx || np < 0 || !work || y_in == y_out || !(wscale > 0.0)) return EMU_ERR_ARG;
+    arm_status(ctx);
+    if (np == 0) return EMU_OK;
+    const int e = ctx->p.nf == 2 ? adj_step_nf<2, T>(ctx, np, pos, pup, Nn, Ll, y_in, y_out, dt, wscale, work, keep)
+                                 : adj_step_nf<3, T>(ctx, np, pos, pup, Nn, Ll, y_in, y_out, dt, wscale, work, keep);
+    if (e) return e;
+    EMU_LAUNCH_CHECK(ctx, "adj_step");
+    return EMU_OK;
+}
+
+template <typename T>
+int adj_step_vjp_any(emu_ctx* ctx, int64_t np, const double* pos, const T* pup, const T* Nn, const T* y_in,
+                     const T* lam_in, T* lam_out, double* lam_Nbar, double dt, double wscale, double* work,
+                     const double* keep)
+{
+    if (!ctx || np < 0 || !work || lam_in == lam_out || !(wscale > 0.0)) return EMU_ERR_ARG;
+    if (np == 0) return EMU_OK;
+    const int e = ctx->p.nf == 2 ? adj_step_vjp_nf<2, T>(ctx, np, pos, pup, Nn, y_in, lam_in, lam_out, lam_Nbar, dt,
+                                                         wscale, work, keep)
+                                 : adj_step_vjp_nf<3, T>(ctx, np, pos, pup, Nn, y_in, lam_in, lam_out, lam_Nbar, dt,
+                                                         wscale, work, keep);
+    if (e) return e;
+    EMU_LAUNCH_CHECK(ctx, "adj_step_vjp");
     return EMU_OK;
 }
 } // namespace
@@ -1226,54 +1450,26 @@ int adj_step_vjp_nf(emu_ctx* ctx, int64_t np, const double* pos, const double* p
 extern "C" int64_t emu_adj_step_work_doubles(const emu_ctx* ctx, int64_t np)
 {
     if (!ctx || np < 0) return -1;
-    const int nh = ctx->p.nf * ctx->p.nf;
-    return 5 * ncell_of(ctx) * 4 * nh + 5 * 2 * nh * np + (int64_t)nh * np + cell_sort_doubles(ctx, np);
-}
-
-static int adj_step_any(emu_ctx* ctx, int64_t np, double* pos, const double* pup, const double* Nn,
-                        const double* Ll, const double* y_in, double* y_out, double dt, double* work, double* keep)
-{
-    if (!ctx || np < 0 || !work || y_in == y_out) return EMU_ERR_ARG;
-    arm_status(ctx);
-    if (np == 0) return EMU_OK;
-    const int e = ctx->p.nf == 2 ? adj_step_nf<2>(ctx, np, pos, pup, Nn, Ll, y_in, y_out, dt, work, keep)
-                                 : adj_step_nf<3>(ctx, np, pos, pup, Nn, Ll, y_in, y_out, dt, work, keep);
-    if (e) return e;
-    EMU_LAUNCH_CHECK(ctx, "adj_step");
-    return EMU_OK;
-}
-
-static int adj_step_vjp_any(emu_ctx* ctx, int64_t np, const double* pos, const double* pup, const double* Nn,
-                            const double* y_in, const double* lam_in, double* lam_out, double* lam_Nbar, double dt,
-                            double* work, const double* keep)
-{
-    if (!ctx || np < 0 || !work || lam_in == lam_out) return EMU_ERR_ARG;
-    if (np == 0) return EMU_OK;
-    const int e = ctx->p.nf == 2
-                      ? adj_step_vjp_nf<2>(ctx, np, pos, pup, Nn, y_in, lam_in, lam_out, lam_Nbar, dt, work, keep)
-                      : adj_step_vjp_nf<3>(ctx, np, pos, pup, Nn, y_in, lam_in, lam_out, lam_Nbar, dt, work, keep);
-    if (e) return e;
-    EMU_LAUNCH_CHECK(ctx, "adj_step_vjp");
-    return EMU_OK;
+    return step_work_doubles<double>(ctx, np);
 }
 
 extern "C" int emu_adj_step(emu_ctx* ctx, int64_t np, double* pos, const double* pup, const double* Nn,
                             const double* Ll, const double* y_in, double* y_out, double dt, double* work)
 {
-    return adj_step_any(ctx, np, pos, pup, Nn, Ll, y_in, y_out, dt, work, nullptr);
+    return adj_step_any<double>(ctx, np, pos, pup, Nn, Ll, y_in, y_out, dt, 1.0, work, nullptr);
 }
 
 extern "C" int emu_adj_step_vjp(emu_ctx* ctx, int64_t np, const double* pos, const double* pup, const double* Nn,
                                 const double* y_in, const double* lam_in, double* lam_out, double* lam_Nbar, double dt,
                                 double* work)
 {
-    return adj_step_vjp_any(ctx, np, pos, pup, Nn, y_in, lam_in, lam_out, lam_Nbar, dt, work, nullptr);
+    return adj_step_vjp_any<double>(ctx, np, pos, pup, Nn, y_in, lam_in, lam_out, lam_Nbar, dt, 1.0, work, nullptr);
 }
 
 extern "C" int64_t emu_adj_step_keep_doubles(const emu_ctx* ctx, int64_t np)
 {
     if (!ctx || np < 0) return -1;
-    return step_keep_doubles(ctx, np);
+    return step_keep_doubles<double>(ctx, np);
 }
 
 extern "C" int emu_adj_step_keep(emu_ctx* ctx, int64_t np, double* pos, const double* pup, const double* Nn,
@@ -1281,7 +1477,7 @@ extern "C" int emu_adj_step_keep(emu_ctx* ctx, int64_t np, double* pos, const do
                                  double* keep)
 {
     if (!keep) return EMU_ERR_ARG;
-    return adj_step_any(ctx, np, pos, pup, Nn, Ll, y_in, y_out, dt, work, keep);
+    return adj_step_any<double>(ctx, np, pos, pup, Nn, Ll, y_in, y_out, dt, 1.0, work, keep);
 }
 
 extern "C" int emu_adj_step_vjp_kept(emu_ctx* ctx, int64_t np, const double* pos, const double* pup,
@@ -1289,13 +1485,57 @@ extern "C" int emu_adj_step_vjp_kept(emu_ctx* ctx, int64_t np, const double* pos
                                      double* lam_out, double* lam_Nbar, double dt, double* work)
 {
     if (!keep) return EMU_ERR_ARG;
-    return adj_step_vjp_any(ctx, np, pos, pup, Nn, y_in, lam_in, lam_out, lam_Nbar, dt, work, keep);
+    return adj_step_vjp_any<double>(ctx, np, pos, pup, Nn, y_in, lam_in, lam_out, lam_Nbar, dt, 1.0, work, keep);
+}
+
+extern "C" int64_t emu_adj_step_work_doubles_f32(const emu_ctx* ctx, int64_t np)
+{
+    if (!ctx || np < 0) return -1;
+    return step_work_doubles<float>(ctx, np);
+}
+
+extern "C" int emu_adj_step_f32(emu_ctx* ctx, int64_t np, double* pos, const float* pup, const float* Nn,
+                                const float* Ll, const float* y_in, float* y_out, double dt, double wscale,
+                                double* work)
+{
+    return adj_step_any<float>(ctx, np, pos, pup, Nn, Ll, y_in, y_out, dt, wscale, work, nullptr);
+}
+
+extern "C" int emu_adj_step_vjp_f32(emu_ctx* ctx, int64_t np, const double* pos, const float* pup, const float* Nn,
+                                    const float* y_in, const float* lam_in, float* lam_out, double* lam_Nbar,
+                                    double dt, double wscale, double* work)
+{
+    return adj_step_vjp_any<float>(ctx, np, pos, pup, Nn, y_in, lam_in, lam_out, lam_Nbar, dt, wscale, work,
+                                   nullptr);
+}
+
+extern "C" int64_t emu_adj_step_keep_doubles_f32(const emu_ctx* ctx, int64_t np)
+{
+    if (!ctx || np < 0) return -1;
+    return step_keep_doubles<float>(ctx, np);
+}
+
+extern "C" int emu_adj_step_keep_f32(emu_ctx* ctx, int64_t np, double* pos, const float* pup, const float* Nn,
+                                     const float* Ll, const float* y_in, float* y_out, double dt, double wscale,
+                                     double* work, double* keep)
+{
+    if (!keep) return EMU_ERR_ARG;
+    return adj_step_any<float>(ctx, np, pos, pup, Nn, Ll, y_in, y_out, dt, wscale, work, keep);
+}
+
+extern "C" int emu_adj_step_vjp_kept_f32(emu_ctx* ctx, int64_t np, const double* pos, const float* pup,
+                                         const float* Nn, const float* y_in, const double* keep, const float* lam_in,
+                                         float* lam_out, double* lam_Nbar, double dt, double wscale, double* work)
+{
+    if (!keep) return EMU_ERR_ARG;
+    return adj_step_vjp_any<float>(ctx, np, pos, pup, Nn, y_in, lam_in, lam_out, lam_Nbar, dt, wscale, work, keep);
 }
 
 namespace {
 // out[r][i] = in[r][idx[i]] (gather) or out[r][idx[i]] = in[r][i] (scatter;
 // accumulate adds), rows of np doubles
-__global__ void k_permute_rows(int rows, int64_t np, const double* in, const int* idx, double* out, int mode)
+template <typename T>
+__global__ void k_permute_rows(int rows, int64_t np, const T* in, const int* idx, T* out, int mode)
 {
     const int64_t n = (int64_t)rows * np;
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
@@ -1318,23 +1558,35 @@ extern "C" int emu_adj_cell_order(emu_ctx* ctx, int64_t np, const double* pos, i
 {
     if (!ctx || np < 0 || !work || (np > 0 && (!pos || !order))) return EMU_ERR_ARG;
     if (np == 0) return EMU_OK;
-    const StepWork w = step_work(ctx, np, work);
+    const CellSort cs = cell_sort_at(ctx, np, work);
     const AdjIn in{np, pos, nullptr, nullptr, nullptr};
-    if (int e = cell_sort(ctx, in, w.cs)) return e;
-    hipLaunchKernelGGL(k_copy_ints, dim3(nblk(np)), dim3(kB), 0, ctx->stream, np, w.cs.perm, order);
+    if (int e = cell_sort(ctx, in, cs)) return e;
+    hipLaunchKernelGGL(k_copy_ints, dim3(nblk(np)), dim3(kB), 0, ctx->stream, np, cs.perm, order);
     EMU_LAUNCH_CHECK(ctx, "adj_cell_order");
+    return EMU_OK;
+}
+
+template <typename T>
+static int adj_permute(emu_ctx* ctx, int rows, int64_t np, const T* in, const int* idx, T* out, int mode)
+{
+    if (!ctx || rows < 0 || np < 0 || mode < 0 || mode > 2 || in == out) return EMU_ERR_ARG;
+    if (rows == 0 || np == 0) return EMU_OK;
+    hipLaunchKernelGGL(k_permute_rows<T>, dim3(nblk((int64_t)rows * np)), dim3(kB), 0, ctx->stream, rows, np, in, idx,
+                       out, mode);
+    EMU_LAUNCH_CHECK(ctx, "adj_permute");
     return EMU_OK;
 }
 
 extern "C" int emu_adj_permute(emu_ctx* ctx, int rows, int64_t np, const double* in, const int* idx, double* out,
                                int mode)
 {
-    if (!ctx || rows < 0 || np < 0 || mode < 0 || mode > 2 || in == out) return EMU_ERR_ARG;
-    if (rows == 0 || np == 0) return EMU_OK;
-    hipLaunchKernelGGL(k_permute_rows, dim3(nblk((int64_t)rows * np)), dim3(kB), 0, ctx->stream, rows, np, in, idx,
-                       out, mode);
-    EMU_LAUNCH_CHECK(ctx, "adj_permute");
-    return EMU_OK;
+    return adj_permute<double>(ctx, rows, np, in, idx, out, mode);
+}
+
+extern "C" int emu_adj_permute_f32(emu_ctx* ctx, int rows, int64_t np, const float* in, const int* idx, float* out,
+                                   int mode)
+{
+    return adj_permute<float>(ctx, rows, np, in, idx, out, mode);
 }
 
 extern "C" int emu_adj_status(emu_ctx* ctx, int64_t* bad_index)

@@ -31,7 +31,8 @@ int emu_adj_rhs_vjp(emu_ctx* ctx, int64_t np, const double* pos, const double* p
                     const double* Y, const double* lam_k, double* lam_Y, double* lam_Nbar, double* work);
 
 /* J = sum over cells of |N_01|^2, N_01 = sum_p w N f01 (the nu part of the
- * N01 grid components); dJdY (optional) receives dJ/dY.  Synchronous. */
+ * N01 grid components); dJdY (optional) receives dJ/dY.  'work': 2 ncell + 1
+ * doubles at least (either work array of this header).  Synchronous. */
 int emu_adj_objective(emu_ctx* ctx, int64_t np, const double* pos, const double* Nn, const double* Y, double* J,
                       double* dJdY, double* work);
 
@@ -84,16 +85,46 @@ int emu_adj_step_vjp_kept(emu_ctx* ctx, int64_t np, const double* pos, const dou
                           const double* y_in, const double* keep, const double* lam_in, double* lam_out,
                           double* lam_Nbar, double dt, double* work);
 
+/* The FP32-bulk form of the step calls (no counterpart in the reference: the
+ * adjoint of the FP32-bulk forward).  The flavour arrays, the particle
+ * constants pup, Nn, Ll and the adjoints are float; positions and lam_Nbar stay
+ * double.  The gathers, commutators and stage algebra run in FP32, the deposits
+ * sum in FP64 and Renormalize runs in FP64, as in the fused FP32-bulk step.
+ * Nn holds the weights times wscale, a power of two chosen so that FP32 holds
+ * them and the adjoints of |N_01|^2 (which grow as N^2); the potential's
+ * coefficient and the matter term take the factor back out, so the evolution
+ * is that of the unscaled weights, and the objective and its adjoints are
+ * wscale^2 times the unscaled ones.  'work' holds
+ * emu_adj_step_work_doubles_f32(ctx, np) doubles, 'keep'
+ * emu_adj_step_keep_doubles_f32(ctx, np). */
+int64_t emu_adj_step_work_doubles_f32(const emu_ctx* ctx, int64_t np);
+int emu_adj_step_f32(emu_ctx* ctx, int64_t np, double* pos, const float* pup, const float* Nn, const float* Ll,
+                     const float* y_in, float* y_out, double dt, double wscale, double* work);
+int emu_adj_step_vjp_f32(emu_ctx* ctx, int64_t np, const double* pos, const float* pup, const float* Nn,
+                         const float* y_in, const float* lam_in, float* lam_out, double* lam_Nbar, double dt,
+                         double wscale, double* work);
+int64_t emu_adj_step_keep_doubles_f32(const emu_ctx* ctx, int64_t np);
+int emu_adj_step_keep_f32(emu_ctx* ctx, int64_t np, double* pos, const float* pup, const float* Nn, const float* Ll,
+                          const float* y_in, float* y_out, double dt, double wscale, double* work, double* keep);
+int emu_adj_step_vjp_kept_f32(emu_ctx* ctx, int64_t np, const double* pos, const float* pup, const float* Nn,
+                              const float* y_in, const double* keep, const float* lam_in, float* lam_out,
+                              double* lam_Nbar, double dt, double wscale, double* work);
+/* emu_adj_objective on float flavour arrays (J summed in FP64). */
+int emu_adj_objective_f32(emu_ctx* ctx, int64_t np, const double* pos, const float* Nn, const float* Y, double* J,
+                          float* dJdY, double* work);
+
 /* The particles listed by cell (the cell of their stencil's centre) at
  * positions pos: order[i] is the i-th particle in that order (within a cell,
  * in no set order).  The step calls run fastest on arrays kept in this
- * order: their particle streams are then read cell by cell.  Asynchronous. */
+ * order: their particle streams are then read cell by cell.  'work' is either
+ * step-call work array.  Asynchronous. */
 int emu_adj_cell_order(emu_ctx* ctx, int64_t np, const double* pos, int* order, double* work);
 
-/* Rows of np doubles through a permutation: mode 0 gathers out[r][i] =
- * in[r][idx[i]], mode 1 scatters out[r][idx[i]] = in[r][i], mode 2 adds the
- * scatter into out.  out must not alias in.  Asynchronous. */
+/* Rows of np doubles (floats: _f32) through a permutation: mode 0 gathers
+ * out[r][i] = in[r][idx[i]], mode 1 scatters out[r][idx[i]] = in[r][i], mode 2
+ * adds the scatter into out.  out must not alias in.  Asynchronous. */
 int emu_adj_permute(emu_ctx* ctx, int rows, int64_t np, const double* in, const int* idx, double* out, int mode);
+int emu_adj_permute_f32(emu_ctx* ctx, int rows, int64_t np, const float* in, const int* idx, float* out, int mode);
 
 /* The first Renormalize error of the emu_adj_step calls since the previous
  * emu_adj_status (then cleared), with the lowest particle index in

@@ -230,3 +230,68 @@ def test_adjoint32_gradient_matches_finite_differences(mods):
         ad = float((gf * v).sum())
         print(f"adjoint32: direction {trial}: adjoint {ad:.10e} fd {fd:.10e}")
         assert ad == pytest.approx(fd, rel=1e-5)
+
+
+@pytest.mark.parametrize("nf", [2, 3])
+def test_fp32_step_and_vjp_track_fp64(mods, oracle, nf):
+    """The FP32-bulk step calls against the FP64 ones from the same state: one
+    step's y_{n+1} and the step VJP's lam_n and lam_Nbar agree to FP32
+    rounding (1e-5 of each array's largest entry), and the FP32 positions are
+    the FP64 ones exactly (positions stay FP64)."""
+    torch, core, adjoint = mods
+    emu, c, deck, parts = setup(mods, oracle, nf, amp=1e-3)
+    dt = 0.4 * 0.5 / 2.99792458e10 * 0.5
+    p64 = adjoint.AdjointProblem(emu, c, dt=dt, nsteps=3)
+    p32 = adjoint.AdjointProblem(emu, c, dt=dt, nsteps=3, precision="fp32")
+
+    def close(a, b, tol):
+        return float((a.double() - b.double()).abs().max() / b.double().abs().max().clamp_min(1e-300)) < tol
+
+    pa, pb = p64.pos0.clone(), p32.pos0.clone()
+    ya = p64.step(pa, p64.Nn_base, p64.f0)
+    yb = p32.step(pb, p32.Nn_base, p32.f0)
+    p64.check_status()
+    p32.check_status()
+    assert yb.dtype == torch.float32
+    assert torch.equal(pa, pb)
+    assert close(yb, ya, 1e-5)
+    gen = torch.Generator(device="cpu").manual_seed(9)
+    lam = torch.randn(p64.f0.shape, generator=gen, dtype=torch.float64).to(p64.f0.device)
+    p64.lam_nbar.zero_()
+    p32.lam_nbar.zero_()
+    la = p64.step_adjoint(p64.pos0, p64.Nn_base, p64.f0, lam)
+    lb = p32.step_adjoint(p32.pos0, p32.Nn_base, p32.f0, lam.float())
+    assert close(lb, la, 1e-5)
+    # the FP32 form's weights carry wscale, so its lam_Nbar is d/d(wscale Nbar)
+    assert p32.wscale != 1.0
+    assert close(p32.lam_nbar * p32.wscale, p64.lam_nbar, 1e-5)
+
+
+def test_adjoint32_fp32_gradient_tracks_fp64(mods):
+    """The 2-flavour 32^3 x 92 workload over 200 steps (checkpoints every 20)
+    in the FP32-bulk form against the FP64 adjoint: J, dJ/dratio and the
+    directional derivative of J along a random direction of the initial
+    flavour state.  The bars (1e-3 relative) hold the FP32 rounding of 200
+    steps of the state and its adjoint; the measured errors are printed."""
+    torch, core, adjoint = mods
+    from emu_amd import configs
+    parms = configs.adjoint32()
+    emu = core.Emu(parms)
+    c = core.FlavoredNeutrinoContainer.init_particles(emu)
+    st = core.FusedState(emu, c.np, "fp64")
+    st.load(c)
+    dt = st.info()["dt"]
+    st.close()
+    p64 = adjoint.AdjointProblem(emu, c, dt=dt, nsteps=200, checkpoint_every=20)
+    J64, g64, r64 = p64.gradients(p64.f0.clone(), 1.0)
+    del p64
+    p32 = adjoint.AdjointProblem(emu, c, dt=dt, nsteps=200, checkpoint_every=20, precision="fp32")
+    J32, g32, r32 = p32.gradients(p32.f0.clone(), 1.0)
+    gen = torch.Generator(device="cpu").manual_seed(13)
+    v = torch.randn(g64.shape, generator=gen, dtype=torch.float64).to(g64.device)
+    d64 = float((g64 * v).sum())
+    d32 = float((g32.double() * v).sum())
+    eJ, er, ed = abs(J32 / J64 - 1), abs(r32 / r64 - 1), abs(d32 / d64 - 1)
+    print(f"adjoint32 fp32 vs fp64: J {eJ:.2e}, dJ/dratio {er:.2e}, directional {ed:.2e}")
+    assert eJ < 1e-3 and er < 1e-3 and ed < 1e-3
+
