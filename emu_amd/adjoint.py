@@ -119,8 +119,9 @@ class AdjointProblem:
         held = ncp + (1 if self.segment == "recompute" else self.cp)
         kept = 8.0 * self.cp * self.keep_doubles if self.segment == "keep" else 0.0
         work = 8.0 * (self.step_work_doubles + self.work_doubles)
-        # pos0; pup, Nn, Ll; f0; Nn(ratio); the segment's reordered pup, Nn, Ll; lam_nbar twice
-        consts = np_ * (8.0 * 3 + e * 8 + e * nh2 + e * 2 + e * 8 + 8.0 * 2)
+        # pos0; pup, Nn, Ll; f0 (while held); Nn(ratio); the segment's reordered pup, Nn, Ll; lam_nbar twice
+        f0 = e * nh2 if getattr(self, "f0", True) is not None else 0.0
+        consts = np_ * (8.0 * 3 + e * 8 + f0 + e * 2 + e * 8 + 8.0 * 2)
         live = 4 * e * nh2 * np_  # y and the next y, the adjoint and its next value
         # a re-sort's new copies of the positions and the flavour state, and the
         # adjoint's move between orders, live beside the old ones for a moment
@@ -319,13 +320,14 @@ class AdjointProblem:
         Nn[1] *= ratio
         return Nn
 
-    def run(self, f0, ratio: float, keep_checkpoints=True):
+    def run(self, f0, ratio: float, keep_checkpoints=True, consume=False):
         """J after nsteps from f0 (problem order); with keep_checkpoints, also
         dJ/dy_N and the checkpoints (n, pos, y, order) for the sweep, and
-        the order dJ/dy_N is in."""
+        the order dJ/dy_N is in.  consume: f0's memory becomes the run's state
+        (no copy; f0 then holds no meaningful values)."""
         Nn = self.Nn(ratio)
         pos = self.pos0.clone()
-        y = f0.clone()
+        y = f0 if consume else f0.clone()
         order = torch.arange(self.np, dtype=torch.int32, device=y.device)
         cps = []
         for n in range(self.nsteps):
