@@ -649,50 +649,197 @@ __device__ __forceinline__ void cell_hamiltonians(const AdjInT<T>& in, int64_t p
     }
 }
 
-// Stage S of one step, in.Y = Y_S, grid = its deposit.
-//  STORE 0 (the forward step): S = 1..3 write Y_{S+1} into ynext (which may be
-//  in.Y: each particle reads its own Y_S first) and the accumulator into acc;
-//  S = 4 finishes acc = y_{n+1}, renormalizes it (status deferred into err)
-//  and advances the particle's position.
+// A wave's deposit of its cell's particles on the f64 matrix cores, a chunk
+// of up to 64 at a time: lane l's particle adds its source row (NH values,
+// zero on inactive lanes) times (1, s_x, s_y, s_z) with its shape factors to
+// the cell's 3 x 3 x 3 neighbourhood, D[t][q] = sum_p W[p][t] R[p][q]; flush()
+// adds D to the grid with one atomic per value.  An order-0 dimension's
+// weights are (0, 1, 0), so its other targets receive zeros.
+template <int NF> struct WaveDeposit {
+    static constexpr int nh = NH<NF>, QG = 4 * nh, NT = 27, MT = 2, NTL = (QG + 15) / 16;
+    static constexpr int NSR = nh + 4;            // the source row, s, 1
+    static constexpr int REG = 64 * 9 + 64 * NSR; // LDS doubles per wave
+    mf_d4 acc[MT][NTL];
+    int tdx[MT][3], bi1[NTL], bi2[NTL];
+
+    __device__ __forceinline__ void init(int lane)
+    {
+#pragma unroll
+        for (int m = 0; m < MT; m++) {
+            const int t = 16 * m + (lane & 15);
+            const int tc = t < NT ? t : NT - 1;
+            tdx[m][0] = tc % 3;
+            tdx[m][1] = 3 + (tc / 3) % 3;
+            tdx[m][2] = 6 + tc / 9;
+        }
+#pragma unroll
+        for (int n = 0; n < NTL; n++) {
+            const int q = 16 * n + (lane & 15);
+            const int f = q / nh, c = q % nh;
+            bi1[n] = q < QG ? c : nh + 3;
+            bi2[n] = q < QG ? (f == 0 ? nh + 3 : nh + f - 1) : nh + 3;
+        }
+#pragma unroll
+        for (int m = 0; m < MT; m++)
+#pragma unroll
+            for (int n = 0; n < NTL; n++) acc[m][n] = mf_d4{0.0, 0.0, 0.0, 0.0};
+    }
+
+    // every lane of the wave calls this (count: the chunk's particles)
+    template <typename T>
+    __device__ __forceinline__ void add(double* Wf, double* Sr, int lane, const double wd[3][3], const T* src,
+                                        const T sv[3], int count)
+    {
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+#pragma unroll
+            for (int e = 0; e < 3; e++) Wf[lane * 9 + 3 * d + e] = wd[d][e];
+#pragma unroll
+        for (int c = 0; c < nh; c++) Sr[lane * NSR + c] = double(src[c]);
+#pragma unroll
+        for (int d = 0; d < 3; d++) Sr[lane * NSR + nh + d] = double(sv[d]);
+        Sr[lane * NSR + nh + 3] = 1.0;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int nks = (min(64, count) + 3) >> 2;
+        for (int ks = 0; ks < nks; ks++) {
+            const int pp = 4 * ks + (lane >> 4);
+            const double* wf = Wf + pp * 9;
+            const double* sr = Sr + pp * NSR;
+            double A[MT];
+#pragma unroll
+            for (int m = 0; m < MT; m++) A[m] = wf[tdx[m][0]] * wf[tdx[m][1]] * wf[tdx[m][2]];
+#pragma unroll
+            for (int n = 0; n < NTL; n++) {
+                const double B = sr[bi1[n]] * sr[bi2[n]];
+#pragma unroll
+                for (int m = 0; m < MT; m++)
+                    acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[m], B, acc[m][n], 0, 0, 0);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+
+    __device__ __forceinline__ void flush(double* grid, const Geom& g, int cell, int lane)
+    {
+        // element r of lane l holds D[16m + 4r + l/16][16n + l%16]
+        const int kx = __builtin_amdgcn_readfirstlane(cell % g.n[0]);
+        const int ky = __builtin_amdgcn_readfirstlane((cell / g.n[0]) % g.n[1]);
+        const int kz = __builtin_amdgcn_readfirstlane(cell / (g.n[0] * g.n[1]));
+        auto wrap = [](int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); };
+#pragma unroll
+        for (int m = 0; m < MT; m++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int t = 16 * m + 4 * r + (lane >> 4);
+                if (t >= NT) continue;
+                const int tx = wrap(kx + t % 3 - 1, g.n[0]);
+                const int ty = wrap(ky + (t / 3) % 3 - 1, g.n[1]);
+                const int tz = wrap(kz + t / 9 - 1, g.n[2]);
+                double* Gd = grid + (((int64_t)tz * g.n[1] + ty) * g.n[0] + tx) * QG;
+#pragma unroll
+                for (int n = 0; n < NTL; n++) {
+                    const int q = 16 * n + (lane & 15);
+                    if (q < QG) unsafeAtomicAdd(&Gd[q], acc[m][n][r]);
+                }
+            }
+    }
+};
+
+// Stage S of one step, in.Y = Y_S, grid = its deposit (rounded to T).
+//  STORE 0 (the forward step): S = 1..3 write Y_{S+1} into ynext and the
+//  accumulator into acc; S = 4 finishes acc = y_{n+1}, renormalizes it
+//  (status deferred into err) and advances the particle's position.
 //  STORE 1 (the VJP's recompute): S = 1..3 write Y_{S+1} into ynext only.
+//  S = 1..3 also deposit Y_{S+1} onto gnext (its base already there), the
+//  wave's particles on the matrix cores as the deposit kernel does: positions
+//  are frozen within the step, so the cell's particles are the ones whose
+//  stencils the deposit needs, and the stage state goes from registers to the
+//  grid without a second pass over the particles.
 template <int NF, int S, int STORE, typename T>
 __global__ __launch_bounds__(64 * kCellWaves) void k_fwd_stage(AdjInT<T> in, Geom g, const int* start, const int* perm,
                                                                int64_t ncell, int bpx, const T* grid, HamParams hp,
                                                                const T* y0, T* ynext, T* acc, const T* Ll,
                                                                double maxError, double dt, double* pos_out,
-                                                               unsigned long long* err)
+                                                               unsigned long long* err, double* gnext)
 {
+    using WD = WaveDeposit<NF>;
+    constexpr bool DEP = S < 4;
     constexpr int nh = NH<NF>, S2 = 2 * nh;
+    __shared__ double lds[DEP ? kCellWaves * WD::REG : 1];
+    const int wave = threadIdx.x >> 6;
+    double* Wf = lds + wave * WD::REG;
+    double* Sr = Wf + 64 * 9;
     const int64_t np = in.np;
     const int cell = wave_cell(ncell, bpx);
     if (cell < 0) return;
     const int beg = __builtin_amdgcn_readfirstlane(start[cell]), end = __builtin_amdgcn_readfirstlane(start[cell + 1]);
+    if (beg == end) return;
     const int lane = threadIdx.x & 63;
     const T a_next = T(S == 3 ? dt * 1.0 : dt * 0.5);
     const T a_acc = T(S == 1 || S == 4 ? dt * (1. / 6.) : dt * (1. / 3.));
+    WD dep;
+    if constexpr (DEP) dep.init(lane);
     for (int base = beg; base < end; base += 64) {
         const bool act = base + lane < end;
         const int64_t p = perm[act ? base + lane : beg];
-        T H[nh], Hb[nh], f[nh], k[S2];
-        cell_hamiltonians<NF, T>(in, p, g, cell, grid, hp, H, Hb);
-        if (!act) continue;
+        const double x[3] = {in.pos[p], in.pos[np + p], in.pos[2 * np + p]};
+        double wd[3][3];
+        T w[3][3];
 #pragma unroll
-        for (int c = 0; c < nh; c++) f[c] = in.Y[(int64_t)c * np + p];
-        commutator_rhs<NF, T>(H, f, k, T(1));
+        for (int d = 0; d < 3; d++) {
+            int idx;
+            shape3((x[d] - g.plo[d]) * g.dxi[d], g.order[d], idx, wd[d]);
 #pragma unroll
-        for (int c = 0; c < nh; c++) f[c] = in.Y[(int64_t)(nh + c) * np + p];
-        commutator_rhs<NF, T>(Hb, f, k + nh, T(1));
+            for (int e = 0; e < 3; e++) w[d][e] = T(wd[d][e]);
+        }
+        const T pt = in.pup[3 * np + p];
+        const T v[3] = {in.pup[p] / pt, in.pup[np + p] / pt, in.pup[2 * np + p] / pt};
+        T H[nh], Hb[nh];
         {
-#pragma clang fp contract(off)
+            const T sv[3] = {-v[0], -v[1], -v[2]};
+            T inside[nh];
+#pragma unroll
+            for (int c = 0; c < nh; c++) inside[c] = T(0);
+            gather_cell_box<NF, T>(g, cell, grid, w, sv, inside);
+#pragma unroll
+            for (int c = 0; c < nh; c++) {
+                const T hv = T(hp.vac[c]) / pt, hs = T(hp.coef) * inside[c];
+                H[c] = hv + hs;
+                Hb[c] = is_im_comp<NF>(c) ? hs - hv : hv - hs;
+            }
+        }
+        T q[nh]; // the sources of Y_{S+1} for the fused deposit (zero on inactive lanes)
+#pragma unroll
+        for (int c = 0; c < nh; c++) q[c] = T(0);
+        if (act) {
+            T f[nh], k[S2];
+#pragma unroll
+            for (int c = 0; c < nh; c++) f[c] = in.Y[(int64_t)c * np + p];
+            commutator_rhs<NF, T>(H, f, k, T(1));
+#pragma unroll
+            for (int c = 0; c < nh; c++) f[c] = in.Y[(int64_t)(nh + c) * np + p];
+            commutator_rhs<NF, T>(Hb, f, k + nh, T(1));
             if constexpr (S < 4) {
+#pragma clang fp contract(off)
+                T yv[S2];
 #pragma unroll
                 for (int c = 0; c < S2; c++) {
                     const T y = y0[(int64_t)c * np + p];
-                    ynext[(int64_t)c * np + p] = a_next * k[c] + y;
+                    yv[c] = a_next * k[c] + y;
+                    ynext[(int64_t)c * np + p] = yv[c];
                     if constexpr (STORE == 0)
                         acc[(int64_t)c * np + p] = a_acc * k[c] + (S == 1 ? y : acc[(int64_t)c * np + p]);
                 }
+                const T N = in.Nn[p], Nb = in.Nn[np + p];
+#pragma unroll
+                for (int c = 0; c < nh; c++)
+                    q[c] = is_im_comp<NF>(c) ? N * yv[c] + Nb * yv[nh + c] : N * yv[c] - Nb * yv[nh + c];
             } else {
+#pragma clang fp contract(off)
                 // y_{n+1} in T, then Renormalize in FP64 (FlavoredNeutrinoContainer.cpp:89-113):
                 // nu, then nubar unless nu failed; a failing half is left as it is
                 double y[S2];
@@ -703,19 +850,19 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_fwd_stage(AdjInT<T> in, Geo
                 if (e != EMU_OK) atomicMin(err, status_word(p, e));
 #pragma unroll
                 for (int c = 0; c < S2; c++) acc[(int64_t)c * np + p] = T(y[c]);
-                double x[3], v[3], nx[3];
+                double vv[3], nx[3];
 #pragma unroll
-                for (int d = 0; d < 3; d++) {
-                    x[d] = in.pos[(int64_t)d * np + p];
-                    v[d] = double(in.pup[(int64_t)d * np + p]) / double(in.pup[3 * np + p]) * phys::c;
-                }
+                for (int d = 0; d < 3; d++)
+                    vv[d] = double(in.pup[(int64_t)d * np + p]) / double(in.pup[3 * np + p]) * phys::c;
                 int nc;
-                end_position(x, v, dt, g, nx, nc);
+                end_position(x, vv, dt, g, nx, nc);
 #pragma unroll
                 for (int d = 0; d < 3; d++) pos_out[(int64_t)d * np + p] = nx[d];
             }
         }
+        if constexpr (DEP) dep.add(Wf, Sr, lane, wd, q, v, end - base);
     }
+    if constexpr (DEP) dep.flush(gnext, g, cell, lane);
 }
 
 // One reverse stage boundary in one pass, one cell per wave (s = 3 .. -1
@@ -741,13 +888,11 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_vjp_fused(AdjInT<T> in, Geo
                                                                const T* lam, T* lY, T* lam_out, double* lamNbar,
                                                                int first, double alpha, double beta)
 {
-    constexpr int nh = NH<NF>, QG = 4 * nh;
-    constexpr int NT = 27, MT = 2, NTL = (QG + 15) / 16;
-    constexpr int NSR = nh + 4;            // mu, -v, 1
-    constexpr int REG = 64 * 9 + 64 * NSR; // doubles per wave: weights, sources
-    __shared__ double lds[C ? kCellWaves * REG : 1];
+    using WD = WaveDeposit<NF>;
+    constexpr int nh = NH<NF>;
+    __shared__ double lds[C ? kCellWaves * WD::REG : 1];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    double* Wf = lds + wave * REG;
+    double* Wf = lds + wave * WD::REG;
     double* Sr = Wf + 64 * 9;
     const int cell = wave_cell(ncell, bpx);
     if (cell < 0) return;
@@ -755,28 +900,8 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_vjp_fused(AdjInT<T> in, Geo
     if (beg == end) return;
     const int64_t np = in.np;
     const T al = T(alpha), be = T(beta);
-    int tdx[MT][3];
-#pragma unroll
-    for (int m = 0; m < MT; m++) {
-        const int t = 16 * m + (lane & 15);
-        const int tc = t < NT ? t : NT - 1;
-        tdx[m][0] = tc % 3;
-        tdx[m][1] = 3 + (tc / 3) % 3;
-        tdx[m][2] = 6 + tc / 9;
-    }
-    int bi1[NTL], bi2[NTL];
-#pragma unroll
-    for (int n = 0; n < NTL; n++) {
-        const int q = 16 * n + (lane & 15);
-        const int f = q / nh, c = q % nh;
-        bi1[n] = q < QG ? c : nh + 3;
-        bi2[n] = q < QG ? (f == 0 ? nh + 3 : nh + f - 1) : nh + 3;
-    }
-    mf_d4 acc[MT][NTL];
-#pragma unroll
-    for (int m = 0; m < MT; m++)
-#pragma unroll
-        for (int n = 0; n < NTL; n++) acc[m][n] = mf_d4{0.0, 0.0, 0.0, 0.0};
+    WD dep;
+    if constexpr (C) dep.init(lane);
     for (int base = beg; base < end; base += 64) {
         const bool act = base + lane < end;
         const int64_t p = perm[act ? base + lane : beg];
@@ -817,9 +942,9 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_vjp_fused(AdjInT<T> in, Geo
             }
         }
         if constexpr (C) {
+            const T sv[3] = {-v[0], -v[1], -v[2]};
             T H[nh], Hb[nh];
             {
-                const T sv[3] = {-v[0], -v[1], -v[2]};
                 T inside[nh];
 #pragma unroll
                 for (int c = 0; c < nh; c++) inside[c] = T(0);
@@ -860,63 +985,11 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_vjp_fused(AdjInT<T> in, Geo
                 for (int c = 0; c < nh; c++)
                     mu[c] = T(hp.coef) * (gH[0][c] + (is_im_comp<NF>(c) ? gH[1][c] : -gH[1][c]));
             }
-            // stage mu with its weights and (-v, 1), then D[t][q] += W^T R on the matrix cores
-#pragma unroll
-            for (int d = 0; d < 3; d++)
-#pragma unroll
-                for (int e = 0; e < 3; e++) Wf[lane * 9 + 3 * d + e] = wd[d][e];
-#pragma unroll
-            for (int c = 0; c < nh; c++) Sr[lane * NSR + c] = double(mu[c]);
-#pragma unroll
-            for (int d = 0; d < 3; d++) Sr[lane * NSR + nh + d] = double(-v[d]);
-            Sr[lane * NSR + nh + 3] = 1.0;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            const int nks = (min(64, end - base) + 3) >> 2;
-            for (int ks = 0; ks < nks; ks++) {
-                const int pp = 4 * ks + (lane >> 4);
-                const double* wf = Wf + pp * 9;
-                const double* sr = Sr + pp * NSR;
-                double A[MT];
-#pragma unroll
-                for (int m = 0; m < MT; m++) A[m] = wf[tdx[m][0]] * wf[tdx[m][1]] * wf[tdx[m][2]];
-#pragma unroll
-                for (int n = 0; n < NTL; n++) {
-                    const double B = sr[bi1[n]] * sr[bi2[n]];
-#pragma unroll
-                    for (int m = 0; m < MT; m++)
-                        acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[m], B, acc[m][n], 0, 0, 0);
-                }
-            }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // mu_s with its weights and (-v, 1) onto gadj_s
+            dep.add(Wf, Sr, lane, wd, mu, sv, end - base);
         }
     }
-    if constexpr (C) {
-        // element r of lane l holds D[16m + 4r + l/16][16n + l%16]
-        const int kx = __builtin_amdgcn_readfirstlane(cell % g.n[0]);
-        const int ky = __builtin_amdgcn_readfirstlane((cell / g.n[0]) % g.n[1]);
-        const int kz = __builtin_amdgcn_readfirstlane(cell / (g.n[0] * g.n[1]));
-        auto wrap = [](int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); };
-#pragma unroll
-        for (int m = 0; m < MT; m++)
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int t = 16 * m + 4 * r + (lane >> 4);
-                if (t >= NT) continue;
-                const int tx = wrap(kx + t % 3 - 1, g.n[0]);
-                const int ty = wrap(ky + (t / 3) % 3 - 1, g.n[1]);
-                const int tz = wrap(kz + t / 9 - 1, g.n[2]);
-                double* Gd = gadj_out + (((int64_t)tz * g.n[1] + ty) * g.n[0] + tx) * QG;
-#pragma unroll
-                for (int n = 0; n < NTL; n++) {
-                    const int q = 16 * n + (lane & 15);
-                    if (q < QG) unsafeAtomicAdd(&Gd[q], acc[m][n][r]);
-                }
-            }
-    }
+    if constexpr (C) dep.flush(gadj_out, g, cell, lane);
 }
 
 __global__ void k_axpy(int64_t n, double a, const double* x, const double* y, double* out)
@@ -1093,13 +1166,18 @@ template <typename T> static int cell_sort(emu_ctx* ctx, const AdjInT<T>& in, co
 
 // the deposit of Y onto the base grid (matter in component 0, carrying the
 // weights' factor wscale like the deposited fields)
-template <typename T>
-static void deposit(emu_ctx* ctx, const AdjInT<T>& in, const CellSort& cs, double* grid, double wscale = 1.0)
+static void fill_base(emu_ctx* ctx, double* grid, double wscale)
 {
     const int64_t ng = ncell_of(ctx) * 4 * ctx->p.nf * ctx->p.nf;
     const double m00 = m00_of(ctx) * wscale;
     if (ctx->p.nf == 2) hipLaunchKernelGGL(k_fill_base<2>, dim3(nblk(ng)), dim3(kB), 0, ctx->stream, grid, ncell_of(ctx), m00);
     else hipLaunchKernelGGL(k_fill_base<3>, dim3(nblk(ng)), dim3(kB), 0, ctx->stream, grid, ncell_of(ctx), m00);
+}
+
+template <typename T>
+static void deposit(emu_ctx* ctx, const AdjInT<T>& in, const CellSort& cs, double* grid, double wscale = 1.0)
+{
+    fill_base(ctx, grid, wscale);
     cell_deposit<0, T>(ctx, in, cs, (const T*)nullptr, grid);
 }
 
@@ -1286,16 +1364,19 @@ int arm_status(emu_ctx* ctx)
     return EMU_OK;
 }
 
+// Stage S reads the deposit of Y_S (grid) and, for S < 4, deposits Y_{S+1}
+// onto gnext
 template <int NF, int S, int STORE, typename T>
 void fwd_stage(emu_ctx* ctx, const AdjInT<T>& in, const StepWork<T>& w, const double* grid, const T* y0, T* ynext,
-               T* acc, const T* Ll, double dt, double* pos, double wscale)
+               T* acc, const T* Ll, double dt, double* pos, double wscale, double* gnext)
 {
     const int64_t nc = ncell_of(ctx);
     const int bpx = cell_blocks_per_xcd(nc);
+    if (S < 4) fill_base(ctx, gnext, wscale);
     const T* gg = gather_grid<T>(ctx, w, grid);
     hipLaunchKernelGGL((k_fwd_stage<NF, S, STORE, T>), dim3(8 * bpx), dim3(64 * kCellWaves), 0, ctx->stream, in,
                        ctx->g, w.cs.start, w.cs.perm, nc, bpx, gg, ham_params(ctx, wscale), y0, ynext, acc, Ll,
-                       ctx->p.maxError, dt, pos, ctx->d_red + 5);
+                       ctx->p.maxError, dt, pos, ctx->d_red + 5, gnext);
 }
 
 // A step's kept data (emu_adj_step_keep): the stage states Y2, Y3, Y4
@@ -1338,16 +1419,13 @@ int adj_step_nf(emu_ctx* ctx, int64_t np, double* pos, const T* pup, const T* Nn
     AdjInT<T> in{np, pos, pup, Nn, y_in};
     if (int e = cell_sort(ctx, in, w.cs)) return e;
     deposit(ctx, in, w.cs, w.G[0], wscale);
-    fwd_stage<NF, 1, 0, T>(ctx, in, w, w.G[0], y_in, w.Ys[0], y_out, Ll, dt, pos, wscale);
+    fwd_stage<NF, 1, 0, T>(ctx, in, w, w.G[0], y_in, w.Ys[0], y_out, Ll, dt, pos, wscale, w.G[1]);
     in.Y = w.Ys[0];
-    deposit(ctx, in, w.cs, w.G[1], wscale);
-    fwd_stage<NF, 2, 0, T>(ctx, in, w, w.G[1], y_in, w.Ys[1], y_out, Ll, dt, pos, wscale);
+    fwd_stage<NF, 2, 0, T>(ctx, in, w, w.G[1], y_in, w.Ys[1], y_out, Ll, dt, pos, wscale, w.G[2]);
     in.Y = w.Ys[1];
-    deposit(ctx, in, w.cs, w.G[2], wscale);
-    fwd_stage<NF, 3, 0, T>(ctx, in, w, w.G[2], y_in, w.Ys[2], y_out, Ll, dt, pos, wscale);
+    fwd_stage<NF, 3, 0, T>(ctx, in, w, w.G[2], y_in, w.Ys[2], y_out, Ll, dt, pos, wscale, w.G[3]);
     in.Y = w.Ys[2];
-    deposit(ctx, in, w.cs, w.G[3], wscale);
-    fwd_stage<NF, 4, 0, T>(ctx, in, w, w.G[3], y_in, (T*)nullptr, y_out, Ll, dt, pos, wscale);
+    fwd_stage<NF, 4, 0, T>(ctx, in, w, w.G[3], y_in, (T*)nullptr, y_out, Ll, dt, pos, wscale, nullptr);
     return EMU_OK;
 }
 
@@ -1369,15 +1447,11 @@ int adj_step_vjp_nf(emu_ctx* ctx, int64_t np, const double* pos, const T* pup, c
     } else {
         // recompute the stage states and their deposits
         deposit(ctx, in, w.cs, w.G[0], wscale);
-        fwd_stage<NF, 1, 1, T>(ctx, in, w, w.G[0], y_in, w.Ys[0], nullptr, nullptr, dt, nullptr, wscale);
+        fwd_stage<NF, 1, 1, T>(ctx, in, w, w.G[0], y_in, w.Ys[0], nullptr, nullptr, dt, nullptr, wscale, w.G[1]);
         in.Y = w.Ys[0];
-        deposit(ctx, in, w.cs, w.G[1], wscale);
-        fwd_stage<NF, 2, 1, T>(ctx, in, w, w.G[1], y_in, w.Ys[1], nullptr, nullptr, dt, nullptr, wscale);
+        fwd_stage<NF, 2, 1, T>(ctx, in, w, w.G[1], y_in, w.Ys[1], nullptr, nullptr, dt, nullptr, wscale, w.G[2]);
         in.Y = w.Ys[1];
-        deposit(ctx, in, w.cs, w.G[2], wscale);
-        fwd_stage<NF, 3, 1, T>(ctx, in, w, w.G[2], y_in, w.Ys[2], nullptr, nullptr, dt, nullptr, wscale);
-        in.Y = w.Ys[2];
-        deposit(ctx, in, w.cs, w.G[3], wscale);
+        fwd_stage<NF, 3, 1, T>(ctx, in, w, w.G[2], y_in, w.Ys[2], nullptr, nullptr, dt, nullptr, wscale, w.G[3]);
     }
     const T* Yst[4] = {y_in, w.Ys[0], w.Ys[1], w.Ys[2]};
     // reverse: lambda_k4 = dt/6 lam; lambda_k3 = dt/3 lam + dt lY4;
