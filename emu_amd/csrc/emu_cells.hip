@@ -342,9 +342,14 @@ __global__ __launch_bounds__((64 * DepPatch<WX, WY, WZ>::WAVES)) __attribute__((
         for (int k = 0; k < NSM; k++) accs[m][k] = 0.0;
     }
     const int cl = lane & (kDepChunk - 1); // the chunk row this lane stages (lanes 32-63 repeat 0-31)
+    // each chunk's permutation entry is loaded one chunk ahead, so that the
+    // particle loads wait on one round trip instead of two
+    int ip_next = beg < end ? perm[beg + cl < end ? beg + cl : beg] : 0;
     for (int base = beg; base < end; base += kDepChunk) {
         const bool act = base + cl < end;
-        const int64_t ip = perm[act ? base + cl : beg];
+        const int64_t ip = ip_next;
+        const int nb = base + kDepChunk;
+        if (nb < end) ip_next = perm[nb + cl < end ? nb + cl : beg];
         int o[3];
         double w[3][3];
         particle_box(P, ip, fab, g, o, w);

@@ -115,15 +115,18 @@ def test_adjoint_gradient_matches_finite_differences(mods, oracle, nf):
     assert float(ratio.grad) == pytest.approx(fd, rel=1e-5)
 
 
+@pytest.mark.parametrize("order", [0, 1, 2])
 @pytest.mark.parametrize("nf", [2, 3])
-def test_fused_step_matches_per_rhs_step(mods, oracle, nf):
+def test_fused_step_matches_per_rhs_step(mods, oracle, nf, order):
     """emu_adj_step (one call: the stages with their deposits, the
     accumulator, Renormalize, the positions) against the same step composed
     from the per-RHS calls and axpys, over 3 steps; and its VJP against the
     central difference of <lam, step(y)> along a random direction (the step is
-    a polynomial in y, so the O(h^2) error is far below the bar)."""
+    a polynomial in y, so the O(h^2) error is far below the bar).  At each
+    shape order: the step kernels deposit onto the full 3 x 3 x 3
+    neighbourhood with the order's weights."""
     torch, core, adjoint = mods
-    emu, c, deck, parts = setup(mods, oracle, nf, amp=1e-3)
+    emu, c, deck, parts = setup(mods, oracle, nf, amp=1e-3, shape_order=order)
     dt = 0.4 * 0.5 / 2.99792458e10 * 0.5
     prob = adjoint.AdjointProblem(emu, c, dt=dt, nsteps=3)
     pa, pb = prob.pos0.clone(), prob.pos0.clone()
