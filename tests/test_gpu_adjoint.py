@@ -194,6 +194,29 @@ def test_kept_stages_match_recompute(mods, oracle, nf):
         assert grads[seg][2] == pytest.approx(grads["keep"][2], rel=1e-10)
 
 
+@pytest.mark.parametrize("nf", [2, 3])
+def test_fp32_segment_modes_agree(mods, oracle, nf):
+    """The FP32-bulk gradient through the three segment modes (kept stage
+    data, kept step states, recompute from the checkpoint), 8 steps with
+    checkpoints every 3: the same J and gradients to FP32 rounding (the FP64
+    deposit sums add in no set order, which can move an FP32 state by an
+    ulp)."""
+    torch, core, adjoint = mods
+    emu, c, deck, parts = setup(mods, oracle, nf, amp=1e-3)
+    dt = 0.4 * 0.5 / 2.99792458e10 * 0.5
+    grads = {}
+    for seg in ("keep", "store", "recompute"):
+        p = adjoint.AdjointProblem(emu, c, dt=dt, nsteps=8, checkpoint_every=3, segment=seg, precision="fp32")
+        grads[seg] = p.gradients(p.f0.clone(), 1.0)
+    J, g, r = grads["keep"]
+    assert g.dtype == torch.float64
+    for seg in ("store", "recompute"):
+        J2, g2, r2 = grads[seg]
+        assert J2 == pytest.approx(J, rel=1e-6)
+        assert float((g2 - g).abs().max() / g.abs().max()) < 1e-5
+        assert r2 == pytest.approx(r, rel=1e-5)
+
+
 def test_adjoint32_gradient_matches_finite_differences(mods):
     """The two-flavour 32^3 x 92 workload (3,014,656 particles), 200 RK4
     steps with checkpoints every 20: dJ/dratio and directional derivatives of
