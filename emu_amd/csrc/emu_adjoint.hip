@@ -754,12 +754,14 @@ template <int NF> struct WaveDeposit {
 //  accumulator into acc; S = 4 finishes acc = y_{n+1}, renormalizes it
 //  (status deferred into err) and advances the particle's position.
 //  STORE 1 (the VJP's recompute): S = 1..3 write Y_{S+1} into ynext only.
-//  S = 1..3 also deposit Y_{S+1} onto gnext (its base already there), the
-//  wave's particles on the matrix cores as the deposit kernel does: positions
-//  are frozen within the step, so the cell's particles are the ones whose
-//  stencils the deposit needs, and the stage state goes from registers to the
-//  grid without a second pass over the particles.
-template <int NF, int S, int STORE, typename T>
+//  With DEP (S = 1..3), the stage also deposits Y_{S+1} onto gnext (its base
+//  already there), the wave's particles on the matrix cores as the deposit
+//  kernel does: positions are frozen within the step, so the cell's particles
+//  are the ones whose stencils the deposit needs, and the stage state goes
+//  from registers to the grid without a second pass over the particles.  That
+//  pays at NF = 2; at NF = 3 the deposit's 36 columns and the gather's 36-value
+//  rows in one kernel ran slower than the two kernels, which NF = 3 keeps.
+template <int NF, int S, int STORE, typename T, bool DEP>
 __global__ __launch_bounds__(64 * kCellWaves) void k_fwd_stage(AdjInT<T> in, Geom g, const int* start, const int* perm,
                                                                int64_t ncell, int bpx, const T* grid, HamParams hp,
                                                                const T* y0, T* ynext, T* acc, const T* Ll,
@@ -767,7 +769,7 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_fwd_stage(AdjInT<T> in, Geo
                                                                unsigned long long* err, double* gnext)
 {
     using WD = WaveDeposit<NF>;
-    constexpr bool DEP = S < 4;
+    static_assert(!DEP || S < 4, "stage 4 deposits nothing");
     constexpr int nh = NH<NF>, S2 = 2 * nh;
     __shared__ double lds[DEP ? kCellWaves * WD::REG : 1];
     const int wave = threadIdx.x >> 6;
@@ -1364,19 +1366,26 @@ int arm_status(emu_ctx* ctx)
     return EMU_OK;
 }
 
-// Stage S reads the deposit of Y_S (grid) and, for S < 4, deposits Y_{S+1}
-// onto gnext
+// Stage S reads the deposit of Y_S (grid) and, for S < 4, leaves the deposit
+// of Y_{S+1} on gnext: within the stage kernel at NF = 2, as a kernel of its
+// own after it at NF = 3.
 template <int NF, int S, int STORE, typename T>
 void fwd_stage(emu_ctx* ctx, const AdjInT<T>& in, const StepWork<T>& w, const double* grid, const T* y0, T* ynext,
                T* acc, const T* Ll, double dt, double* pos, double wscale, double* gnext)
 {
+    constexpr bool DEP = S < 4 && NF == 2;
     const int64_t nc = ncell_of(ctx);
     const int bpx = cell_blocks_per_xcd(nc);
-    if (S < 4) fill_base(ctx, gnext, wscale);
+    if (DEP) fill_base(ctx, gnext, wscale);
     const T* gg = gather_grid<T>(ctx, w, grid);
-    hipLaunchKernelGGL((k_fwd_stage<NF, S, STORE, T>), dim3(8 * bpx), dim3(64 * kCellWaves), 0, ctx->stream, in,
+    hipLaunchKernelGGL((k_fwd_stage<NF, S, STORE, T, DEP>), dim3(8 * bpx), dim3(64 * kCellWaves), 0, ctx->stream, in,
                        ctx->g, w.cs.start, w.cs.perm, nc, bpx, gg, ham_params(ctx, wscale), y0, ynext, acc, Ll,
                        ctx->p.maxError, dt, pos, ctx->d_red + 5, gnext);
+    if (S < 4 && !DEP) {
+        AdjInT<T> next = in;
+        next.Y = ynext;
+        deposit(ctx, next, w.cs, gnext, wscale);
+    }
 }
 
 // A step's kept data (emu_adj_step_keep): the stage states Y2, Y3, Y4
