@@ -295,6 +295,14 @@ def run_plain(args):
     parms = configs.WORKLOADS[args.workload]()
     emu = core.Emu(parms)
     parts = core.FlavoredNeutrinoContainer.init_particles(emu)
+    if args.shuffle:
+        # the particles in a random order (the reference's containers hold them in
+        # no particular order): the deposit and gather then list them by box
+        g = torch.Generator(device="cpu").manual_seed(1)
+        perm = torch.randperm(parts.np, generator=g).to(parts.pos.device)
+        parts.pos = parts.pos[:, perm].contiguous()
+        parts.rdata = parts.rdata[:, perm].contiguous()
+        parts.ids = parts.ids[perm].contiguous()
     integ = core.Integrator(emu, parts)
     integ.integrate(args.warmup)
     torch.cuda.synchronize()
@@ -317,7 +325,7 @@ def run_plain(args):
         "data": "synthetic (seeded initial conditions of the workload, generated on the GPU)",
         "config": {"workload": f"{args.workload}: {parms.nf} flavours, {parms.ncell[0]}x{parms.ncell[1]}x{parms.ncell[2]} "
                                f"periodic cells, {round(ppc)} directions/cell, {parts.np} particles",
-                   "particles": parts.np, "path": "plain"},
+                   "particles": parts.np, "path": "plain", "order": "shuffled" if args.shuffle else "cell"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes_per_update": bpu,
                      "note": "wall time of the whole step (every launch), not one kernel"},
@@ -341,6 +349,8 @@ def main():
                     help="time one gradient through the hand-written adjoint instead (forward with checkpoints and "
                          "the reverse sweep; the 2-flavour 32^3 x 92 workload, 200 steps, checkpoints every 20, "
                          "unless --workload / --steps say otherwise)")
+    ap.add_argument("--shuffle", action="store_true",
+                    help="--path plain: the particles in a random order instead of the initializer's cell order")
     ap.add_argument("--segment", default="keep", choices=["keep", "store", "recompute"],
                     help="--adjoint: what the reverse sweep holds per segment")
     ap.add_argument("--checkpoint-every", type=int, default=20, help="--adjoint: steps per checkpoint")
