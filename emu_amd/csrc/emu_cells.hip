@@ -6,17 +6,19 @@
 // Each particle's stencil lies in the 3 x 3 x 3 box (one cell wide in the
 // dimensions of shape order 0) around its nearest cell: shape3's window.
 // A counting sort lists the particles by that box (the key: the fab-box index
-// of the box's first cell), and one wave takes one box:
-//  - the deposit (Evolve.cpp:91-126, fill generate_code.py:150-169) sums
-//    D[t][q] = sum_p w_t(p) Q_p,q over the box's particles on the f64
-//    16x16x4 matrix cores (t: the box's cells, q: the 8 NF^2 deposited
-//    components N f, N f p/p_t for nu and nubar) and adds D to the fab with
-//    one atomic per value, 27 x 8 NF^2 per box instead of per particle;
-//  - the gather (Evolve.cpp:128-176, fills generate_code.py:239-389) reads
-//    the box's fields as wave-uniform (scalar) loads of a cell-major copy
-//    that combines each cell's nu and nubar fields as the gather uses them:
-//    per component c, (N + s Nbar + matter delta_c0) and (F_d + s Fbar_d),
-//    s = -1 for real and +1 for imaginary parts (generate_code.py:317-323).
+// of the box's first cell):
+//  - the deposit (Evolve.cpp:91-126, fill generate_code.py:150-169) takes
+//    one box per wave and sums D[t][q] = sum_p w_t(p) Q_p,q over the box's
+//    particles on the f64 matrix cores (t: the box's cells, q: the 8 NF^2
+//    deposited components N f, N f p/p_t for nu and nubar); a workgroup's
+//    boxes merge their sums in LDS and add them to the fab with one atomic
+//    per value, instead of 27 x 8 NF^2 per particle;
+//  - the gather (Evolve.cpp:128-176, fills generate_code.py:239-389) takes
+//    two boxes per wave and reads their fields from LDS, where the workgroup
+//    stages its run of boxes from a cell-major copy that combines each
+//    cell's nu and nubar fields as the gather uses them: per component c,
+//    (N + s Nbar + matter delta_c0) and (F_d + s Fbar_d), s = -1 for real and
+//    +1 for imaginary parts (generate_code.py:317-323).
 //
 // The per-particle formulas are the reference's; the order of the sums
 // differs (the deposit's also between runs, as with the reference's GPU
@@ -146,32 +148,63 @@ __device__ __forceinline__ int64_t box_block(int blocks_per_xcd)
     return (int64_t)(b & 7) * blocks_per_xcd + (b >> 3);
 }
 
-constexpr int kWavesGather = 4;
+// The gather takes boxes in runs of kGatherRun along x, one run per
+// workgroup and two consecutive boxes per wave.  The run's boxes overlap, so
+// the workgroup stages their union once (kGatherRun + 2 cells along x
+// instead of 3 per box), and a wave's two boxes fill its lanes better than
+// one: 92 particles per box take 3 passes of 64 per pair instead of 4.
+constexpr int kGatherRun = 8;
+constexpr int kWavesGather = kGatherRun / 2;
 
 template <int NF, int WX, int WY, int WZ>
-__global__ __launch_bounds__(64 * kWavesGather) void k_gather_boxes(PView P, Fab fab, Geom g, const double* comb_,
-                                                                     const double* __restrict__ M2,
-                                                                     const int* __restrict__ start,
-                                                                     const int* __restrict__ perm, int nk,
-                                                                     int blocks_per_xcd)
+__global__ __launch_bounds__(64 * kWavesGather) __attribute__((amdgpu_waves_per_eu(5))) void k_gather_boxes(
+    PView P, Fab fab, Geom g, const double* __restrict__ comb, const double* __restrict__ M2,
+    const int* __restrict__ start, const int* __restrict__ perm, int nrun_x, int64_t nruns, int blocks_per_xcd)
 {
     using I = PIdx<NF>;
-    constexpr int nh = NH<NF>, QG = 4 * nh;
-    // the box's fields are the same for every lane: read them as scalar loads
-    typedef __attribute__((address_space(4))) const double* cdp;
-    const cdp comb = (cdp)comb_;
-    const int lane = threadIdx.x & 63;
-    const int64_t box = box_block(blocks_per_xcd) * kWavesGather + (threadIdx.x >> 6);
-    if (box >= nk) return;
-    const int kb = __builtin_amdgcn_readfirstlane((int)box);
+    constexpr int nh = NH<NF>, QG = 4 * nh, NT = WX * WY * WZ, UX = kGatherRun + WX - 1;
+    // the run's fields, the same for every lane of a box: a lane's read of a
+    // cell's values is an LDS broadcast (two addresses per wave, one per box)
+    __shared__ double2 run_lds[WZ * WY * UX * QG / 2];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t run = box_block(blocks_per_xcd);
+    if (run >= nruns) return; // the whole workgroup
+    const int nf0 = fab.nf[0], nf1 = fab.nf[1];
+    const int rx = (int)(run % nrun_x);
+    const int64_t ryz = run / nrun_x;
+    const int by = (int)(ryz % nf1), bz = (int)(ryz / nf1);
+    const int bx0 = rx * kGatherRun;
+    const int nbx = min(kGatherRun, nf0 - bx0);                    // boxes in the run
+    const int64_t kb0 = ((int64_t)bz * nf1 + by) * nf0 + bx0;      // fab-box index of the first
+    if (start[kb0] == start[kb0 + nbx]) return;                    // no particles: the whole workgroup
+    {
+        // a box with particles lies inside the fab, so rows by..by+WY-1 and
+        // bz..bz+WZ-1 exist; along x the union is clamped to the fab
+        const int ux = min(nbx + WX - 1, nf0 - bx0);
+        const int row2 = ux * QG / 2;
+        const int64_t sy = nf0, sz = (int64_t)nf0 * nf1;
+        for (int e = threadIdx.x; e < WY * WZ * row2; e += 64 * kWavesGather) {
+            const int r = e / row2, wi = e - r * row2;
+            const int jj = r % WY, kk = r / WY;
+            run_lds[r * (UX * QG / 2) + wi] =
+                reinterpret_cast<const double2*>(comb + (kb0 + kk * sz + jj * sy) * QG)[wi];
+        }
+        __syncthreads();
+    }
+    if (2 * wave >= nbx) return;
+    const int nb = min(2, nbx - 2 * wave);
+    const int64_t kb = kb0 + 2 * wave;
     const int beg = __builtin_amdgcn_readfirstlane(start[kb]);
-    const int end = __builtin_amdgcn_readfirstlane(start[kb + 1]);
+    const int mid = __builtin_amdgcn_readfirstlane(start[kb + 1]);
+    const int end = __builtin_amdgcn_readfirstlane(start[kb + nb]);
     if (beg >= end) return;
-    const int64_t sy = fab.nf[0], sz = (int64_t)fab.nf[0] * fab.nf[1];
     const double coef = M_SQRT2 * phys::GF * g.inv_cell_volume;
+    const double* runf = reinterpret_cast<const double*>(run_lds);
     for (int base = beg; base < end; base += 64) {
         const bool act = base + lane < end;
         const int64_t ip = perm[act ? base + lane : beg];
+        // this lane's box: the wave's first or second
+        const double* boxf = runf + (2 * wave + (base + lane >= mid ? 1 : 0)) * QG;
         const double pupx = P.r[I::pupx][ip], pupy = P.r[I::pupy][ip], pupz = P.r[I::pupz][ip],
                      pupt = P.r[I::pupt][ip];
         int o[3];
@@ -181,16 +214,15 @@ __global__ __launch_bounds__(64 * kWavesGather) void k_gather_boxes(PView P, Fab
         double in[nh];
 #pragma unroll
         for (int c = 0; c < nh; c++) in[c] = 0.0;
-        // one box cell per iteration (rolled: the box's 27 x 4 NH fields would
-        // not fit the scalar registers at once): per field v the lane's
-        // coefficient w_t s_v, s = (1, -v_x, -v_y, -v_z), times the NH
-        // uniform values of that field
+        // one box cell per iteration: per field v the lane's coefficient
+        // w_t s_v, s = (1, -v_x, -v_y, -v_z), times the NH uniform values of
+        // that field
         auto pick = [](const double* x, int i) { return i == 0 ? x[0] : (i == 1 ? x[1] : x[2]); };
 #pragma unroll 1
-        for (int t = 0; t < WX * WY * WZ; t++) {
+        for (int t = 0; t < NT; t++) {
             const int ii = t % WX, jj = (t / WX) % WY, kk = t / (WX * WY);
             const double wt = pick(w[0], WX == 3 ? ii : 1) * pick(w[1], WY == 3 ? jj : 1) * pick(w[2], WZ == 3 ? kk : 1);
-            const cdp G = comb + ((int64_t)kb + kk * sz + jj * sy + ii) * QG;
+            const double* G = boxf + ((kk * WY + jj) * UX + ii) * QG;
             const double a[4] = {wt, -wt * vx, -wt * vy, -wt * vz};
 #pragma unroll
             for (int v = 0; v < 4; v++)
@@ -259,16 +291,18 @@ template <int WX, int WY, int WZ> struct DepPatch {
     static constexpr int RX = PX + WX - 1, RY = PY + WY - 1, RZ = PZ + WZ - 1, NR = RX * RY * RZ;
 };
 
-// Particles staged per chunk, and the LDS row of one: 9 shape factors, then
-// its sources: N f_c and Nbar fbar_c (2 NH), the multipliers (1, p/p_t) of
-// N, Fx, Fy, Fz, and a zero for the padding columns; odd strides, so that four
-// particle rows read in one instruction fall on different banks.
+// Particles staged per chunk, and the LDS row of one: the products of its
+// shape factors for the box's NT cells (t = ii + WX (jj + WY kk), the matrix
+// product's A row), then its sources: N f_c and Nbar fbar_c (2 NH), the
+// multipliers (1, p/p_t) of N, Fx, Fy, Fz, and a zero for the padding
+// columns.  Both strides are odd, so that four particle rows read in one
+// instruction fall on different banks.
 constexpr int kDepChunk = 32;
-template <int NF> struct DepRow {
+template <int NF, int NT> struct DepRow {
     static constexpr int nh = NH<NF>;
-    static constexpr int W = 9;
+    static constexpr int W = NT;
     static constexpr int S = 2 * nh + 5;
-    static constexpr int ZERO = 2 * nh + 4;
+    static constexpr int ONE = 2 * nh, ZERO = 2 * nh + 4;
     static constexpr int doubles_per_wave = kDepChunk * (W + S);
 };
 
@@ -276,7 +310,7 @@ template <int NF, int WX, int WY, int WZ>
 constexpr int dep_lds_doubles()
 {
     using PT = DepPatch<WX, WY, WZ>;
-    const int stage = PT::WAVES * DepRow<NF>::doubles_per_wave, merge = PT::NR * (8 * NH<NF> + 1);
+    const int stage = PT::WAVES * DepRow<NF, WX * WY * WZ>::doubles_per_wave, merge = PT::NR * (8 * NH<NF> + 1);
     return stage > merge ? stage : merge;
 }
 
@@ -286,12 +320,14 @@ __global__ __launch_bounds__((64 * DepPatch<WX, WY, WZ>::WAVES)) __attribute__((
     int64_t npatch, int blocks_per_xcd)
 {
     using I = PIdx<NF>;
-    using R = DepRow<NF>;
     using PT = DepPatch<WX, WY, WZ>;
     constexpr int nh = NH<NF>, NQ = 8 * nh;         // deposited components
+    constexpr int NT = WX * WY * WZ;
+    using R = DepRow<NF, NT>;
     // full 16-column tiles (f64 16x16x4), then 4-column blocks (f64 4x4x4,
     // four row blocks) for the rest: NF = 3's 72 columns are 4 tiles + 2 blocks
-    constexpr int NT = WX * WY * WZ, MT = (NT + 15) / 16, NTF = NQ / 16, NSM = (NQ - 16 * NTF + 3) / 4;
+    constexpr int MT = (NT + 15) / 16, NTF = NQ / 16, NSM = (NQ - 16 * NTF + 3) / 4;
+    static_assert(PT::NR <= 64, "one lane per region cell in the flush");
     __shared__ double lds[dep_lds_doubles<NF, WX, WY, WZ>()];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t patch = box_block(blocks_per_xcd);
@@ -306,19 +342,8 @@ __global__ __launch_bounds__((64 * DepPatch<WX, WY, WZ>::WAVES)) __attribute__((
     const int end = have ? __builtin_amdgcn_readfirstlane(start[kb + 1]) : 0;
     double* Wf = lds + wave * R::doubles_per_wave;
     double* Src = Wf + kDepChunk * R::W;
-    // operand indices: A row t = 16m + l%16 is the product of the particle's
-    // weights at the target's (i, j, k) (rows past the box's NT cells are
-    // padding: tdx[m][2] = -1 gives them A = 0); B column q = (2 v + bar) NH + c
-    // is source (bar, c) times multiplier v (fab component order, GIdx)
-    int tdx[MT][3];
-#pragma unroll
-    for (int m = 0; m < MT; m++) {
-        const int t = 16 * m + (lane & 15);
-        const int ii = t % WX, jj = (t / WX) % WY, kk = t / (WX * WY);
-        tdx[m][0] = WX == 3 ? ii : 1;
-        tdx[m][1] = 3 + (WY == 3 ? jj : 1);
-        tdx[m][2] = t < NT ? 6 + (WZ == 3 ? kk : 1) : -1;
-    }
+    // B column q = (2 v + bar) NH + c is source (bar, c) times multiplier v
+    // (fab component order, GIdx); columns past NQ read the zero
     auto src1 = [&](int q) { return q < NQ ? ((q / nh) % 2) * nh + q % nh : R::ZERO; };
     auto src2 = [&](int q) { return q < NQ ? 2 * nh + (q / nh) / 2 : R::ZERO; };
     int bi1[NTF], bi2[NTF], bs1[NSM > 0 ? NSM : 1], bs2[NSM > 0 ? NSM : 1];
@@ -342,6 +367,19 @@ __global__ __launch_bounds__((64 * DepPatch<WX, WY, WZ>::WAVES)) __attribute__((
         for (int k = 0; k < NSM; k++) accs[m][k] = 0.0;
     }
     const int cl = lane & (kDepChunk - 1); // the chunk row this lane stages (lanes 32-63 repeat 0-31)
+    const bool row = lane < kDepChunk;
+    double* wr = Wf + cl * R::W;
+    double* sr = Src + cl * R::S;
+    if (row) {
+        sr[R::ONE] = 1.0;
+        sr[R::ZERO] = 0.0;
+    }
+    // operand pointers of this lane: particle row 4 ks + lane / 16 of k-step
+    // ks; A row t = 16 m + lane % 16 (rows past NT are padding that reads
+    // in-range values: row t of the product depends only on row t of A, and
+    // those rows are never flushed)
+    const double* wl = Wf + (lane >> 4) * R::W + (lane & 15);
+    const double* sl = Src + (lane >> 4) * R::S;
     // each chunk's permutation entry is loaded one chunk ahead, so that the
     // particle loads wait on one round trip instead of two
     int ip_next = beg < end ? perm[beg + cl < end ? beg + cl : beg] : 0;
@@ -354,10 +392,8 @@ __global__ __launch_bounds__((64 * DepPatch<WX, WY, WZ>::WAVES)) __attribute__((
         double w[3][3];
         particle_box(P, ip, fab, g, o, w);
         // every lane loads (inactive ones a valid particle), so that the loads
-        // issue together, nu's batch then nubar's; inactive lanes stage zeros
-        double* wr = Wf + cl * R::W;
-        double* sr = Src + cl * R::S;
-        const bool row = lane < kDepChunk;
+        // issue together, nu's batch then nubar's.  An inactive row gets zero
+        // weights: its sources then add nothing.
         {
             const double pupt = P.r[I::pupt][ip], pv[3] = {P.r[I::pupx][ip], P.r[I::pupy][ip], P.r[I::pupz][ip]};
             const double N = P.r[I::N][ip];
@@ -365,16 +401,23 @@ __global__ __launch_bounds__((64 * DepPatch<WX, WY, WZ>::WAVES)) __attribute__((
 #pragma unroll
             for (int c = 0; c < nh; c++) fv[c] = P.r[I::f + c][ip];
             if (row) {
+                auto pick = [](const double* x, int i) { return i == 0 ? x[0] : (i == 1 ? x[1] : x[2]); };
 #pragma unroll
-                for (int d = 0; d < 3; d++)
+                for (int kk = 0; kk < WZ; kk++)
 #pragma unroll
-                    for (int e = 0; e < 3; e++) wr[3 * d + e] = act ? w[d][e] : 0.0;
+                    for (int jj = 0; jj < WY; jj++) {
+                        const double yz = pick(w[1], WY == 3 ? jj : 1) * pick(w[2], WZ == 3 ? kk : 1);
 #pragma unroll
-                for (int c = 0; c < nh; c++) sr[c] = act ? N * fv[c] : 0.0;
-                sr[2 * nh] = 1.0;
+                        for (int ii = 0; ii < WX; ii++) {
+                            const double x = pick(w[0], WX == 3 ? ii : 1);
+                            wr[(kk * WY + jj) * WX + ii] = (act ? x : 0.0) * yz;
+                        }
+                    }
 #pragma unroll
-                for (int d = 0; d < 3; d++) sr[2 * nh + 1 + d] = pv[d] / pupt;
-                sr[R::ZERO] = 0.0;
+                for (int c = 0; c < nh; c++) sr[c] = N * fv[c];
+                const double ipt = 1.0 / pupt;
+#pragma unroll
+                for (int d = 0; d < 3; d++) sr[R::ONE + 1 + d] = pv[d] * ipt;
             }
         }
         {
@@ -384,32 +427,38 @@ __global__ __launch_bounds__((64 * DepPatch<WX, WY, WZ>::WAVES)) __attribute__((
             for (int c = 0; c < nh; c++) fv[c] = P.r[I::fbar + c][ip];
             if (row) {
 #pragma unroll
-                for (int c = 0; c < nh; c++) sr[nh + c] = act ? Nb * fv[c] : 0.0;
+                for (int c = 0; c < nh; c++) sr[nh + c] = Nb * fv[c];
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const int nks = (min(kDepChunk, end - base) + 3) >> 2; // k-steps of 4 particles
-        for (int ks = 0; ks < nks; ks++) {
-            const int k = 4 * ks + (lane >> 4);
-            const double* wk = Wf + k * R::W;
-            const double* sk = Src + k * R::S;
-            double A[MT];
+        // k-steps of 4 particles, two per iteration (the second at immediate
+        // offsets); an odd count runs one more k-step over zero-weight rows
+        int nks = (min(kDepChunk, end - base) + 3) >> 2;
+        nks += nks & 1;
+        for (int ks = 0; ks < nks; ks += 2) {
 #pragma unroll
-            for (int m = 0; m < MT; m++)
-                A[m] = tdx[m][2] < 0 ? 0.0 : wk[tdx[m][0]] * wk[tdx[m][1]] * wk[tdx[m][2]];
+            for (int h = 0; h < 2; h++) {
+                const double* wk = wl + (ks + h) * 4 * R::W;
+                const double* sk = sl + (ks + h) * 4 * R::S;
+                double A[MT];
 #pragma unroll
-            for (int n = 0; n < NTF; n++) {
-                const double Bv = sk[bi1[n]] * sk[bi2[n]];
+                for (int m = 0; m < MT; m++) A[m] = wk[16 * m];
 #pragma unroll
-                for (int m = 0; m < MT; m++) acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[m], Bv, acc[m][n], 0, 0, 0);
-            }
+                for (int n = 0; n < NTF; n++) {
+                    const double Bv = sk[bi1[n]] * sk[bi2[n]];
 #pragma unroll
-            for (int k = 0; k < NSM; k++) {
-                const double Bs = sk[bs1[k]] * sk[bs2[k]];
+                    for (int m = 0; m < MT; m++)
+                        acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[m], Bv, acc[m][n], 0, 0, 0);
+                }
 #pragma unroll
-                for (int m = 0; m < MT; m++) accs[m][k] = __builtin_amdgcn_mfma_f64_4x4x4f64(A[m], Bs, accs[m][k], 0, 0, 0);
+                for (int k = 0; k < NSM; k++) {
+                    const double Bs = sk[bs1[k]] * sk[bs2[k]];
+#pragma unroll
+                    for (int m = 0; m < MT; m++)
+                        accs[m][k] = __builtin_amdgcn_mfma_f64_4x4x4f64(A[m], Bs, accs[m][k], 0, 0, 0);
+                }
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -449,17 +498,20 @@ __global__ __launch_bounds__((64 * DepPatch<WX, WY, WZ>::WAVES)) __attribute__((
             }
     }
     __syncthreads();
-    // one component per wave-wide pass over the region's cells, so that a wave's
-    // atomics fall on runs of neighbouring cells of one fab component
-    const int ox = ppx * PT::PX, oy = ppy * PT::PY, oz = ppz * PT::PZ;
-    for (int e = threadIdx.x; e < PT::NR * NQ; e += 64 * PT::WAVES) {
-        const int q = e / PT::NR, rc = e - q * PT::NR;
-        const double v = Mb[rc * NQS + q];
-        if (v == 0.0) continue;
-        const int cx = ox + rc % PT::RX, cy = oy + (rc / PT::RX) % PT::RY, cz = oz + rc / (PT::RX * PT::RY);
-        if (cx >= fab.nf[0] || cy >= fab.nf[1] || cz >= fab.nf[2]) continue;
-        const int64_t cell = ((int64_t)cz * fab.nf[1] + cy) * fab.nf[0] + cx;
-        unsafeAtomicAdd(&fab.d[(int64_t)(3 + q) * fab.cstride + cell], v);
+    // lane l takes region cell l, and each wave a quarter of the components:
+    // a wave's atomics per pass fall on the region's cells of one fab component
+    {
+        const int rc = lane;
+        const int cx = ppx * PT::PX + rc % PT::RX, cy = ppy * PT::PY + (rc / PT::RX) % PT::RY;
+        const int cz = ppz * PT::PZ + rc / (PT::RX * PT::RY);
+        if (rc < PT::NR && cx < fab.nf[0] && cy < fab.nf[1] && cz < fab.nf[2]) {
+            double* dst = fab.d + 3 * fab.cstride + ((int64_t)cz * fab.nf[1] + cy) * fab.nf[0] + cx;
+            const double* srcm = Mb + rc * NQS;
+            for (int q = wave; q < NQ; q += PT::WAVES) {
+                const double v = srcm[q];
+                if (v != 0.0) unsafeAtomicAdd(dst + (int64_t)q * fab.cstride, v);
+            }
+        }
     }
 }
 
@@ -476,10 +528,11 @@ void launch_boxes(bool deposit, hipStream_t s, const PView& P, const Fab& fab, c
         hipLaunchKernelGGL((k_deposit_boxes<NF, WX, WY, WZ>), dim3(8 * bpx), dim3(64 * PT::WAVES), 0, s, P, fab, g,
                            start, perm, npx, npy, npatch, bpx);
     } else {
-        const int64_t nblock = ((int64_t)nk + kWavesGather - 1) / kWavesGather;
-        const int bpx = (int)((nblock + 7) / 8);
+        const int nrun_x = (fab.nf[0] + kGatherRun - 1) / kGatherRun;
+        const int64_t nruns = (int64_t)nrun_x * fab.nf[1] * fab.nf[2];
+        const int bpx = (int)((nruns + 7) / 8);
         hipLaunchKernelGGL((k_gather_boxes<NF, WX, WY, WZ>), dim3(8 * bpx), dim3(64 * kWavesGather), 0, s, P, fab, g,
-                           comb, M2, start, perm, nk, bpx);
+                           comb, M2, start, perm, nrun_x, nruns, bpx);
     }
 }
 

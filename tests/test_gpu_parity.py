@@ -103,7 +103,8 @@ def test_init_sim_types_match_oracle(core, oracle, nf, st):
 
 
 @pytest.mark.parametrize("nf", [2, 3])
-@pytest.mark.parametrize("ncell", [(4, 3, 5), (1, 1, 6), (2, 2, 2)])
+# (13, 2, 3): fab rows longer than the gather's 8-box runs, the last run partial
+@pytest.mark.parametrize("ncell", [(4, 3, 5), (1, 1, 6), (2, 2, 2), (13, 2, 3)])
 def test_deposit_fill_interpolate_parity(core, oracle, nf, ncell):
     L = tuple(0.5 * n for n in ncell)
     d = cases.fast_flavor_dict(nf=nf, ncell=ncell, L=L, rho_g_ccm=1e10, Ye=0.45)
