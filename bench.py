@@ -243,10 +243,10 @@ def run_adjoint(args):
         warm.gradients(warm.f0.clone(), 1.0)
         del warm
     prob = adjoint.AdjointProblem(emu, parts, dt=dt, nsteps=nsteps, checkpoint_every=cp, segment=args.segment,
-                                  precision=prec)
+                                  precision=prec, snapshots=args.snapshots)
     np_total = parts.np
-    f0 = prob.f0
-    prob.f0 = None  # the run takes over f0's memory
+    f0 = [prob.f0]  # the run takes over f0's memory: nothing else holds it
+    prob.f0 = None
     del parts
     footprint = prob.footprint_bytes()
     torch.cuda.empty_cache()
@@ -254,10 +254,22 @@ def run_adjoint(args):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     J, lam, cps, Nn, order = prob.run(f0, 1.0, consume=True)
-    del f0
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    g = prob.adjoint_sweep(lam, cps, Nn, order)
+    peak_fwd = torch.cuda.max_memory_allocated() / 2**30
+    torch.cuda.reset_peak_memory_stats()
+    print(f"forward run: {nsteps} steps in {t1 - t0:.1f} s, {peak_fwd:.1f} GiB at the peak", file=sys.stderr,
+          flush=True)
+
+    def progress(n):
+        # a line every 10 reverse steps on long sweeps (the 128^3 box)
+        if large and n % 10 == 0:
+            print(f"reverse step {n} done at {time.perf_counter() - t1:.1f} s, "
+                  f"{torch.cuda.max_memory_allocated() / 2**30:.1f} GiB at the peak so far", file=sys.stderr,
+                  flush=True)
+
+    lam = [lam]  # the sweep holds dJ/dy_N alone, so that it goes after the first reverse step
+    g = prob.adjoint_sweep(lam, cps, Nn, order, progress=progress)
     dratio = float(prob.ratio_gradient().item())
     torch.cuda.synchronize()
     t2 = time.perf_counter()
@@ -274,9 +286,12 @@ def run_adjoint(args):
         "config": {"workload": f"{workload}: {parms.nf} flavours, {parms.ncell[0]}x{parms.ncell[1]}x{parms.ncell[2]} "
                                f"periodic cells, {round(ppc)} directions/cell, {np_total} particles",
                    "particles": np_total, "checkpoint_every": cp, "segment": args.segment, "precision": prec,
-                   "dt_s": dt},
-        "footprint_gib": footprint / 2**30, "peak_allocated_gib": torch.cuda.max_memory_allocated() / 2**30,
+                   "dt_s": dt, **({"snapshots": prob.snapshots} if args.segment == "binomial" else {})},
+        "footprint_gib": footprint / 2**30,
+        "peak_allocated_gib": max(peak_fwd, torch.cuda.max_memory_allocated() / 2**30),
+        "peak_forward_gib": peak_fwd, "peak_reverse_gib": torch.cuda.max_memory_allocated() / 2**30,
         "forward_ms_per_step": (t1 - t0) / nsteps * 1e3, "reverse_ms_per_step": (t2 - t1) / nsteps * 1e3,
+        "replayed_steps": prob.steps_run - nsteps,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes_per_particle_step": bps},
         "J": J, "dJ_dratio": dratio, "grad_f0_norm": float(g.norm().item()),
@@ -351,8 +366,10 @@ def main():
                          "unless --workload / --steps say otherwise)")
     ap.add_argument("--shuffle", action="store_true",
                     help="--path plain: the particles in a random order instead of the initializer's cell order")
-    ap.add_argument("--segment", default="keep", choices=["keep", "store", "recompute"],
+    ap.add_argument("--segment", default="keep", choices=["keep", "store", "recompute", "binomial"],
                     help="--adjoint: what the reverse sweep holds per segment")
+    ap.add_argument("--snapshots", type=int, default=2,
+                    help="--adjoint --segment binomial: step states held at once per segment")
     ap.add_argument("--checkpoint-every", type=int, default=20, help="--adjoint: steps per checkpoint")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device (5 times): the known-size kernel that scales the "

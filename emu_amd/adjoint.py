@@ -15,8 +15,11 @@ reverse step then recomputes nothing), keeping the step states only
 (``segment="store"``: (nsteps/cp + cp) states in memory; each reverse step
 recomputes its stages) or recomputing each step from the segment's start
 (``segment="recompute"``: nsteps/cp + 1 states, about cp/2 more forward
-steps per step).  The footprint is checked against ``memory_budget`` bytes
-before anything runs.
+steps per step), or reversing each segment on a binomial schedule with
+``snapshots`` more step states (``segment="binomial"``, Griewank's revolve:
+t steps with s snapshots cost about r t forward steps, r the least count
+with C(s + r, s) >= t, instead of t^2/2).  The footprint is checked against
+``memory_budget`` bytes before anything runs.
 
 Each forward step is one library call (emu_adj_step: one cell list, the four
 stages with their deposits, Renormalize with a deferred status, the position
@@ -43,12 +46,65 @@ def _p(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
+def _take(x):
+    """x, or the one tensor a one-element list holds (the list is emptied).
+    A call's arguments stay referenced by the caller until the call returns,
+    so a large array handed over in a list is the only way the callee can
+    free it early."""
+    return x.pop() if isinstance(x, list) else x
+
+
+def binomial_split(t: int, s: int) -> int:
+    """Where the binomial schedule stores its next snapshot when reversing t
+    steps from a held state with s free snapshot slots: after m steps, so that
+    the t - m steps to its right are reversed from the new snapshot with s - 1
+    free slots and the m to its left from the held state with s.  With the
+    held state counted, S = s + 1 states reverse at most b(S, r) = C(S + r, S)
+    steps when no step runs forward more than r times (Griewank, Algorithm
+    799: revolve, ACM TOMS 26, 2000), at the least cost r t - C(S + r, r - 1)
+    for the least such r.  The split reaches that cost when the right part
+    fits S - 1 states in r runs but needs more than r - 1 (t - m <= b(S - 1, r),
+    t - m >= b(S - 1, r - 1)) and the left part fits S states in r - 1 runs
+    but needs more than r - 2 (m <= b(S, r - 1), m >= b(S, r - 2)); the
+    smallest such m is taken."""
+    if t < 2 or s < 1:
+        raise ValueError("binomial_split needs t >= 2 steps and s >= 1 slots")
+    S = s + 1
+
+    def b(k, r):
+        return math.comb(k + r, k) if r >= 0 else 0
+
+    r = 1
+    while b(S, r) < t:
+        r += 1
+    return max(1, t - b(S - 1, r), b(S, r - 2))
+
+
+def binomial_reverse(start, n0: int, n1: int, s: int, advance, reverse):
+    """Reverse the steps n0 .. n1-1, last first, from `start`, the state at
+    step n0, holding at most s more states at once.  advance(state, n) returns
+    the state at step n >= state's step, run forward from `state` (a new
+    state: `state` stays valid); reverse(state_at_n, n) goes back through
+    step n.  With s = 0 every step is run again from `start`."""
+    while n1 > n0:
+        t = n1 - n0
+        if t == 1 or s == 0:
+            for n in range(n1 - 1, n0 - 1, -1):
+                reverse(start if n == n0 else advance(start, n), n)
+            return
+        m = n0 + binomial_split(t, s)
+        snap = advance(start, m)
+        binomial_reverse(snap, m, n1, s - 1, advance, reverse)
+        del snap
+        n1 = m
+
+
 class AdjointProblem:
     """Fixed-order particle data and scratch for forward/adjoint sweeps."""
 
     def __init__(self, emu, parts, dt: float, nsteps: int, checkpoint_every: int | None = None,
                  segment: str = "keep", memory_budget: float | None = None, resort_every: int = 4,
-                 precision: str = "fp64"):
+                 precision: str = "fp64", snapshots: int = 2):
         self.emu = emu
         if precision not in ("fp64", "fp32"):
             raise ValueError("precision is 'fp64' or 'fp32'")
@@ -65,19 +121,23 @@ class AdjointProblem:
         # the particle arrays go back to cell order every `resort` steps, and at
         # every checkpoint (so resort divides cp)
         self.resort = math.gcd(self.cp, max(1, int(resort_every)))
-        if segment not in ("keep", "store", "recompute"):
-            raise ValueError("segment is 'keep', 'store' or 'recompute'")
+        if segment not in ("keep", "store", "recompute", "binomial"):
+            raise ValueError("segment is 'keep', 'store', 'recompute' or 'binomial'")
         self.segment = segment
+        # binomial: step states held at once within a segment, beside its checkpoint
+        self.snapshots = max(0, int(snapshots))
         self.keep_doubles = int(self._fn("emu_adj_step_keep_doubles")(emu.ctx, self.np))
         self.step_work_doubles = int(self._fn("emu_adj_step_work_doubles")(emu.ctx, self.np))
         self.work_doubles = int(lib().emu_adj_work_doubles(emu.ctx, self.np)) if precision == "fp64" else 0
         self._keeps = None
+        self.steps_run = 0  # forward steps run so far: the runs' own and the sweeps' replays
         need = self.footprint_bytes()
         if memory_budget is not None and need > memory_budget:
             from .capi import EmuError
             raise EmuError(13, "AdjointProblem",
                            f"the checkpoints and work arrays need {need / 2**30:.1f} GiB, above the budget of "
-                           f"{memory_budget / 2**30:.1f} GiB (fewer checkpoints or segment='recompute')")
+                           f"{memory_budget / 2**30:.1f} GiB (fewer checkpoints, or segment='recompute' or "
+                           f"'binomial' with fewer snapshots)")
         K = emu.K
         r = parts.rdata
         dt_ = self.dtype
@@ -108,25 +168,38 @@ class AdjointProblem:
         return getattr(lib(), name + self._sfx)
 
     def footprint_bytes(self) -> float:
-        """Device bytes of a gradient: the run's checkpoints (flavour state,
-        FP64 positions, cell order), the segment's held step states and kept
-        stage data, the live flavour states and adjoints, the particle
-        constants and the work arrays."""
+        """Device bytes at the peak of a gradient.  Held throughout: the work
+        arrays, the kept stage data, the particle constants, the run's
+        checkpoints and the segment's held step states (a state: flavour
+        state, FP64 positions, cell order; the binomial schedule's snapshots,
+        or a stored segment's steps).  On top, the larger of the two phases:
+          - a reverse step: the state it goes back through, the adjoint and
+            its next value, the step's constants in its order, and the index
+            arrays of the adjoint's move between orders;
+          - a forward step across a re-sort (in the run or a replay): the
+            adjoint, the flavour state and its next or re-sorted copy, two
+            copies of the positions, the old and new constants.
+        On the 128^3 box (FP32-bulk, 100 steps, two binomial snapshots) this
+        says 184 GiB against 178 GiB measured at the peak."""
         nh2, np_ = 2 * self.nh, self.np
         e = 8.0 if self.precision == "fp64" else 4.0
-        state = np_ * (e * nh2 + 8.0 * 3 + 4.0)
+        flav = e * nh2 * np_
+        pos = 8.0 * 3 * np_
+        state = flav + pos + 4.0 * np_
         ncp = -(-self.nsteps // self.cp)
-        held = ncp + (1 if self.segment == "recompute" else self.cp)
+        seg = {"recompute": 0, "binomial": self.snapshots}.get(self.segment, self.cp)
         kept = 8.0 * self.cp * self.keep_doubles if self.segment == "keep" else 0.0
         work = 8.0 * (self.step_work_doubles + self.work_doubles)
-        # pos0; pup, Nn, Ll; f0 (while held); Nn(ratio); the segment's reordered pup, Nn, Ll; lam_nbar twice
-        f0 = e * nh2 if getattr(self, "f0", True) is not None else 0.0
-        consts = np_ * (8.0 * 3 + e * 8 + f0 + e * 2 + e * 8 + 8.0 * 2)
-        live = 4 * e * nh2 * np_  # y and the next y, the adjoint and its next value
-        # a re-sort's new copies of the positions and the flavour state, and the
-        # adjoint's move between orders, live beside the old ones for a moment
-        transient = state + e * nh2 * np_
-        return held * state + kept + work + consts + live + transient
+        step_consts = e * 8 * np_  # pup, Nn, Ll in a step's order
+        # a stored or kept segment holds its steps' constants, one set per re-sort
+        if self.segment in ("keep", "store"):
+            kept += -(-self.cp // self.resort) * step_consts
+        # pos0; pup, Nn, Ll and Nn(ratio) in the problem's order; f0 while held; lam_nbar twice
+        f0 = flav if getattr(self, "f0", True) is not None else 0.0
+        base = work + kept + (ncp + seg) * state + pos + e * 10 * np_ + f0 + 16.0 * np_
+        reverse = state + 2 * flav + step_consts + 24.0 * np_
+        forward = 3 * flav + 2 * pos + 2 * step_consts
+        return base + max(reverse, forward)
 
     # ---- kernels ---------------------------------------------------------
     def _fp64_only(self, what):
@@ -206,6 +279,7 @@ class AdjointProblem:
         pup, Ll = consts if consts is not None else (self.pup, self.Ll)
         out = torch.empty_like(y)
         ws = (self.wscale,) if self.precision == "fp32" else ()
+        self.steps_run += 1
         if keep is None:
             check(self._fn("emu_adj_step")(self.emu.ctx, self.np, _p(pos), _p(pup), _p(Nn), _p(Ll), _p(y), _p(out),
                                            self.dt, *ws, _p(self.step_work)), "emu_adj_step", self.emu.ctx)
@@ -288,6 +362,22 @@ class AdjointProblem:
             if n < upto:
                 y = self.step(pos, consts[1], y, (consts[0], consts[2]))
 
+    def _advance(self, state, Nn, n):
+        """The state (n, pos, y, order) at step n >= state's, run forward from
+        `state` (which stays valid) with the forward run's re-sorts.  Unlike
+        _replay it holds only the current step's arrays, so that a re-sort
+        (a new copy of the positions and the flavour state) comes on top of
+        one state, not two."""
+        m, pos, y, order = state
+        pos = pos.clone()
+        consts = self._segment_consts(order, Nn)
+        for m in range(m, n):
+            y = self.step(pos, consts[1], y, (consts[0], consts[2]))
+            if (m + 1) % self.resort == 0:
+                consts = None
+                pos, y, order, consts = self._resort(pos, y, order, Nn)
+        return n, pos, y, order
+
     def _replay_keep(self, cp, Nn, n1):
         """The steps n0 .. n1-1 of a checkpoint's segment run again (re-sorting
         at the same steps as the forward run), each keeping its stage data:
@@ -309,10 +399,13 @@ class AdjointProblem:
         return out
 
     def _to_order(self, lam, lam_order, order):
-        """lam from lam_order into order."""
+        """lam from lam_order into order, in one gather: element i of the
+        result is particle order[i], which lam holds where lam_order has it."""
         if order is lam_order:
             return lam
-        return self._permute(self._permute(lam, lam_order, 1), order, 0)
+        inv = torch.empty_like(lam_order)
+        inv[lam_order.long()] = torch.arange(self.np, dtype=lam_order.dtype, device=lam_order.device)
+        return self._permute(lam, inv[order.long()], 0)
 
     # ---- whole runs ------------------------------------------------------
     def Nn(self, ratio: float):
@@ -324,10 +417,19 @@ class AdjointProblem:
         """J after nsteps from f0 (problem order); with keep_checkpoints, also
         dJ/dy_N and the checkpoints (n, pos, y, order) for the sweep, and
         the order dJ/dy_N is in.  consume: f0's memory becomes the run's state
-        (no copy; f0 then holds no meaningful values)."""
+        and the initial positions the run's positions (no copies; f0 then
+        holds no meaningful values, and the problem runs no more); f0 given
+        as a one-element list is taken out of it, so that it is freed once the
+        run has moved on."""
+        f0 = _take(f0)
+        if self.pos0 is None:
+            raise RuntimeError("AdjointProblem.run after a run with consume=True")
         Nn = self.Nn(ratio)
-        pos = self.pos0.clone()
+        pos = self.pos0.clone() if not consume else self.pos0
+        if consume:
+            self.pos0 = None
         y = f0 if consume else f0.clone()
+        del f0
         order = torch.arange(self.np, dtype=torch.int32, device=y.device)
         cps = []
         for n in range(self.nsteps):
@@ -340,45 +442,70 @@ class AdjointProblem:
         self.check_status()
         return J / self.wscale**2, dJ, cps, Nn, order
 
-    def adjoint_sweep(self, lam, cps, Nn, lam_order):
-        """Backward through the recorded run: lam = dJ/dy_N (in lam_order) ->
-        dJ/dy_0 in the problem's order; the nubar-weight adjoint ends in
-        self.lam_nbar, in the problem's order too."""
+    def adjoint_sweep(self, lam_end, cps, Nn, order_end, progress=None):
+        """Backward through the recorded run: lam_end = dJ/dy_N (in order_end)
+        -> dJ/dy_0 in the problem's order; the nubar-weight adjoint ends in
+        self.lam_nbar, in the problem's order too.  lam_end given as a
+        one-element list is taken out of it: then nothing else holds it, and
+        it is freed after the first reverse step.  progress(n), when given, is
+        called after the reverse step through step n."""
         lam_nbar = torch.zeros_like(self.lam_nbar)
         self.lam_nbar.zero_()
+        # the adjoint lives in `lam` alone, which each reverse step rebinds (an
+        # argument that a nested function rebinds stays referenced by the
+        # frame, so the arguments are moved out first)
+        lam, lam_order = _take(lam_end), order_end
+        del lam_end, order_end
         nbar_order = lam_order
 
-        def back_through(n, pos, y, order, consts, lam, lam_order, nbar_order, keep=None):
+        def back_through(n, pos, y, order, consts, keep=None):
+            # the sweep's adjoint into this step's order, then back through the
+            # step; each rebinding drops the old array as soon as the new exists
+            nonlocal lam, lam_order, nbar_order
             lam = self._to_order(lam, lam_order, order)
+            lam_order = order
             if order is not nbar_order:
                 self._permute(self.lam_nbar, nbar_order, 2, out=lam_nbar)
                 self.lam_nbar.zero_()
+                nbar_order = order
             lam = self.step_adjoint(pos, consts[1], y, lam, (consts[0], consts[2]), keep=keep)
-            return lam, order, order
+            if progress is not None:
+                progress(n)
 
         for cp in reversed(cps):
             n0 = cp[0]
             n1 = min(n0 + self.cp, self.nsteps)
+            # (each branch lets go of its segment's states before the next segment's are made)
             if self.segment == "keep":
-                for (n, pos, y, order, consts, keep) in reversed(self._replay_keep(cp, Nn, n1)):
-                    lam, lam_order, nbar_order = back_through(n, pos, y, order, consts, lam, lam_order, nbar_order,
-                                                              keep)
+                for st in reversed(self._replay_keep(cp, Nn, n1)):
+                    back_through(*st)
+                del st
             elif self.segment == "store":
                 states = [(n, pos.clone(), y, order, consts) for (n, pos, y, order, consts)
                           in self._replay(cp, Nn, n1 - 1)]
                 for st in reversed(states):
-                    lam, lam_order, nbar_order = back_through(*st, lam, lam_order, nbar_order)
+                    back_through(*st)
+                del states, st
+            elif self.segment == "binomial":
+                def reverse(state, n):
+                    back_through(*state, self._segment_consts(state[3], Nn))
+
+                binomial_reverse(cp, n0, n1, self.snapshots, lambda st, n: self._advance(st, Nn, n), reverse)
             else:
-                for n in range(n1 - 1, n0 - 1, -1):  # each step's start, replayed from the checkpoint
-                    for st in self._replay(cp, Nn, n):
-                        pass
-                    lam, lam_order, nbar_order = back_through(*st, lam, lam_order, nbar_order)
+                for n in range(n1 - 1, n0 - 1, -1):  # each step's start, run again from the checkpoint
+                    st = self._advance(cp, Nn, n)
+                    back_through(*st, self._segment_consts(st[3], Nn))
+                    del st
         self._permute(self.lam_nbar, nbar_order, 2, out=lam_nbar)
         self.check_status()
         self.lam_nbar.copy_(lam_nbar)
         lam = self._permute(lam, lam_order, 1)
+        if self.wscale == 1.0:
+            return lam
         # the unscaled gradient outgrows FP32's range: it comes back in FP64
-        return lam if self.wscale == 1.0 else lam.double() / self.wscale**2
+        # (the power-of-two factor divides out exactly)
+        lam = lam.double()
+        return lam.mul_(1.0 / self.wscale**2)
 
     def ratio_gradient(self):
         """dJ/dratio from the nubar-weight adjoint of the last sweep."""
@@ -387,6 +514,7 @@ class AdjointProblem:
     def gradients(self, f0, ratio: float):
         """J, dJ/df0 and dJ/dratio by the adjoint sweep."""
         J, lam, cps, Nn, order = self.run(f0, ratio)
+        lam = [lam]  # the sweep holds dJ/dy_N alone
         lam = self.adjoint_sweep(lam, cps, Nn, order)
         return J, lam, float(self.ratio_gradient().item())
 
@@ -406,8 +534,10 @@ class FlavorObjective(torch.autograd.Function):
     @staticmethod
     def backward(ctx, gJ):
         p = ctx.problem
-        lam, cps, Nn, order = ctx.saved
-        grad_f0 = p.adjoint_sweep(lam, cps, Nn, order)
-        grad_ratio = p.ratio_gradient()
+        saved = list(ctx.saved)
         ctx.saved = None
+        cps, Nn, order = saved[1:]
+        # the sweep holds dJ/dy_N alone, so that it goes as soon as the first step is through
+        grad_f0 = p.adjoint_sweep([saved.pop(0)], cps, Nn, order)
+        grad_ratio = p.ratio_gradient()
         return (gJ * grad_f0).to(ctx.f0_dtype), gJ * grad_ratio, None

@@ -157,7 +157,7 @@ def test_fused_step_matches_per_rhs_step(mods, oracle, nf, order):
 def test_kept_stages_match_recompute(mods, oracle, nf):
     """emu_adj_step_keep gives emu_adj_step's y_{n+1} and positions, and
     emu_adj_step_vjp_kept (the stage states and deposits taken from the kept
-    buffer) the recomputing VJP's lam_n and lam_Nbar; the three segment modes
+    buffer) the recomputing VJP's lam_n and lam_Nbar; the four segment modes
     give one gradient.  The deposits' atomics add in no set order, so the bar
     is 1e-12 of each array's largest entry, not equality."""
     torch, core, adjoint = mods
@@ -185,10 +185,16 @@ def test_kept_stages_match_recompute(mods, oracle, nf):
     assert close(lb, la)
     assert close(prob.lam_nbar, na)
     grads = {}
-    for seg in ("keep", "store", "recompute"):
+    for seg in ("keep", "store", "recompute", "binomial"):
         p = adjoint.AdjointProblem(emu, c, dt=dt, nsteps=8, checkpoint_every=3, segment=seg)
         grads[seg] = p.gradients(p.f0.clone(), 1.0)
-    for seg in ("store", "recompute"):
+    # the binomial schedule over one 8-step segment with one snapshot: its
+    # snapshots fall between the re-sorts (every 4 steps), and it runs the
+    # fewest forward steps revolve allows (8 for the run, 14 replayed)
+    p = adjoint.AdjointProblem(emu, c, dt=dt, nsteps=8, checkpoint_every=8, segment="binomial", snapshots=1)
+    grads["binomial1"] = p.gradients(p.f0.clone(), 1.0)
+    assert p.steps_run == 8 + 14
+    for seg in ("store", "recompute", "binomial", "binomial1"):
         assert grads[seg][0] == pytest.approx(grads["keep"][0], rel=1e-12)
         assert close(grads[seg][1], grads["keep"][1])
         assert grads[seg][2] == pytest.approx(grads["keep"][2], rel=1e-10)
@@ -196,21 +202,25 @@ def test_kept_stages_match_recompute(mods, oracle, nf):
 
 @pytest.mark.parametrize("nf", [2, 3])
 def test_fp32_segment_modes_agree(mods, oracle, nf):
-    """The FP32-bulk gradient through the three segment modes (kept stage
-    data, kept step states, recompute from the checkpoint), 8 steps with
-    checkpoints every 3: the same J and gradients to FP32 rounding (the FP64
+    """The FP32-bulk gradient through the four segment modes (kept stage
+    data, kept step states, recompute from the checkpoint, the binomial
+    schedule), 8 steps with checkpoints every 3, and the binomial schedule
+    over one 8-step segment: the same J and gradients to FP32 rounding (the FP64
     deposit sums add in no set order, which can move an FP32 state by an
     ulp)."""
     torch, core, adjoint = mods
     emu, c, deck, parts = setup(mods, oracle, nf, amp=1e-3)
     dt = 0.4 * 0.5 / 2.99792458e10 * 0.5
     grads = {}
-    for seg in ("keep", "store", "recompute"):
+    for seg in ("keep", "store", "recompute", "binomial"):
         p = adjoint.AdjointProblem(emu, c, dt=dt, nsteps=8, checkpoint_every=3, segment=seg, precision="fp32")
         grads[seg] = p.gradients(p.f0.clone(), 1.0)
+    p = adjoint.AdjointProblem(emu, c, dt=dt, nsteps=8, checkpoint_every=8, segment="binomial", snapshots=1,
+                               precision="fp32")
+    grads["binomial1"] = p.gradients(p.f0.clone(), 1.0)
     J, g, r = grads["keep"]
     assert g.dtype == torch.float64
-    for seg in ("store", "recompute"):
+    for seg in ("store", "recompute", "binomial", "binomial1"):
         J2, g2, r2 = grads[seg]
         assert J2 == pytest.approx(J, rel=1e-6)
         assert float((g2 - g).abs().max() / g.abs().max()) < 1e-5
@@ -321,3 +331,37 @@ def test_adjoint32_fp32_gradient_tracks_fp64(mods):
     print(f"adjoint32 fp32 vs fp64: J {eJ:.2e}, dJ/dratio {er:.2e}, directional {ed:.2e}")
     assert eJ < 1e-3 and er < 1e-3 and ed < 1e-3
 
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_footprint_bounds_the_peak(mods, prec):
+    """AdjointProblem.footprint_bytes() bounds the device memory a gradient
+    allocates in every segment mode, on the two-flavour 32^3 x 92 workload
+    (20 steps, checkpoints every 10), and a budget below it is refused before
+    anything runs."""
+    torch, core, adjoint = mods
+    from emu_amd import capi, configs
+    emu = core.Emu(configs.adjoint32())
+    c = core.FlavoredNeutrinoContainer.init_particles(emu)
+    st = core.FusedState(emu, c.np, "fp64")
+    st.load(c)
+    dt = st.info()["dt"]
+    st.close()
+    del st
+    for seg in ("keep", "store", "recompute", "binomial"):
+        torch.cuda.synchronize()
+        base = torch.cuda.memory_allocated()
+        torch.cuda.reset_peak_memory_stats()
+        p = adjoint.AdjointProblem(emu, c, dt=dt, nsteps=20, checkpoint_every=10, segment=seg, precision=prec,
+                                   snapshots=2)
+        need = p.footprint_bytes()
+        p.gradients(p.f0, 1.0)
+        torch.cuda.synchronize()
+        peak = torch.cuda.max_memory_allocated() - base
+        print(f"{prec} {seg}: peak {peak / 2**30:.3f} GiB, footprint {need / 2**30:.3f} GiB")
+        assert peak <= need
+        assert peak >= 0.4 * need
+        del p
+    with pytest.raises(capi.EmuError):
+        adjoint.AdjointProblem(emu, c, dt=dt, nsteps=20, checkpoint_every=10, segment="binomial", precision=prec,
+                               memory_budget=0.5 * need)
