@@ -441,7 +441,11 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_cell_deposit(AdjInT<T> in, 
                                                                   const T* mu, double* grid)
 {
     constexpr int nh = NH<NF>, QG = 4 * nh;
-    constexpr int NT = WX * WY * WZ, MT = (NT + 15) / 16, NTL = (QG + 15) / 16;
+    constexpr int NT = WX * WY * WZ, MT = (NT + 15) / 16;
+    // full 16-column tiles (f64 16x16x4), then the last columns (NF = 3: 4 of
+    // 36) as 4-column tiles of the four-block f64 4x4x4 form: 16 cycles in
+    // place of a 64-cycle 16x16x4 step over 12 padding columns
+    constexpr int NTF = QG / 16, NSM = (QG - 16 * NTF + 3) / 4;
     constexpr int NSR = nh + 4;            // q (or mu), v (or -v), 1
     constexpr int REG = 64 * 9 + 64 * NSR; // doubles per wave: weights, sources
     __shared__ double lds[kCellWaves * REG];
@@ -465,19 +469,26 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_cell_deposit(AdjInT<T> in, 
         tdx[m][1] = 3 + (WY == 1 ? 1 : jj);
         tdx[m][2] = 6 + (WZ == 1 ? 1 : kk);
     }
-    int bi1[NTL], bi2[NTL];
+    // B column q (of a 16-column tile: 16n + l%16; of a 4-column one:
+    // 16 NTF + 4k + l%4) is source c times multiplier f, q = f nh + c
+    constexpr int NCG = NTF + NSM;
+    int bi1[NCG], bi2[NCG];
 #pragma unroll
-    for (int n = 0; n < NTL; n++) {
-        const int q = 16 * n + (lane & 15);
+    for (int n = 0; n < NCG; n++) {
+        const int q = n < NTF ? 16 * n + (lane & 15) : 16 * NTF + 4 * (n - NTF) + (lane & 3);
         const int f = q / nh, c = q % nh;
         bi1[n] = q < QG ? c : nh + 3;
         bi2[n] = q < QG ? (f == 0 ? nh + 3 : nh + f - 1) : nh + 3;
     }
-    mf_d4 acc[MT][NTL];
+    mf_d4 acc[MT][NTF];
+    double accs[MT][NSM > 0 ? NSM : 1];
 #pragma unroll
-    for (int m = 0; m < MT; m++)
+    for (int m = 0; m < MT; m++) {
 #pragma unroll
-        for (int n = 0; n < NTL; n++) acc[m][n] = mf_d4{0.0, 0.0, 0.0, 0.0};
+        for (int n = 0; n < NTF; n++) acc[m][n] = mf_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int k = 0; k < NSM; k++) accs[m][k] = 0.0;
+    }
     for (int base = beg; base < end; base += 64) {
         // stage the chunk: every lane a valid particle, the inactive ones with zero sources
         const bool act = base + lane < end;
@@ -514,19 +525,34 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_cell_deposit(AdjInT<T> in, 
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const int nks = (min(64, end - base) + 3) >> 2; // k-steps of 4 particles
-        for (int ks = 0; ks < nks; ks++) {
-            const int pp = 4 * ks + (lane >> 4);
-            const double* wf = Wf + pp * 9;
-            const double* sr = Sr + pp * NSR;
-            double A[MT];
+        // k-steps of 4 particles, two per iteration (the second at immediate
+        // offsets from the same per-lane pointers); an odd count runs one more
+        // over rows with zero sources, which add nothing
+        int nks = (min(64, end - base) + 3) >> 2;
+        nks += nks & 1;
+        const double* wl = Wf + (lane >> 4) * 9;
+        const double* sl = Sr + (lane >> 4) * NSR;
+        for (int ks = 0; ks < nks; ks += 2) {
 #pragma unroll
-            for (int m = 0; m < MT; m++) A[m] = wf[tdx[m][0]] * wf[tdx[m][1]] * wf[tdx[m][2]];
+            for (int h = 0; h < 2; h++) {
+                const double* wf = wl + (ks + h) * 4 * 9;
+                const double* sr = sl + (ks + h) * 4 * NSR;
+                double A[MT];
 #pragma unroll
-            for (int n = 0; n < NTL; n++) {
-                const double B = sr[bi1[n]] * sr[bi2[n]];
+                for (int m = 0; m < MT; m++) A[m] = wf[tdx[m][0]] * wf[tdx[m][1]] * wf[tdx[m][2]];
 #pragma unroll
-                for (int m = 0; m < MT; m++) acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[m], B, acc[m][n], 0, 0, 0);
+                for (int n = 0; n < NCG; n++) {
+                    const double B = sr[bi1[n]] * sr[bi2[n]];
+                    if (n < NTF) {
+#pragma unroll
+                        for (int m = 0; m < MT; m++)
+                            acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[m], B, acc[m][n], 0, 0, 0);
+                    } else {
+#pragma unroll
+                        for (int m = 0; m < MT; m++)
+                            accs[m][n - NTF] = __builtin_amdgcn_mfma_f64_4x4x4f64(A[m], B, accs[m][n - NTF], 0, 0, 0);
+                    }
+                }
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -552,10 +578,20 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_cell_deposit(AdjInT<T> in, 
             const int tz = WZ == 1 ? kz : wrap(kz + kk - 1, g.n[2]);
             double* G = grid + (((int64_t)tz * g.n[1] + ty) * g.n[0] + tx) * QG;
 #pragma unroll
-            for (int n = 0; n < NTL; n++) {
-                const int q = 16 * n + (lane & 15);
-                if (q < QG) unsafeAtomicAdd(&G[q], acc[m][n][r]);
-            }
+            for (int n = 0; n < NTF; n++) unsafeAtomicAdd(&G[16 * n + (lane & 15)], acc[m][n][r]);
+        }
+    // 4x4x4 tiles: element of lane l is D[16m + 4((l >> 2) & 3) + l/16][16 NTF + 4k + l%4]
+#pragma unroll
+    for (int m = 0; m < MT; m++)
+#pragma unroll
+        for (int k = 0; k < NSM; k++) {
+            const int t = 16 * m + 4 * ((lane >> 2) & 3) + (lane >> 4), q = 16 * NTF + 4 * k + (lane & 3);
+            if (t >= NT || q >= QG) continue;
+            const int ii = t % WX, jj = (t / WX) % WY, kk = t / (WX * WY);
+            const int tx = WX == 1 ? kx : wrap(kx + ii - 1, g.n[0]);
+            const int ty = WY == 1 ? ky : wrap(ky + jj - 1, g.n[1]);
+            const int tz = WZ == 1 ? kz : wrap(kz + kk - 1, g.n[2]);
+            unsafeAtomicAdd(&grid[(((int64_t)tz * g.n[1] + ty) * g.n[0] + tx) * QG + q], accs[m][k]);
         }
 }
 
@@ -656,11 +692,15 @@ __device__ __forceinline__ void cell_hamiltonians(const AdjInT<T>& in, int64_t p
 // adds D to the grid with one atomic per value.  An order-0 dimension's
 // weights are (0, 1, 0), so its other targets receive zeros.
 template <int NF> struct WaveDeposit {
-    static constexpr int nh = NH<NF>, QG = 4 * nh, NT = 27, MT = 2, NTL = (QG + 15) / 16;
+    static constexpr int nh = NH<NF>, QG = 4 * nh, NT = 27, MT = 2;
+    // full 16-column tiles, then the last QG % 16 columns as 4-column tiles of
+    // the four-block f64 4x4x4 form (as k_cell_deposit)
+    static constexpr int NTF = QG / 16, NSM = (QG - 16 * NTF + 3) / 4, NCG = NTF + NSM;
     static constexpr int NSR = nh + 4;            // the source row, s, 1
     static constexpr int REG = 64 * 9 + 64 * NSR; // LDS doubles per wave
-    mf_d4 acc[MT][NTL];
-    int tdx[MT][3], bi1[NTL], bi2[NTL];
+    mf_d4 acc[MT][NTF];
+    double accs[MT][NSM > 0 ? NSM : 1];
+    int tdx[MT][3], bi1[NCG], bi2[NCG];
 
     __device__ __forceinline__ void init(int lane)
     {
@@ -673,19 +713,23 @@ template <int NF> struct WaveDeposit {
             tdx[m][2] = 6 + tc / 9;
         }
 #pragma unroll
-        for (int n = 0; n < NTL; n++) {
-            const int q = 16 * n + (lane & 15);
+        for (int n = 0; n < NCG; n++) {
+            const int q = n < NTF ? 16 * n + (lane & 15) : 16 * NTF + 4 * (n - NTF) + (lane & 3);
             const int f = q / nh, c = q % nh;
             bi1[n] = q < QG ? c : nh + 3;
             bi2[n] = q < QG ? (f == 0 ? nh + 3 : nh + f - 1) : nh + 3;
         }
 #pragma unroll
-        for (int m = 0; m < MT; m++)
+        for (int m = 0; m < MT; m++) {
 #pragma unroll
-            for (int n = 0; n < NTL; n++) acc[m][n] = mf_d4{0.0, 0.0, 0.0, 0.0};
+            for (int n = 0; n < NTF; n++) acc[m][n] = mf_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int k = 0; k < NSM; k++) accs[m][k] = 0.0;
+        }
     }
 
-    // every lane of the wave calls this (count: the chunk's particles)
+    // every lane of the wave calls this (count: the chunk's particles; the
+    // rows of lanes past it must hold zero sources)
     template <typename T>
     __device__ __forceinline__ void add(double* Wf, double* Sr, int lane, const double wd[3][3], const T* src,
                                         const T sv[3], int count)
@@ -702,20 +746,33 @@ template <int NF> struct WaveDeposit {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const int nks = (min(64, count) + 3) >> 2;
-        for (int ks = 0; ks < nks; ks++) {
-            const int pp = 4 * ks + (lane >> 4);
-            const double* wf = Wf + pp * 9;
-            const double* sr = Sr + pp * NSR;
-            double A[MT];
+        // two k-steps of 4 particles per iteration, the second at immediate
+        // offsets; an odd count runs one more over zero-source rows
+        int nks = (min(64, count) + 3) >> 2;
+        nks += nks & 1;
+        const double* wl = Wf + (lane >> 4) * 9;
+        const double* sl = Sr + (lane >> 4) * NSR;
+        for (int ks = 0; ks < nks; ks += 2) {
 #pragma unroll
-            for (int m = 0; m < MT; m++) A[m] = wf[tdx[m][0]] * wf[tdx[m][1]] * wf[tdx[m][2]];
+            for (int h = 0; h < 2; h++) {
+                const double* wf = wl + (ks + h) * 4 * 9;
+                const double* sr = sl + (ks + h) * 4 * NSR;
+                double A[MT];
 #pragma unroll
-            for (int n = 0; n < NTL; n++) {
-                const double B = sr[bi1[n]] * sr[bi2[n]];
+                for (int m = 0; m < MT; m++) A[m] = wf[tdx[m][0]] * wf[tdx[m][1]] * wf[tdx[m][2]];
 #pragma unroll
-                for (int m = 0; m < MT; m++)
-                    acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[m], B, acc[m][n], 0, 0, 0);
+                for (int n = 0; n < NCG; n++) {
+                    const double B = sr[bi1[n]] * sr[bi2[n]];
+                    if (n < NTF) {
+#pragma unroll
+                        for (int m = 0; m < MT; m++)
+                            acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[m], B, acc[m][n], 0, 0, 0);
+                    } else {
+#pragma unroll
+                        for (int m = 0; m < MT; m++)
+                            accs[m][n - NTF] = __builtin_amdgcn_mfma_f64_4x4x4f64(A[m], B, accs[m][n - NTF], 0, 0, 0);
+                    }
+                }
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -725,26 +782,35 @@ template <int NF> struct WaveDeposit {
 
     __device__ __forceinline__ void flush(double* grid, const Geom& g, int cell, int lane)
     {
-        // element r of lane l holds D[16m + 4r + l/16][16n + l%16]
         const int kx = __builtin_amdgcn_readfirstlane(cell % g.n[0]);
         const int ky = __builtin_amdgcn_readfirstlane((cell / g.n[0]) % g.n[1]);
         const int kz = __builtin_amdgcn_readfirstlane(cell / (g.n[0] * g.n[1]));
         auto wrap = [](int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); };
+        auto target = [&](int t) {
+            const int tx = wrap(kx + t % 3 - 1, g.n[0]);
+            const int ty = wrap(ky + (t / 3) % 3 - 1, g.n[1]);
+            const int tz = wrap(kz + t / 9 - 1, g.n[2]);
+            return grid + (((int64_t)tz * g.n[1] + ty) * g.n[0] + tx) * QG;
+        };
+        // element r of lane l holds D[16m + 4r + l/16][16n + l%16]
 #pragma unroll
         for (int m = 0; m < MT; m++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int t = 16 * m + 4 * r + (lane >> 4);
                 if (t >= NT) continue;
-                const int tx = wrap(kx + t % 3 - 1, g.n[0]);
-                const int ty = wrap(ky + (t / 3) % 3 - 1, g.n[1]);
-                const int tz = wrap(kz + t / 9 - 1, g.n[2]);
-                double* Gd = grid + (((int64_t)tz * g.n[1] + ty) * g.n[0] + tx) * QG;
+                double* Gd = target(t);
 #pragma unroll
-                for (int n = 0; n < NTL; n++) {
-                    const int q = 16 * n + (lane & 15);
-                    if (q < QG) unsafeAtomicAdd(&Gd[q], acc[m][n][r]);
-                }
+                for (int n = 0; n < NTF; n++) unsafeAtomicAdd(&Gd[16 * n + (lane & 15)], acc[m][n][r]);
+            }
+        // 4x4x4 tiles: element of lane l is D[16m + 4((l >> 2) & 3) + l/16][16 NTF + 4k + l%4]
+#pragma unroll
+        for (int m = 0; m < MT; m++)
+#pragma unroll
+            for (int k = 0; k < NSM; k++) {
+                const int t = 16 * m + 4 * ((lane >> 2) & 3) + (lane >> 4), q = 16 * NTF + 4 * k + (lane & 3);
+                if (t >= NT || q >= QG) continue;
+                unsafeAtomicAdd(&target(t)[q], accs[m][k]);
             }
     }
 };
