@@ -299,6 +299,26 @@ def run_adjoint(args):
     print(json.dumps(out), flush=True)
 
 
+def time_per_function(core, emu, parts, steps, warmup):
+    """ms of `steps` steps of the per-function FP64 path (core.Integrator),
+    after `warmup` untimed ones, from the given particles (which the
+    integrator's double buffering overwrites)."""
+    import torch
+    integ = core.Integrator(emu, parts)
+    integ.integrate(warmup)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    integ.integrate(steps)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1)
+    del integ
+    torch.cuda.empty_cache()
+    return ms
+
+
 def run_plain(args):
     """The per-function drop-in path (core.Integrator: deposit_to_mesh,
     FillBoundary, copyParticles, interpolate_rhs_from_mesh and the RK4
@@ -318,16 +338,7 @@ def run_plain(args):
         parts.pos = parts.pos[:, perm].contiguous()
         parts.rdata = parts.rdata[:, perm].contiguous()
         parts.ids = parts.ids[perm].contiguous()
-    integ = core.Integrator(emu, parts)
-    integ.integrate(args.warmup)
-    torch.cuda.synchronize()
-    stream = torch.cuda.current_stream()
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    ev0.record(stream)
-    integ.integrate(args.steps)
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    ms = ev0.elapsed_time(ev1)
+    ms = time_per_function(core, emu, parts, args.steps, args.warmup)
     ncell = parms.ncell[0] * parms.ncell[1] * parms.ncell[2]
     ppc = parts.np / ncell
     bpu = algorithmic_bytes(parms.nf, ppc, "fp64")
@@ -355,6 +366,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="ff64")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-per-function", action="store_true",
+                    help="skip timing the per-function FP64 path beside the fused steps")
     ap.add_argument("--no-precision-check", action="store_true",
                     help="skip the FP32-bulk vs FP64 run through saturation on the 1-D fiducial workload")
     ap.add_argument("--precision", default="both", choices=["both", "fp64", "fp32"])
@@ -457,6 +470,16 @@ def main():
                                "growth_rate_fp32": g32, "growth_rate_fp64": g64,
                                "window_s_fp32": res["fp32"]["time_window"], "window_s_fp64": res["fp64"]["time_window"],
                                "amplitude_fp32": res["fp32"]["amp"], "amplitude_fp64": res["fp64"]["amp"]}
+    if "fp32" in res and not args.no_per_function:
+        # the FP32-bulk step against the per-function FP64 drop-in path (the
+        # reference's call structure through the C-ABI) on the same particles
+        pf_steps = 3
+        ms_pf = time_per_function(core, emu, parts, pf_steps, 1) / pf_steps
+        out["fp32_vs_per_function_fp64"] = {
+            "ms_per_step_per_function": ms_pf, "steps": pf_steps,
+            "speedup": ms_pf / res["fp32"]["ms_per_step"],
+            "path": "core.Integrator: deposit_to_mesh, FillBoundary, copyParticles, interpolate_rhs_from_mesh and "
+                    "the RK4 Copy/Saxpy calls of main.cpp, FP64"}
     out["kernels_ms"] = {p: {k: res[p][k] for k in ("stage_ms", "fill_ms", "sort_ms", "dt_ms")} for p in res}
     # the same algorithmic bytes over the whole step's wall time (every launch)
     out["roofline"]["wall_frac"] = parts.np * bpu / (r["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS
