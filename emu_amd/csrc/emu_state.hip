@@ -513,6 +513,10 @@ struct emu_state {
     int profile = 0;
     struct Mark { int k; hipEvent_t a, b; };
     std::vector<Mark> marks;
+    // the sort plan runs on a stream of its own, beside stages 2 and 3: it
+    // needs stage 1's end-of-step cells, and only stage 4 reads it
+    hipStream_t side = nullptr;
+    hipEvent_t ev_stage1 = nullptr, ev_plan = nullptr;
 };
 
 namespace {
@@ -557,14 +561,15 @@ int ensure_cub(emu_state* st, size_t need)
     return EMU_OK;
 }
 
-// out = exclusive scan of in (ncell + 1 entries, in[ncell] = 0)
-int scan_counts(emu_state* st, const int* in, int* out)
+// out = exclusive scan of in (ncell + 1 entries, in[ncell] = 0), on stream s
+// (the context's when null)
+int scan_counts(emu_state* st, const int* in, int* out, hipStream_t s = nullptr)
 {
+    if (!s) s = st->ctx->stream;
     size_t need = 0;
-    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, need, in, out, (int)(st->ncell + 1), st->ctx->stream);
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, need, in, out, (int)(st->ncell + 1), s);
     if (ensure_cub(st, need)) return set_error(st->ctx, EMU_ERR_NOMEM, "scan scratch");
-    EMU_HIP(st->ctx, hipcub::DeviceScan::ExclusiveSum(st->cub_tmp, st->cub_bytes, in, out, (int)(st->ncell + 1),
-                                                     st->ctx->stream));
+    EMU_HIP(st->ctx, hipcub::DeviceScan::ExclusiveSum(st->cub_tmp, st->cub_bytes, in, out, (int)(st->ncell + 1), s));
     return EMU_OK;
 }
 
@@ -706,12 +711,12 @@ int dt_from_grid(emu_state* st, const double* g, double fixed_dt)
 }
 
 // Kernel classes: 0 stage kernels, 1 grid fills, 2 sort, 3 dt reduction.
-void ev_record(emu_state* st, int k, bool start)
+void ev_record(emu_state* st, int k, bool start, hipStream_t stream = nullptr)
 {
     if (!st->profile) return;
     hipEvent_t e;
     (void)hipEventCreate(&e);
-    (void)hipEventRecord(e, st->ctx->stream);
+    (void)hipEventRecord(e, stream ? stream : st->ctx->stream);
     if (start) st->marks.push_back({k, e, nullptr});
     else st->marks.back().b = e;
 }
@@ -768,6 +773,10 @@ extern "C" int emu_state_create(emu_ctx* ctx, int64_t np, int precision, emu_sta
               hipMalloc(&st->scal, 4 * sizeof(double)) == hipSuccess &&
               hipMalloc(&st->steps, sizeof(int64_t)) == hipSuccess &&
               hipMalloc(&st->red, 8 * sizeof(unsigned long long)) == hipSuccess;
+    // default priority, and no implicit synchronisation with the legacy stream
+    ok = ok && hipStreamCreateWithFlags(&st->side, hipStreamNonBlocking) == hipSuccess &&
+         hipEventCreateWithFlags(&st->ev_stage1, hipEventDisableTiming) == hipSuccess &&
+         hipEventCreateWithFlags(&st->ev_plan, hipEventDisableTiming) == hipSuccess;
     if (!ok) {
         emu_state_destroy(st);
         return set_error(ctx, EMU_ERR_NOMEM, "state allocation failed");
@@ -789,6 +798,12 @@ extern "C" int emu_state_destroy(emu_state* st)
         (void)hipEventDestroy(m.a);
         if (m.b) (void)hipEventDestroy(m.b);
     }
+    if (st->side) {
+        (void)hipStreamSynchronize(st->side);
+        (void)hipStreamDestroy(st->side);
+    }
+    if (st->ev_stage1) (void)hipEventDestroy(st->ev_stage1);
+    if (st->ev_plan) (void)hipEventDestroy(st->ev_plan);
     delete st;
     return EMU_OK;
 }
@@ -984,17 +999,23 @@ extern "C" int emu_state_evolve(emu_state* st, int nsteps, double fixed_dt)
         ev_record(st, 0, true);
         launch_stage(st, 1, A, B, false);   // also bins the end-of-step positions
         ev_record(st, 0, false);
-        // stable counting-sort plan by the new cell
-        ev_record(st, 2, true);
-        hipLaunchKernelGGL(k_move_ranks, dim3((unsigned)st->ncell), dim3(64), 0, s, st->newcell, st->cell_start,
+        // stable counting-sort plan by the new cell, on the side stream while
+        // stages 2 and 3 run (it reads stage 1's cells and the current run
+        // table; it writes only the plan, which stage 4 waits for)
+        hipStream_t ss = st->side;
+        EMU_HIP(ctx, hipEventRecord(st->ev_stage1, s));
+        EMU_HIP(ctx, hipStreamWaitEvent(ss, st->ev_stage1, 0));
+        ev_record(st, 2, true, ss);
+        hipLaunchKernelGGL(k_move_ranks, dim3((unsigned)st->ncell), dim3(64), 0, ss, st->newcell, st->cell_start,
                            st->cell_cnt, current_ids(st), ctx->g, st->rank, st->mv, st->mcount, st->red + 2);
-        hipLaunchKernelGGL(k_new_counts, dim3(grid_blocks(st->ncell + 1)), dim3(256), 0, s, st->mcount, ctx->g,
+        hipLaunchKernelGGL(k_new_counts, dim3(grid_blocks(st->ncell + 1)), dim3(256), 0, ss, st->mcount, ctx->g,
                            st->ncell, st->align, st->counts, st->new_cnt);
-        int e = scan_counts(st, st->counts, st->new_start);
+        int e = scan_counts(st, st->counts, st->new_start, ss);
         if (e) return e;
-        hipLaunchKernelGGL(k_bases, dim3(grid_blocks(st->ncell)), dim3(256), 0, s, st->mcount, st->new_start, ctx->g,
+        hipLaunchKernelGGL(k_bases, dim3(grid_blocks(st->ncell)), dim3(256), 0, ss, st->mcount, st->new_start, ctx->g,
                            st->ncell, st->mbase);
-        ev_record(st, 2, false);
+        ev_record(st, 2, false, ss);
+        EMU_HIP(ctx, hipEventRecord(st->ev_plan, ss));
         ev_record(st, 1, true);
         fill_grid(st, A, false);
         ev_record(st, 1, false);
@@ -1011,6 +1032,7 @@ extern "C" int emu_state_evolve(emu_state* st, int nsteps, double fixed_dt)
         e = dt_from_grid(st, B, fixed_dt);
         ev_record(st, 3, false);
         if (e) return e;
+        EMU_HIP(ctx, hipStreamWaitEvent(s, st->ev_plan, 0));
         ev_record(st, 0, true);
         launch_stage(st, 4, B, nullptr, false); // writes y_{n+1} into the other copy, new order
         ev_record(st, 0, false);
