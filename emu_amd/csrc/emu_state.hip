@@ -513,10 +513,11 @@ struct emu_state {
     int profile = 0;
     struct Mark { int k; hipEvent_t a, b; };
     std::vector<Mark> marks;
-    // the sort plan runs on a stream of its own, beside stages 2 and 3: it
-    // needs stage 1's end-of-step cells, and only stage 4 reads it
+    // the sort plan runs on a stream of its own, beside stages 2 and 3 (it
+    // needs stage 1's end-of-step cells, and only stage 4 reads it), and so
+    // does compute_dt, beside stage 4 and the step-end deposit
     hipStream_t side = nullptr;
-    hipEvent_t ev_stage1 = nullptr, ev_plan = nullptr;
+    hipEvent_t ev_stage1 = nullptr, ev_plan = nullptr, ev_stage3 = nullptr, ev_dt = nullptr;
 };
 
 namespace {
@@ -681,19 +682,20 @@ void fill_grid(emu_state* st, double* g, bool with_aux)
                        st->ncell, st->QG, st->m00 * st->nscale, with_aux ? st->aux : nullptr, st->naux);
 }
 
-int dt_from_grid(emu_state* st, const double* g, double fixed_dt)
+int dt_from_grid(emu_state* st, const double* g, double fixed_dt, hipStream_t s = nullptr)
 {
     emu_ctx* ctx = st->ctx;
     const emu_params& p = ctx->p;
-    hipLaunchKernelGGL(k_fill_u64v, dim3(1), dim3(64), 0, ctx->stream, st->red, 2, 0ull);
+    if (!s) s = ctx->stream;
+    hipLaunchKernelGGL(k_fill_u64v, dim3(1), dim3(64), 0, s, st->red, 2, 0ull);
     const bool use_flavor = p.flavor_cfl_factor > 0.0;
     const int adaptive = p.max_adaptive_speedup > 1 ? 1 : 0;
     if (use_flavor && fixed_dt <= 0) {
         if (st->nf == 2)
-            hipLaunchKernelGGL(k_dt_reduce<2>, dim3(grid_blocks(st->ncell)), dim3(256), 0, ctx->stream, g, st->aux,
+            hipLaunchKernelGGL(k_dt_reduce<2>, dim3(grid_blocks(st->ncell)), dim3(256), 0, s, g, st->aux,
                                st->ncell, ctx->g.cell_volume * st->nscale, st->m00 * st->nscale, adaptive, st->red);
         else
-            hipLaunchKernelGGL(k_dt_reduce<3>, dim3(grid_blocks(st->ncell)), dim3(256), 0, ctx->stream, g, st->aux,
+            hipLaunchKernelGGL(k_dt_reduce<3>, dim3(grid_blocks(st->ncell)), dim3(256), 0, s, g, st->aux,
                                st->ncell, ctx->g.cell_volume * st->nscale, st->m00 * st->nscale, adaptive, st->red);
     }
     DtParams dp;
@@ -705,7 +707,7 @@ int dt_from_grid(emu_state* st, const double* g, double fixed_dt)
     dp.Vvac_max = st->Vvac_max;
     dp.fixed_dt = fixed_dt;
     dp.use_flavor = use_flavor;
-    hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(64), 0, ctx->stream, st->red, dp, st->scal + 2, st->red + 2);
+    hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(64), 0, s, st->red, dp, st->scal + 2, st->red + 2);
     EMU_LAUNCH_CHECK(ctx, "compute_dt");
     return EMU_OK;
 }
@@ -776,7 +778,9 @@ extern "C" int emu_state_create(emu_ctx* ctx, int64_t np, int precision, emu_sta
     // default priority, and no implicit synchronisation with the legacy stream
     ok = ok && hipStreamCreateWithFlags(&st->side, hipStreamNonBlocking) == hipSuccess &&
          hipEventCreateWithFlags(&st->ev_stage1, hipEventDisableTiming) == hipSuccess &&
-         hipEventCreateWithFlags(&st->ev_plan, hipEventDisableTiming) == hipSuccess;
+         hipEventCreateWithFlags(&st->ev_plan, hipEventDisableTiming) == hipSuccess &&
+         hipEventCreateWithFlags(&st->ev_stage3, hipEventDisableTiming) == hipSuccess &&
+         hipEventCreateWithFlags(&st->ev_dt, hipEventDisableTiming) == hipSuccess;
     if (!ok) {
         emu_state_destroy(st);
         return set_error(ctx, EMU_ERR_NOMEM, "state allocation failed");
@@ -802,8 +806,8 @@ extern "C" int emu_state_destroy(emu_state* st)
         (void)hipStreamSynchronize(st->side);
         (void)hipStreamDestroy(st->side);
     }
-    if (st->ev_stage1) (void)hipEventDestroy(st->ev_stage1);
-    if (st->ev_plan) (void)hipEventDestroy(st->ev_plan);
+    for (hipEvent_t ev : {st->ev_stage1, st->ev_plan, st->ev_stage3, st->ev_dt})
+        if (ev) (void)hipEventDestroy(ev);
     delete st;
     return EMU_OK;
 }
@@ -1028,10 +1032,15 @@ extern "C" int emu_state_evolve(emu_state* st, int nsteps, double fixed_dt)
         ev_record(st, 0, true);
         launch_stage(st, 3, A, B, want_aux);
         ev_record(st, 0, false);
-        ev_record(st, 3, true);
-        e = dt_from_grid(st, B, fixed_dt);
-        ev_record(st, 3, false);
+        // compute_dt from stage 3's deposit, also on the side stream: it reads B
+        // beside stage 4 and writes the next dt, which k_step_end waits for
+        EMU_HIP(ctx, hipEventRecord(st->ev_stage3, s));
+        EMU_HIP(ctx, hipStreamWaitEvent(ss, st->ev_stage3, 0));
+        ev_record(st, 3, true, ss);
+        e = dt_from_grid(st, B, fixed_dt, ss);
+        ev_record(st, 3, false, ss);
         if (e) return e;
+        EMU_HIP(ctx, hipEventRecord(st->ev_dt, ss));
         EMU_HIP(ctx, hipStreamWaitEvent(s, st->ev_plan, 0));
         ev_record(st, 0, true);
         launch_stage(st, 4, B, nullptr, false); // writes y_{n+1} into the other copy, new order
@@ -1046,6 +1055,7 @@ extern "C" int emu_state_evolve(emu_state* st, int nsteps, double fixed_dt)
         ev_record(st, 0, true);
         launch_stage(st, 0, nullptr, A, false);
         ev_record(st, 0, false);
+        EMU_HIP(ctx, hipStreamWaitEvent(s, st->ev_dt, 0));
         hipLaunchKernelGGL(k_step_end, dim3(1), dim3(64), 0, s, st->scal, st->scal + 1, st->scal + 2, st->steps);
         if (fixed_dt > 0) EMU_HIP(ctx, hipMemcpyAsync(st->scal + 1, &fixed_dt, sizeof(double), hipMemcpyHostToDevice, s));
         EMU_LAUNCH_CHECK(ctx, "state_evolve");
