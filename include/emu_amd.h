@@ -183,7 +183,10 @@ int emu_state_set_vvac_max(emu_state* st, double vvac_max);
 /* Write the state back (sorted order) with each particle's id.  Synchronous. */
 int emu_state_store(emu_state* st, emu_particles* parts, int64_t* ids);
 /* Advance nsteps RK4 steps (asynchronous; dt stays on the device between
- * steps).  A fixed_dt > 0 replaces compute_dt's value for every step. */
+ * steps).  A fixed_dt > 0 replaces compute_dt's value for every step.  Each
+ * step's sort plan and compute_dt run on a second stream that the state owns,
+ * forked from and joined back into the context's stream with events, so all of
+ * a call's work is ordered before later work on the context's stream. */
 int emu_state_evolve(emu_state* st, int nsteps, double fixed_dt);
 /* Time, last dt, steps taken and the deferred status of the steps since load:
  * EMU_ERR_TRACE/NEGDIAG/LENGTH (Renormalize) or EMU_ERR_MOVE (a particle that
