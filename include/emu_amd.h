@@ -193,9 +193,9 @@ int emu_state_evolve(emu_state* st, int nsteps, double fixed_dt);
  * moved more than one cell, which only |p| > p_t allows), with *bad_index = the
  * lowest offending particle id; EMU_ERR_DT with *bad_index = -1.  Synchronous. */
 int emu_state_info(emu_state* st, double* time, double* dt, int64_t* steps, int64_t* bad_index);
-/* Per-launch timing on the context stream: enable = 1 records a HIP event pair
- * around every launch of the following emu_state_evolve calls (and clears
- * earlier records). */
+/* Per-launch timing: enable = 1 records a HIP event pair around every launch
+ * of the following emu_state_evolve calls, on the stream it runs on (and
+ * clears earlier records). */
 int emu_state_profile(emu_state* st, int enable);
 /* Average milliseconds per launch of kernel class k (0 = stage kernels,
  * 1 = grid fills, 2 = cell sort, 3 = dt reduction) since profiling was
