@@ -782,16 +782,19 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         // ---- D += W^T R, four particles per MFMA
-        // two k-steps (8 particles) per iteration, so the operand reads of one
-        // overlap the matrix work of the other; rows past the chunk's particles
-        // hold zero sources (inactive lanes), so rounding the count up is exact
-        // (the deposit-only launch keeps one k-step per iteration: there the
-        // registers of the second would cost a wave of occupancy).  Each operand
-        // has one per-lane LDS pointer that advances by a whole iteration; the
-        // second k-step is an immediate offset.  B is the product in T: in
-        // FP32-bulk the flux sources q v are FP32 values like q itself, and the
-        // sum over particles stays FP64.
-        constexpr int KU = s == 0 ? 1 : 2;
+        // KU k-steps (4 KU particles) per iteration, so the operand reads of one
+        // overlap the matrix work of the others: two, and four in FP32-bulk
+        // stages 1 and 2, whose registers have room (one tile of 16 slots an
+        // iteration, so the second cell's tiles still begin at an iteration);
+        // rows past the chunk's particles hold zero sources (inactive lanes), so
+        // rounding the count up is exact.  The deposit-only launch keeps one
+        // k-step per iteration: there the registers of the second cost a wave of
+        // occupancy, and measured slower.  Each operand has one per-lane LDS
+        // pointer that advances by a whole iteration; the later k-steps are
+        // immediate offsets.  B is the product in T: in FP32-bulk the flux
+        // sources q v are FP32 values like q itself, and the sum over particles
+        // stays FP64.
+        constexpr int KU = s == 0 ? 1 : (sizeof(T) == 4 && (s == 1 || s == 2) ? 4 : 2);
         // (32-bit LDS pointers made opaque once, so that the wave's base is not
         // split off again and re-added to every address in the loop)
         typedef __attribute__((address_space(3))) const double* lds_d;
