@@ -784,8 +784,8 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
         // ---- D += W^T R, four particles per MFMA
         // KU k-steps (4 KU particles) per iteration, so the operand reads of one
         // overlap the matrix work of the others: two, and four in FP32-bulk
-        // stages 1 and 2, whose registers have room (one tile of 16 slots an
-        // iteration, so the second cell's tiles still begin at an iteration);
+        // stages 1 to 3 (one tile of 16 slots an iteration, so the second
+        // cell's tiles still begin at an iteration; FP64 measured no faster);
         // rows past the chunk's particles hold zero sources (inactive lanes), so
         // rounding the count up is exact.  The deposit-only launch keeps one
         // k-step per iteration: there the registers of the second cost a wave of
@@ -794,7 +794,7 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
         // immediate offsets.  B is the product in T: in FP32-bulk the flux
         // sources q v are FP32 values like q itself, and the sum over particles
         // stays FP64.
-        constexpr int KU = s == 0 ? 1 : (sizeof(T) == 4 && (s == 1 || s == 2) ? 4 : 2);
+        constexpr int KU = s == 0 ? 1 : (sizeof(T) == 4 ? 4 : 2);
         // (32-bit LDS pointers made opaque once, so that the wave's base is not
         // split off again and re-added to every address in the loop)
         typedef __attribute__((address_space(3))) const double* lds_d;
