@@ -596,6 +596,9 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
                     const T v0 = vp[0], v1 = vp[1], v2 = vp[2];
                     const T sg = g4 == 0 ? T(1) : (g4 == 1 ? -v0 : (g4 == 2 ? -v1 : -v2));
                     // A = (w_x s_g) (w_y w_z): 3 + WY WZ products, then one per target
+                    // (FP32-bulk forms w_y w_z in FP32 from the rounded factors, one
+                    // conversion per factor instead of one per product: the A
+                    // operand is FP32 either way)
                     T ax[WX], ayz[WY * WZ];
 #pragma unroll
                     for (int ii = 0; ii < WX; ii++) ax[ii] = T(w3[WX == 1 ? 1 : ii]) * sg;
@@ -603,7 +606,9 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(sizeof
                     for (int kk = 0; kk < WZ; kk++)
 #pragma unroll
                         for (int jj = 0; jj < WY; jj++)
-                            ayz[kk * WY + jj] = T(w3[3 + (WY == 1 ? 1 : jj)] * w3[6 + (WZ == 1 ? 1 : kk)]);
+                            ayz[kk * WY + jj] = sizeof(T) == 4
+                                                     ? T(w3[3 + (WY == 1 ? 1 : jj)]) * T(w3[6 + (WZ == 1 ? 1 : kk)])
+                                                     : T(w3[3 + (WY == 1 ? 1 : jj)] * w3[6 + (WZ == 1 ? 1 : kk)]);
                     if constexpr (sizeof(T) == 8) {
                         // FP64: the NH result columns in blocks of 4 on the 4-block
                         // 4x4x4 form (16 cycles each): 3 blocks instead of one 16-column
