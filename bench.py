@@ -177,7 +177,9 @@ def precision_check(core, configs, nsteps=2000):
 
 def cpu_baseline(parms_dict, budget_s=15.0):
     """FP64 oracle, OpenMP over this host's cores, on a 16^3-cell box of the
-    same physics (same cell size, directions and densities)."""
+    same physics (same cell size, directions and densities), for up to 10
+    steps or the time budget.  For the cpu16 workload (16^3 cells, 10 steps)
+    that box is the workload itself, run in full."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
     ncores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
@@ -199,7 +201,7 @@ def cpu_baseline(parms_dict, budget_s=15.0):
         err, dt, _ = O.rk4_step(deck.nf, prm, parts, g, dt, vv)
         steps += 1
         el = time.perf_counter() - t0
-        if el > budget_s or steps >= 8:
+        if el > budget_s or steps >= 10:
             break
     cpu_model = ""
     try:
